@@ -1,0 +1,89 @@
+/*
+ * mchip.h -- C ABI of the MI355X (gfx950) codec engine behind the
+ * matter-compressor drop-in library.  Plain pointers and sizes only.
+ *
+ * These entry points are the device half of the boundary: the plain-C host
+ * library (include/matter_compressor_amd.h) implements the reference's own
+ * mc_codec / mc_archive API on top of them, and a caller that keeps chunk
+ * blobs resident in HBM can call them directly.  Each one replaces the work
+ * loop of a reference function (SuperOptimizer/matter-compressor):
+ *
+ *   mchip_decode_chunks      mc_archive_decode_chunk's worker pool
+ *                            (src/mc_archive.c:783-820) over n chunks at once
+ *   mchip_decode_blocks      mc_archive_decode_block / mc_dec_block
+ *                            (src/mc_archive.c:728-762, src/mc_codec.c:739-783)
+ *   mchip_encode_chunks      mc_archive_append_chunk_par's encode + stitch
+ *                            (src/mc_archive.c:822-901) and, with tau > 0,
+ *                            encode_chunk_blob_codec (src/mc_codec.c:809-845)
+ *   mchip_encode_blocks      mc_enc_block (src/mc_codec.c:629-737)
+ *
+ * All functions return 0 on success and a negative code on failure; the
+ * message of the last failure on the calling thread is mchip_last_error().
+ * Functions taking an mchip_ctx enqueue work on that context's HIP stream;
+ * calls on one context are serialised, distinct contexts may run
+ * concurrently.
+ */
+#ifndef MCHIP_H
+#define MCHIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mchip_ctx mchip_ctx;
+
+/* Select the HIP device and upload the constant tables.  Idempotent. */
+int         mchip_init(int device);
+const char *mchip_last_error(void);
+
+/* Per-volume prior override (u16[8][32] q=1 and q=12 endpoint tables);
+ * NULL,NULL restores the trained defaults.  Mirrors mc_codec_set_priors
+ * (src/mc_codec.c:141-154, :533). */
+int         mchip_set_priors(const uint16_t *plo, const uint16_t *phi);
+
+/* A context owns one HIP stream and a growable device workspace. */
+mchip_ctx  *mchip_ctx_create(void);
+void        mchip_ctx_destroy(mchip_ctx *ctx);
+void       *mchip_ctx_stream(mchip_ctx *ctx);        /* the hipStream_t */
+int         mchip_ctx_sync(mchip_ctx *ctx);
+
+/* Device memory helpers (so callers need no HIP headers). */
+void       *mchip_malloc(size_t bytes);
+void        mchip_free(void *dptr);
+int         mchip_h2d(mchip_ctx *ctx, void *dst, const void *src, size_t bytes);
+int         mchip_d2h(mchip_ctx *ctx, void *dst, const void *src, size_t bytes);
+int         mchip_memset(mchip_ctx *ctx, void *dst, int value, size_t bytes);
+
+/* Decode n chunk blobs (format v7) that are resident in device memory.
+ * Blob i lives at d_blobs + blob_off[i] with blob_len[i] readable bytes; q[i]
+ * is the quality stored in its header.  Chunk i is written as a dense 256^3
+ * z,y,x raster at d_out + i*out_stride (absent blocks -> 0).  Asynchronous on
+ * the context's stream.  Host arrays are read before the call returns. */
+int mchip_decode_chunks(mchip_ctx *ctx, const uint8_t *d_blobs, const uint64_t *blob_off,
+                        const uint64_t *blob_len, const float *q, int n,
+                        uint8_t *d_out, uint64_t out_stride);
+
+/* Host-memory convenience: upload n blobs, decode, download the rasters to
+ * out + i*out_stride, synchronise. */
+int mchip_decode_chunks_host(mchip_ctx *ctx, const uint8_t *const *blobs, const uint64_t *blob_len,
+                             int n, uint8_t *out, uint64_t out_stride);
+
+/* Decode n independent block payloads (host memory) at quality q[i] into
+ * out + 4096*i (16^3 z,y,x).  A NULL payload or blob-absent block writes
+ * zeros.  Synchronous. */
+int mchip_decode_blocks_host(mchip_ctx *ctx, const uint8_t *const *payloads, const uint32_t *lens,
+                             const float *q, int n, uint8_t *out);
+
+/* Kernel timing: when enabled, every kernel launched on the context is
+ * bracketed by HIP events on the context's stream; stats accumulate per
+ * kernel name ("index", "entropy", "recon", ...). */
+int mchip_ctx_enable_timing(mchip_ctx *ctx, int on);
+int mchip_ctx_kernel_time(mchip_ctx *ctx, const char *kernel, double *total_ms, long *launches);
+int mchip_ctx_reset_timing(mchip_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,503 @@
+// mchip_api.hip -- C ABI of the HIP codec engine (declared in include/mchip.h):
+// device selection, constant tables, per-quality tables, contexts (stream +
+// growable workspace), launches and kernel timing.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mchip.h"
+#include "mchip_internal.hpp"
+
+using namespace mchip;
+
+namespace {
+
+thread_local std::string t_err;
+
+int fail(const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    t_err = buf;
+    return -1;
+}
+#define HIPCHK(x)                                                                         \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess) return fail("%s failed: %s", #x, hipGetErrorString(e_));   \
+    } while (0)
+
+// Trained prior endpoint tables (q=1, q=12), P(bit==0)/4096, class order
+// sig, mag, eob, mask, mask-u, mask-a, flag, dc; unlisted slots are 2048.
+// Format constants: identical numbers to src/mc_codec.c:121-140.
+const uint16_t kLoHead[8][24] = {
+    {115, 110, 112, 144, 3866, 3522, 3330, 948, 3939, 3723, 3483, 2383,
+     4007, 3795, 3575, 3130, 4042, 3912, 3805, 3686, 4064, 4064, 4028, 4064},
+    {1996, 1234, 948, 743, 595, 539, 441, 421, 333, 358, 297, 305},
+    {32, 32, 32, 32, 32, 32, 32, 32, 32, 92, 285, 2295, 4064},
+    {4025, 3064, 2995, 827, 2107, 473, 576, 50},
+    {250, 1713, 2961, 2991},
+    {3936, 32},
+    {1023, 4064, 4064, 4064},
+    {3873, 366, 1820, 2056, 2040, 2055, 2041, 2045},
+};
+const int kLoN[8] = {24, 12, 13, 8, 4, 2, 4, 8};
+const uint16_t kHiHead[8][12] = {
+    {456, 335, 535, 1133, 3950, 3755, 3590, 2892, 4064, 4051, 3990, 3686},
+    {2121, 1410, 1081, 863, 746, 631, 569, 508, 465, 431, 399, 370},
+    {59, 32, 32, 32, 32, 63, 92, 315, 2273, 4064, 4064},
+    {4025, 3064, 2995, 827, 2107, 473, 576, 50},
+    {250, 1713, 2961, 2991},
+    {3936, 32},
+    {1023, 4064, 4064, 4064},
+    {3873, 366, 1820, 2056, 2040, 2055, 2041, 2045},
+};
+const int kHiN[8] = {12, 12, 11, 8, 4, 2, 4, 8};
+
+constexpr int kMaxQTabs = 4096;
+
+struct Global {
+    std::mutex mu;
+    int device = -1;
+    bool ready = false;
+    uint16_t plo[8][32], phi[8][32];
+    std::map<uint32_t, uint32_t> qidx;      // sanitised-q bit pattern -> slot
+    QTab *d_qtabs = nullptr;
+};
+Global &G() {
+    static Global g;
+    return g;
+}
+
+void default_priors(uint16_t lo[8][32], uint16_t hi[8][32]) {
+    for (int c = 0; c < 8; ++c)
+        for (int s = 0; s < 32; ++s) {
+            lo[c][s] = s < kLoN[c] ? kLoHead[c][s] : 2048;
+            hi[c][s] = s < kHiN[c] ? kHiHead[c][s] : 2048;
+        }
+}
+
+float sanitize_q(float q) { return (!(q > 0.0f) || q > 1024.0f) ? 8.0f : q; }
+
+// Host evaluation of the per-quality tables with the reference's float
+// expressions (src/mc_codec.c:156-167 priors, :541-552 steps).
+QTab make_qtab(float q, const uint16_t lo[8][32], const uint16_t hi[8][32]) {
+    QTab t;
+    memset(&t, 0, sizeof t);
+    for (int l = 0; l < 46; ++l) {
+        float w = powf(1.0f + (float)l, 0.65f);
+        t.step[l] = q * w;
+        t.rstep[l] = 1.0f / t.step[l];
+    }
+    const float flo = 0.0f, fhi = 3.585f;
+    float w = (q <= 1.0f) ? 0.0f : ((float)(log((double)q) / log(2.0)) - flo) / (fhi - flo);
+    if (w < 0.0f) w = 0.0f;
+    if (w > 1.0f) w = 1.0f;
+    uint16_t pri[8][32];
+    for (int c = 0; c < 8; ++c)
+        for (int s = 0; s < 32; ++s) {
+            int a = lo[c][s], b = hi[c][s];
+            float v = (float)a + (float)(b - a) * w;
+            v = v + 0.5f;
+            pri[c][s] = (uint16_t)v;
+        }
+    memcpy(t.sig, pri[0], sizeof t.sig);
+    memcpy(t.mag, pri[1], sizeof t.mag);
+    memcpy(t.eob, pri[2], sizeof t.eob);
+    memcpy(t.flag, pri[6], sizeof t.flag);
+    memcpy(t.dc, pri[7], sizeof t.dc);
+    return t;
+}
+
+int init_locked(Global &g, int device) {
+    if (g.ready && (device < 0 || device == g.device)) return 0;
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (ndev <= 0) return fail("no HIP device");
+    if (device < 0) device = 0;
+    if (device >= ndev) return fail("device %d out of range (%d devices)", device, ndev);
+    HIPCHK(hipSetDevice(device));
+    // DCT-II matrix in double rounded to float (src/mc_codec_float.h:70-78)
+    float cmf[256];
+    for (int k = 0; k < 16; ++k) {
+        double ck = (k == 0) ? sqrt(1.0 / 16) : sqrt(2.0 / 16);
+        for (int n = 0; n < 16; ++n)
+            cmf[k * 16 + n] = (float)(ck * cos(3.14159265358979323846 * (2.0 * n + 1.0) * k / 32.0));
+    }
+    // ascending-L1 scan, ties by raster index (src/mc_codec.c:283-304):
+    // counting sort over the 46 L1 values keeps raster order within a value.
+    uint16_t scan[kNVox];
+    int count[47] = {0};
+    for (int i = 0; i < kNVox; ++i) count[(i >> 8) + ((i >> 4) & 15) + (i & 15) + 1]++;
+    for (int l = 1; l < 47; ++l) count[l] += count[l - 1];
+    uint16_t band_start[9];
+    for (int b = 0; b < 8; ++b) band_start[b] = (uint16_t)count[6 * b];   // band = min(7, L1/6)
+    band_start[8] = kNVox;
+    for (int i = 0; i < kNVox; ++i) scan[count[(i >> 8) + ((i >> 4) & 15) + (i & 15)]++] = (uint16_t)i;
+    HIPCHK(upload_constants(cmf, band_start, scan));
+    if (!g.d_qtabs) HIPCHK(hipMalloc((void **)&g.d_qtabs, sizeof(QTab) * kMaxQTabs));
+    default_priors(g.plo, g.phi);
+    g.qidx.clear();
+    g.device = device;
+    g.ready = true;
+    return 0;
+}
+
+// Slot of the quality table for q (built and uploaded on first use).
+int qtab_slot(float q, uint32_t *slot) {
+    Global &g = G();
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (init_locked(g, -1)) return -1;
+    float qs = sanitize_q(q);
+    uint32_t key;
+    memcpy(&key, &qs, 4);
+    auto it = g.qidx.find(key);
+    if (it != g.qidx.end()) { *slot = it->second; return 0; }
+    if ((int)g.qidx.size() >= kMaxQTabs) {
+        HIPCHK(hipDeviceSynchronize());
+        g.qidx.clear();
+    }
+    uint32_t s = (uint32_t)g.qidx.size();
+    QTab t = make_qtab(qs, g.plo, g.phi);
+    HIPCHK(hipMemcpy(g.d_qtabs + s, &t, sizeof t, hipMemcpyHostToDevice));
+    g.qidx[key] = s;
+    *slot = s;
+    return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+struct TimedLaunch {
+    const char *name;
+    hipEvent_t a, b;
+};
+
+struct mchip_ctx {
+    hipStream_t stream = nullptr;
+    // growable device buffers
+    void *d_cd = nullptr; size_t cap_cd = 0;
+    void *d_rec = nullptr; size_t cap_rec = 0;
+    void *d_plist = nullptr; size_t cap_plist = 0;
+    void *d_cinfo = nullptr; size_t cap_cinfo = 0;
+    void *d_binfo = nullptr; size_t cap_binfo = 0;
+    void *d_scratch = nullptr; size_t cap_scratch = 0;
+    void *d_stage_in = nullptr; size_t cap_stage_in = 0;
+    void *d_stage_out = nullptr; size_t cap_stage_out = 0;
+    void *d_aux = nullptr; size_t cap_aux = 0;
+    // pinned descriptor staging, reusable once ev_desc has completed
+    void *h_desc = nullptr; size_t cap_h_desc = 0;
+    hipEvent_t ev_desc = nullptr;
+    bool desc_pending = false;
+    // timing
+    bool timing = false;
+    std::vector<TimedLaunch> pending;
+    std::vector<hipEvent_t> free_events;
+    std::map<std::string, std::pair<double, long>> stats;
+};
+
+namespace {
+
+int ensure(mchip_ctx *c, void **p, size_t *cap, size_t need) {
+    if (need <= *cap) return 0;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (*p) HIPCHK(hipFree(*p));
+    *p = nullptr;
+    size_t n = need + need / 4 + 256;
+    hipError_t e = hipMalloc(p, n);
+    if (e != hipSuccess) { *cap = 0; *p = nullptr; return fail("hipMalloc(%zu) failed: %s", n, hipGetErrorString(e)); }
+    *cap = n;
+    return 0;
+}
+
+int desc_buffer(mchip_ctx *c, size_t need, void **h) {
+    if (c->desc_pending) {
+        HIPCHK(hipEventSynchronize(c->ev_desc));
+        c->desc_pending = false;
+    }
+    if (need > c->cap_h_desc) {
+        if (c->h_desc) HIPCHK(hipHostFree(c->h_desc));
+        c->h_desc = nullptr;
+        size_t n = need + need / 2 + 4096;
+        HIPCHK(hipHostMalloc(&c->h_desc, n, hipHostMallocDefault));
+        c->cap_h_desc = n;
+    }
+    *h = c->h_desc;
+    return 0;
+}
+
+hipEvent_t get_event(mchip_ctx *c) {
+    if (!c->free_events.empty()) {
+        hipEvent_t e = c->free_events.back();
+        c->free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+template <class F>
+void timed(mchip_ctx *c, const char *name, F &&launch) {
+    if (!c->timing) { launch(); return; }
+    TimedLaunch t{name, get_event(c), get_event(c)};
+    (void)hipEventRecord(t.a, c->stream);
+    launch();
+    (void)hipEventRecord(t.b, c->stream);
+    c->pending.push_back(t);
+}
+
+int collect_timing(mchip_ctx *c) {
+    for (auto &t : c->pending) {
+        HIPCHK(hipEventSynchronize(t.b));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, t.a, t.b));
+        auto &s = c->stats[t.name];
+        s.first += ms;
+        s.second += 1;
+        c->free_events.push_back(t.a);
+        c->free_events.push_back(t.b);
+    }
+    c->pending.clear();
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mchip_init(int device) {
+    Global &g = G();
+    std::lock_guard<std::mutex> lk(g.mu);
+    return init_locked(g, device);
+}
+
+const char *mchip_last_error(void) { return t_err.c_str(); }
+
+int mchip_set_priors(const uint16_t *plo, const uint16_t *phi) {
+    Global &g = G();
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (init_locked(g, -1)) return -1;
+    HIPCHK(hipDeviceSynchronize());
+    if (plo && phi) {
+        memcpy(g.plo, plo, sizeof g.plo);
+        memcpy(g.phi, phi, sizeof g.phi);
+    } else {
+        default_priors(g.plo, g.phi);
+    }
+    g.qidx.clear();
+    return 0;
+}
+
+mchip_ctx *mchip_ctx_create(void) {
+    if (mchip_init(-1)) return nullptr;
+    mchip_ctx *c = new mchip_ctx();
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_desc, hipEventDisableTiming) != hipSuccess) {
+        fail("stream/event creation failed");
+        delete c;
+        return nullptr;
+    }
+    return c;
+}
+
+void mchip_ctx_destroy(mchip_ctx *c) {
+    if (!c) return;
+    hipStreamSynchronize(c->stream);
+    collect_timing(c);
+    void *bufs[] = {c->d_cd, c->d_rec, c->d_plist, c->d_cinfo, c->d_binfo, c->d_scratch,
+                    c->d_stage_in, c->d_stage_out, c->d_aux};
+    for (void *p : bufs)
+        if (p) hipFree(p);
+    if (c->h_desc) hipHostFree(c->h_desc);
+    for (hipEvent_t e : c->free_events) hipEventDestroy(e);
+    hipEventDestroy(c->ev_desc);
+    hipStreamDestroy(c->stream);
+    delete c;
+}
+
+void *mchip_ctx_stream(mchip_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int mchip_ctx_sync(mchip_ctx *c) {
+    if (!c) return fail("null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return collect_timing(c);
+}
+
+void *mchip_malloc(size_t bytes) {
+    if (mchip_init(-1)) return nullptr;
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes + 256) != hipSuccess) { fail("hipMalloc(%zu) failed", bytes); return nullptr; }
+    return p;
+}
+void mchip_free(void *p) { if (p) hipFree(p); }
+
+int mchip_h2d(mchip_ctx *c, void *dst, const void *src, size_t n) {
+    HIPCHK(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+int mchip_d2h(mchip_ctx *c, void *dst, const void *src, size_t n) {
+    HIPCHK(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+int mchip_memset(mchip_ctx *c, void *dst, int v, size_t n) {
+    HIPCHK(hipMemsetAsync(dst, v, n, c->stream));
+    return 0;
+}
+
+int mchip_decode_chunks(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *blob_off,
+                        const uint64_t *blob_len, const float *q, int n, uint8_t *d_out,
+                        uint64_t out_stride) {
+    if (!c) return fail("null context");
+    if (n <= 0) return 0;
+    if (!d_blobs || !blob_off || !blob_len || !q || !d_out) return fail("null argument");
+    if (out_stride < (uint64_t)16777216) return fail("out_stride %llu < 256^3", (unsigned long long)out_stride);
+    void *h = nullptr;
+    if (desc_buffer(c, sizeof(ChunkDesc) * (size_t)n, &h)) return -1;
+    ChunkDesc *cd = (ChunkDesc *)h;
+    uint64_t sw = 0;
+    for (int i = 0; i < n; ++i) {
+        uint32_t slot;
+        if (qtab_slot(q[i], &slot)) return -1;
+        if (blob_len[i] > 0xFFFFFFF0ull) return fail("blob %d too long", i);
+        cd[i].blob_off = blob_off[i];
+        cd[i].blob_len = blob_len[i];
+        cd[i].out_off = (uint64_t)i * out_stride;
+        cd[i].scratch_off = sw;
+        cd[i].qtab = slot;
+        cd[i].pad = 0;
+        sw += scratch_words_for(blob_len[i]);
+    }
+    size_t nb = (size_t)n * kNVox;
+    if (ensure(c, &c->d_cd, &c->cap_cd, sizeof(ChunkDesc) * n) ||
+        ensure(c, &c->d_rec, &c->cap_rec, sizeof(BlkRec) * nb) ||
+        ensure(c, &c->d_plist, &c->cap_plist, sizeof(uint16_t) * nb) ||
+        ensure(c, &c->d_cinfo, &c->cap_cinfo, sizeof(ChunkInfo) * n) ||
+        ensure(c, &c->d_binfo, &c->cap_binfo, sizeof(uint2) * nb) ||
+        ensure(c, &c->d_scratch, &c->cap_scratch, sizeof(uint32_t) * sw))
+        return -1;
+    HIPCHK(hipMemcpyAsync(c->d_cd, cd, sizeof(ChunkDesc) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->ev_desc, c->stream));
+    c->desc_pending = true;
+    DecodeArgs a;
+    a.blobs = d_blobs;
+    a.cd = (const ChunkDesc *)c->d_cd;
+    a.qtabs = G().d_qtabs;
+    a.rec = (BlkRec *)c->d_rec;
+    a.plist = (uint16_t *)c->d_plist;
+    a.cinfo = (ChunkInfo *)c->d_cinfo;
+    a.scratch = (uint32_t *)c->d_scratch;
+    a.binfo = (uint2 *)c->d_binfo;
+    a.out = d_out;
+    a.n = n;
+    timed(c, "index", [&] { launch_index(a, c->stream); });
+    timed(c, "entropy", [&] { launch_entropy(a, c->stream); });
+    timed(c, "recon", [&] { launch_recon(a, c->stream); });
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mchip_decode_chunks_host(mchip_ctx *c, const uint8_t *const *blobs, const uint64_t *blob_len, int n,
+                             uint8_t *out, uint64_t out_stride) {
+    if (!c) return fail("null context");
+    if (n <= 0) return 0;
+    std::vector<uint64_t> off(n);
+    std::vector<float> q(n);
+    uint64_t tot = 0;
+    for (int i = 0; i < n; ++i) {
+        off[i] = tot;
+        tot += (blob_len[i] + 255) & ~255ull;
+        float qq = 8.0f;
+        if (blob_len[i] >= 4) memcpy(&qq, blobs[i], 4);
+        q[i] = qq;
+    }
+    if (ensure(c, &c->d_stage_in, &c->cap_stage_in, tot + 256) ||
+        ensure(c, &c->d_stage_out, &c->cap_stage_out, (size_t)n * 16777216ull))
+        return -1;
+    uint8_t *din = (uint8_t *)c->d_stage_in, *dout = (uint8_t *)c->d_stage_out;
+    for (int i = 0; i < n; ++i)
+        if (blob_len[i]) HIPCHK(hipMemcpyAsync(din + off[i], blobs[i], blob_len[i], hipMemcpyHostToDevice, c->stream));
+    if (mchip_decode_chunks(c, din, off.data(), blob_len, q.data(), n, dout, 16777216ull)) return -1;
+    for (int i = 0; i < n; ++i)
+        HIPCHK(hipMemcpyAsync(out + (size_t)i * out_stride, dout + (size_t)i * 16777216ull, 16777216ull,
+                              hipMemcpyDeviceToHost, c->stream));
+    return mchip_ctx_sync(c);
+}
+
+int mchip_decode_blocks_host(mchip_ctx *c, const uint8_t *const *payloads, const uint32_t *lens,
+                             const float *q, int n, uint8_t *out) {
+    if (!c) return fail("null context");
+    if (n <= 0) return 0;
+    uint64_t tot = 0;
+    for (int i = 0; i < n; ++i) if (payloads[i]) tot += ((uint64_t)lens[i] + 15) & ~15ull;
+    size_t meta = (size_t)n * (8 + 4 + 4);
+    void *h = nullptr;
+    if (desc_buffer(c, meta, &h)) return -1;
+    uint64_t *hoff = (uint64_t *)h;
+    uint32_t *hlen = (uint32_t *)(hoff + n);
+    uint32_t *hq = hlen + n;
+    uint64_t at = 0;
+    for (int i = 0; i < n; ++i) {
+        if (!payloads[i]) { hoff[i] = ~0ull; hlen[i] = 0; hq[i] = 0; continue; }
+        if (qtab_slot(q[i], &hq[i])) return -1;
+        hoff[i] = at;
+        hlen[i] = lens[i];
+        at += ((uint64_t)lens[i] + 15) & ~15ull;
+    }
+    if (ensure(c, &c->d_stage_in, &c->cap_stage_in, tot + 256) ||
+        ensure(c, &c->d_aux, &c->cap_aux, meta + 64) ||
+        ensure(c, &c->d_stage_out, &c->cap_stage_out, (size_t)n * kNVox))
+        return -1;
+    uint8_t *din = (uint8_t *)c->d_stage_in;
+    for (int i = 0; i < n; ++i)
+        if (payloads[i] && lens[i])
+            HIPCHK(hipMemcpyAsync(din + hoff[i], payloads[i], lens[i], hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_aux, h, meta, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->ev_desc, c->stream));
+    c->desc_pending = true;
+    const uint64_t *doff = (const uint64_t *)c->d_aux;
+    const uint32_t *dlen = (const uint32_t *)(doff + n);
+    const uint32_t *dq = dlen + n;
+    timed(c, "block_single", [&] {
+        launch_block_single(din, doff, dlen, dq, G().d_qtabs, (uint8_t *)c->d_stage_out, n, c->stream);
+    });
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, c->d_stage_out, (size_t)n * kNVox, hipMemcpyDeviceToHost, c->stream));
+    return mchip_ctx_sync(c);
+}
+
+int mchip_ctx_enable_timing(mchip_ctx *c, int on) {
+    if (!c) return fail("null context");
+    c->timing = on != 0;
+    return 0;
+}
+int mchip_ctx_kernel_time(mchip_ctx *c, const char *kernel, double *total_ms, long *launches) {
+    if (!c || !kernel) return fail("null argument");
+    if (collect_timing(c)) return -1;
+    auto it = c->stats.find(kernel);
+    if (total_ms) *total_ms = it == c->stats.end() ? 0.0 : it->second.first;
+    if (launches) *launches = it == c->stats.end() ? 0 : it->second.second;
+    return 0;
+}
+int mchip_ctx_reset_timing(mchip_ctx *c) {
+    if (!c) return fail("null context");
+    if (collect_timing(c)) return -1;
+    c->stats.clear();
+    return 0;
+}
+
+}  // extern "C"

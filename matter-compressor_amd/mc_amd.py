@@ -1,0 +1,203 @@
+"""ctypes binding of libmc_amd.so (the MI355X codec engine and the plain-C
+matter-compressor drop-in API) and libmc_synth.so (workload generator).
+
+Loading fails loudly when the built library is missing: there is no CPU
+fallback for any codec entry point.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+CHUNK_BYTES = 256 * 256 * 256
+
+_lib = None
+_synth = None
+
+
+class MchipError(RuntimeError):
+    pass
+
+
+def lib_path():
+    return os.path.join(LIB_DIR, "libmc_amd.so")
+
+
+def load():
+    """Load libmc_amd.so and declare its C ABI (idempotent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = lib_path()
+    if not os.path.exists(p):
+        raise MchipError(f"{p} is not built: run python matter-compressor_amd/build.py")
+    L = C.CDLL(p)
+    vp, u8p, i = C.c_void_p, C.c_void_p, C.c_int
+    sig = {
+        "mchip_init": (i, [i]),
+        "mchip_last_error": (C.c_char_p, []),
+        "mchip_set_priors": (i, [vp, vp]),
+        "mchip_ctx_create": (vp, []),
+        "mchip_ctx_destroy": (None, [vp]),
+        "mchip_ctx_stream": (vp, [vp]),
+        "mchip_ctx_sync": (i, [vp]),
+        "mchip_malloc": (vp, [C.c_size_t]),
+        "mchip_free": (None, [vp]),
+        "mchip_h2d": (i, [vp, vp, vp, C.c_size_t]),
+        "mchip_d2h": (i, [vp, vp, vp, C.c_size_t]),
+        "mchip_memset": (i, [vp, vp, i, C.c_size_t]),
+        "mchip_decode_chunks": (i, [vp, u8p, vp, vp, vp, i, u8p, C.c_uint64]),
+        "mchip_decode_chunks_host": (i, [vp, vp, vp, i, u8p, C.c_uint64]),
+        "mchip_decode_blocks_host": (i, [vp, vp, vp, vp, i, u8p]),
+        "mchip_ctx_enable_timing": (i, [vp, i]),
+        "mchip_ctx_kernel_time": (i, [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)]),
+        "mchip_ctx_reset_timing": (i, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise MchipError(load().mchip_last_error().decode(errors="replace"))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _ptr_array(bufs):
+    arr = (C.c_void_p * len(bufs))()
+    for k, b in enumerate(bufs):
+        arr[k] = None if b is None else b.ctypes.data
+    return arr
+
+
+class DeviceBuffer:
+    """Raw device allocation owned from Python."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = load().mchip_malloc(self.nbytes)
+        if not self.ptr:
+            raise MchipError(load().mchip_last_error().decode())
+
+    def free(self):
+        if self.ptr:
+            load().mchip_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Ctx:
+    """An mchip context: one HIP stream + its device workspace."""
+
+    def __init__(self, device=0):
+        L = load()
+        check(L.mchip_init(device))
+        self.h = L.mchip_ctx_create()
+        if not self.h:
+            raise MchipError(L.mchip_last_error().decode())
+
+    def close(self):
+        if self.h:
+            load().mchip_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        check(load().mchip_ctx_sync(self.h))
+
+    def h2d(self, dbuf, arr, offset=0):
+        a = np.ascontiguousarray(arr)
+        check(load().mchip_h2d(self.h, C.c_void_p(dbuf.ptr + offset), _ptr(a), a.nbytes))
+
+    def d2h(self, arr, dbuf, offset=0):
+        check(load().mchip_d2h(self.h, _ptr(arr), C.c_void_p(dbuf.ptr + offset), arr.nbytes))
+
+    def enable_timing(self, on=True):
+        check(load().mchip_ctx_enable_timing(self.h, 1 if on else 0))
+
+    def reset_timing(self):
+        check(load().mchip_ctx_reset_timing(self.h))
+
+    def kernel_time(self, name):
+        ms, n = C.c_double(), C.c_long()
+        check(load().mchip_ctx_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    # ---- decode ----------------------------------------------------------
+    def decode_chunks(self, d_blobs, offs, lens, qs, d_out, out_stride=CHUNK_BYTES):
+        n = len(offs)
+        offs = np.ascontiguousarray(offs, np.uint64)
+        lens = np.ascontiguousarray(lens, np.uint64)
+        qs = np.ascontiguousarray(qs, np.float32)
+        check(load().mchip_decode_chunks(self.h, C.c_void_p(d_blobs.ptr), _ptr(offs), _ptr(lens), _ptr(qs), n,
+                                         C.c_void_p(d_out.ptr), out_stride))
+
+    def decode_chunks_host(self, blobs):
+        n = len(blobs)
+        arrs = [np.frombuffer(b, np.uint8) if not isinstance(b, np.ndarray) else b for b in blobs]
+        lens = np.array([a.size for a in arrs], np.uint64)
+        out = np.empty((n, 256, 256, 256), np.uint8)
+        check(load().mchip_decode_chunks_host(self.h, _ptr_array(arrs), _ptr(lens), n, _ptr(out), CHUNK_BYTES))
+        return out
+
+    def decode_blocks_host(self, payloads, qs):
+        n = len(payloads)
+        arrs = [None if p is None else (np.frombuffer(p, np.uint8) if not isinstance(p, np.ndarray) else p)
+                for p in payloads]
+        lens = np.array([0 if a is None else a.size for a in arrs], np.uint32)
+        qs = np.ascontiguousarray(qs, np.float32)
+        out = np.empty((n, 4096), np.uint8)
+        check(load().mchip_decode_blocks_host(self.h, _ptr_array(arrs), _ptr(lens), _ptr(qs), n, _ptr(out)))
+        return out
+
+
+# ---- synthetic workload generator -------------------------------------------
+def synth():
+    global _synth
+    if _synth is not None:
+        return _synth
+    p = os.path.join(LIB_DIR, "libmc_synth.so")
+    if not os.path.exists(p):
+        raise MchipError(f"{p} is not built: run python matter-compressor_amd/build.py")
+    S = C.CDLL(p)
+    S.mcs_chunk.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p]
+    S.mcs_chunks.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                             C.c_int]
+    _synth = S
+    return S
+
+
+def synth_chunks(seeds, air, coords=None, nthreads=None):
+    """Generate len(seeds) 256^3 chunks (air fraction per chunk)."""
+    n = len(seeds)
+    seeds = np.ascontiguousarray(seeds, np.uint64)
+    air = np.ascontiguousarray(np.broadcast_to(np.asarray(air, np.float64), (n,)))
+    out = np.empty((n, 256, 256, 256), np.uint8)
+    if coords is None:
+        cz = cy = cx = None
+    else:
+        cc = np.ascontiguousarray(coords, np.int32).reshape(n, 3)
+        cz, cy, cx = [np.ascontiguousarray(cc[:, k]) for k in range(3)]
+    nt = nthreads or min(16, os.cpu_count() or 4)
+    synth().mcs_chunks(n, _ptr(seeds), None if cz is None else _ptr(cz), None if cy is None else _ptr(cy),
+                       None if cx is None else _ptr(cx), _ptr(air), _ptr(out), nt)
+    return out

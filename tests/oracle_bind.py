@@ -1,0 +1,144 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) and, where it was
+built in this container, the reference codec compiled from its own sources
+(oracle/_ref/libmcref.so).  Test infrastructure only."""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libmcref.so")
+ENC_CAP = 4096 * 4 + 1024
+
+_o = None
+_r = None
+
+
+def oracle():
+    global _o
+    if _o is not None:
+        return _o
+    if not os.path.exists(ORACLE_SO):
+        raise RuntimeError(f"{ORACLE_SO} missing: run make -C oracle")
+    O = C.CDLL(ORACLE_SO)
+    vp = C.c_void_p
+    O.orc_enc_block.argtypes = [C.c_float, C.c_int, vp, vp, C.c_size_t]
+    O.orc_enc_block.restype = C.c_uint32
+    O.orc_enc_block_ex.argtypes = [C.c_float, C.c_int, vp, vp, C.c_size_t, vp, vp, vp]
+    O.orc_enc_block_ex.restype = C.c_uint32
+    O.orc_dec_block.argtypes = [C.c_float, vp, C.c_uint32, vp]
+    O.orc_dec_block_levels.argtypes = [C.c_float, vp, C.c_uint32, vp, vp, vp, vp, vp, vp]
+    O.orc_encode_chunk.argtypes = [C.c_float, C.c_int, vp, vp, C.c_size_t]
+    O.orc_encode_chunk.restype = C.c_size_t
+    O.orc_decode_chunk.argtypes = [vp, C.c_uint64, vp]
+    O.orc_decode_blob_block.argtypes = [vp, C.c_uint64, C.c_int, C.c_int, C.c_int, vp]
+    O.orc_blob_len.argtypes = [vp]
+    O.orc_blob_len.restype = C.c_uint64
+    O.orc_xxh64.argtypes = [vp, C.c_size_t, C.c_uint64]
+    O.orc_xxh64.restype = C.c_uint64
+    O.orc_blob_hash.argtypes = [vp, C.c_uint64]
+    O.orc_blob_hash.restype = C.c_uint64
+    O.orc_enc_fracmap.argtypes = [vp, vp, C.c_size_t]
+    O.orc_enc_fracmap.restype = C.c_uint32
+    O.orc_dec_fracmap.argtypes = [vp, C.c_uint32, vp]
+    O.orc_dct3_fwd.argtypes = [vp, vp]
+    O.orc_dct3_inv.argtypes = [vp, vp]
+    O.orc_dct_matrix.argtypes = [vp]
+    O.orc_scan_order.argtypes = [vp]
+    O.orc_step_table.argtypes = [C.c_float, vp, vp]
+    O.orc_priors_for_q.argtypes = [C.c_float, vp]
+    O.orc_set_priors.argtypes = [vp, vp]
+    O.orc_rc_encode_ops.argtypes = [C.c_int, vp, vp, vp, vp, vp, C.c_size_t]
+    O.orc_rc_encode_ops.restype = C.c_size_t
+    O.orc_rc_decode_ops.argtypes = [C.c_int, vp, vp, vp, vp, C.c_size_t, vp]
+    O.orc_decode_chunks_mt.argtypes = [C.c_int, vp, vp, vp, C.c_int]
+    O.orc_encode_chunks_mt.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, C.c_int]
+    _o = O
+    return O
+
+
+def p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def enc_block(vox, q, tau=0):
+    vox = np.ascontiguousarray(vox, np.uint8).ravel()
+    out = np.zeros(ENC_CAP, np.uint8)
+    n = oracle().orc_enc_block(q, tau, p(vox), p(out), out.size)
+    return out[:n].tobytes()
+
+
+def dec_block(payload, q):
+    a = np.frombuffer(payload, np.uint8) if len(payload) else np.zeros(1, np.uint8)
+    out = np.zeros(4096, np.uint8)
+    oracle().orc_dec_block(q, p(a), len(payload), p(out))
+    return out
+
+
+def encode_chunk(chunk, q, tau=0):
+    chunk = np.ascontiguousarray(chunk, np.uint8)
+    out = np.zeros(20 << 20, np.uint8)
+    n = oracle().orc_encode_chunk(q, tau, p(chunk), p(out), out.size)
+    return out[:n].tobytes()
+
+
+def decode_chunk(blob):
+    a = np.frombuffer(blob, np.uint8)
+    out = np.zeros((256, 256, 256), np.uint8)
+    oracle().orc_decode_chunk(p(a), len(blob), p(out))
+    return out
+
+
+def xxh64(data, seed=0):
+    a = np.frombuffer(bytes(data), np.uint8) if not isinstance(data, np.ndarray) else np.ascontiguousarray(data)
+    if a.size == 0:
+        a = np.zeros(1, np.uint8)
+        return oracle().orc_xxh64(p(a), 0, seed)
+    return oracle().orc_xxh64(p(a), a.nbytes, seed)
+
+
+# ---- the reference build (only in the container that holds its sources) ----
+class _McBuf(C.Structure):
+    _fields_ = [("p", C.c_void_p), ("len", C.c_size_t), ("cap", C.c_size_t)]
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _r
+    if _r is not None:
+        return _r
+    R = C.CDLL(REF_SO)
+    vp = C.c_void_p
+    R.mc_codec_ctx_new.restype = vp
+    R.mc_codec_ctx_free.argtypes = [vp]
+    R.mc_codec_ctx_set_quality.argtypes = [vp, C.c_float]
+    R.mc_codec_ctx_set_max_error.argtypes = [vp, C.c_int]
+    R.mc_apply_preset.argtypes = [vp, C.c_int]
+    R.mc_apply_preset.restype = C.c_float
+    R.mc_enc_block.argtypes = [vp, vp, C.POINTER(_McBuf), C.POINTER(C.c_uint32)]
+    R.mc_dec_block.argtypes = [vp, vp, C.c_uint32, vp]
+    R.mc_enc_fracmap.argtypes = [vp, vp, C.c_size_t]
+    R.mc_enc_fracmap.restype = C.c_uint32
+    R.mc_xxh64.argtypes = [vp, C.c_size_t, C.c_uint64]
+    R.mc_xxh64.restype = C.c_uint64
+    R.mc_archive_open_dims.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_float]
+    R.mc_archive_open_dims.restype = vp
+    R.mc_archive_close.argtypes = [vp]
+    R.mc_archive_append_chunk_par.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_float, C.c_int]
+    R.mc_archive_append_chunk_ctx.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]
+    R.mc_archive_chunk_offset.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+    R.mc_archive_chunk_offset.restype = C.c_uint64
+    R.mc_archive_decode_chunk.argtypes = [vp, C.c_uint64, vp, C.c_int]
+    R.mc_archive_decode_block.argtypes = [vp, C.c_uint64, C.c_int, C.c_int, C.c_int, vp]
+    R.mc_archive_read_regions.argtypes = [vp, C.c_int, vp, C.c_int, C.c_long, C.c_long, C.c_long, vp,
+                                          C.c_size_t, C.c_int]
+    R.mc_archive_sample_boxes.argtypes = [vp, C.c_int, C.c_uint64, C.c_int, C.c_long, C.c_long, C.c_long,
+                                          C.c_float, vp]
+    R.mc_archive_sample_boxes.restype = C.c_int
+    R.mc_codec_init()
+    _r = R
+    return R

@@ -1,0 +1,122 @@
+"""Decode parity of the HIP path against the oracle and the reference's
+golden fixtures (bit-exact)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def kat_payload(kat, i):
+    return kat["payload"][kat["offs"][i]:kat["offs"][i + 1]].tobytes()
+
+
+def test_block_decode_matches_reference_kat(gpu_ctx, kat):
+    idx = [i for i in range(len(kat["q"])) if kat["offs"][i + 1] > kat["offs"][i]]
+    print(f"decoding {len(idx)} KAT payloads", flush=True)
+    out = gpu_ctx.decode_blocks_host([kat_payload(kat, i) for i in idx], kat["q"][idx])
+    for j, i in enumerate(idx):
+        assert np.array_equal(out[j], kat["dec"][i]), f"case {i} {kat['names'][i]} q={kat['q'][i]}"
+
+
+def test_block_decode_corrupt_payloads_match_oracle(gpu_ctx, kat):
+    rng = np.random.default_rng(9)
+    pays, qs = [], []
+    for trial in range(300):
+        i = int(rng.integers(0, len(kat["q"])))
+        p = bytearray(kat_payload(kat, i) or b"\x00")
+        for _ in range(int(rng.integers(1, 4))):
+            p[int(rng.integers(0, len(p)))] ^= int(rng.integers(1, 256))
+        if trial % 4 == 0:
+            p = p[: int(rng.integers(0, len(p) + 1))]
+        pays.append(bytes(p))
+        qs.append(float(kat["q"][i]))
+    print("decoding 300 corrupted payloads", flush=True)
+    out = gpu_ctx.decode_blocks_host(pays, np.array(qs, np.float32))
+    for k in range(len(pays)):
+        assert np.array_equal(out[k], ob.dec_block(pays[k], qs[k])), f"trial {k}"
+
+
+def _chunks(specs):
+    import mc_amd
+    return mc_amd.synth_chunks([s["seed"] for s in specs], [s["air"] for s in specs])
+
+
+def test_chunk_decode_matches_oracle_and_manifest(gpu_ctx):
+    man = json.load(open(os.path.join(ROOT, "tests", "golden", "chunk_manifest.json")))
+    specs = [man["plain64_q6"][0], man["mixed256"][0], man["mixed256"][1], man["mixed256"][5]]
+    chunks = _chunks(specs)
+    blobs = []
+    for s, ch in zip(specs, chunks):
+        assert f"{ob.xxh64(ch):016x}" == s["input_xxh64"], "generator output drifted"
+        b = ob.encode_chunk(ch, s["q"], s["tau"])
+        assert f"{ob.xxh64(b):016x}" == s["blob_xxh64"]
+        blobs.append(b)
+    print("decoding 4 chunks on the GPU", flush=True)
+    out = gpu_ctx.decode_chunks_host(blobs)
+    for k, s in enumerate(specs):
+        assert f"{ob.xxh64(out[k]):016x}" == s["decoded_xxh64"], f"chunk {k} seed {s['seed']}"
+
+
+def test_chunk_decode_truncated_and_corrupt_blobs_match_oracle(gpu_ctx):
+    import mc_amd
+    ch = mc_amd.synth_chunks([42], [0.5])[0]
+    blob = bytearray(ob.encode_chunk(ch, 3.0, 2))
+    rng = np.random.default_rng(4)
+    variants = [bytes(blob[: len(blob) - 1000]), bytes(blob[: len(blob) // 2])]
+    b2 = bytearray(blob)
+    for _ in range(50):
+        b2[int(rng.integers(700, len(b2)))] ^= 0x5A
+    variants.append(bytes(b2))
+    out = gpu_ctx.decode_chunks_host(variants)
+    for k, v in enumerate(variants):
+        assert np.array_equal(out[k], ob.decode_chunk(v)), f"variant {k}"
+
+
+def test_all_air_and_sparse_chunks(gpu_ctx):
+    ch = np.zeros((2, 256, 256, 256), np.uint8)
+    ch[1, 100:105, 7, 200] = 77
+    ch[1, 255, 255, 255] = 3
+    blobs = [ob.encode_chunk(ch[1], 6.0, 0)]
+    assert ob.encode_chunk(ch[0], 6.0, 0) == b""
+    out = gpu_ctx.decode_chunks_host(blobs)
+    assert np.array_equal(out[0], ob.decode_chunk(blobs[0]))
+
+
+def test_device_resident_batch_equals_per_chunk(gpu_ctx):
+    import mc_amd
+    specs = [dict(seed=s, air=a) for s, a in [(7, 0.3), (8, 0.6), (9, 0.9)]]
+    chunks = _chunks(specs)
+    blobs = [ob.encode_chunk(c, q, 0) for c, q in zip(chunks, [6.0, 1.0, 32.0])]
+    offs = np.zeros(len(blobs), np.uint64)
+    tot = 0
+    for k, b in enumerate(blobs):
+        offs[k] = tot
+        tot += (len(b) + 255) // 256 * 256
+    buf = np.zeros(tot, np.uint8)
+    for k, b in enumerate(blobs):
+        buf[int(offs[k]):int(offs[k]) + len(b)] = np.frombuffer(b, np.uint8)
+    d_in = mc_amd.DeviceBuffer(tot)
+    d_out = mc_amd.DeviceBuffer(len(blobs) * mc_amd.CHUNK_BYTES)
+    gpu_ctx.h2d(d_in, buf)
+    qs = [np.frombuffer(b[:4], np.float32)[0] for b in blobs]
+    gpu_ctx.decode_chunks(d_in, offs, [len(b) for b in blobs], qs, d_out)
+    gpu_ctx.sync()
+    got = np.empty((len(blobs), 256, 256, 256), np.uint8)
+    gpu_ctx.d2h(got, d_out)
+    for k in range(len(blobs)):
+        assert np.array_equal(got[k], ob.decode_chunk(blobs[k]))
+    # one chunk per launch gives the same bytes
+    for k in range(len(blobs)):
+        gpu_ctx.decode_chunks(d_in, offs[k:k + 1], [len(blobs[k])], qs[k:k + 1], d_out)
+    gpu_ctx.sync()
+    one = np.empty((256, 256, 256), np.uint8)
+    gpu_ctx.d2h(one, d_out)
+    assert np.array_equal(one, got[-1])
+    d_in.free()
+    d_out.free()
