@@ -76,6 +76,26 @@ int mchip_decode_chunks_host(mchip_ctx *ctx, const uint8_t *const *blobs, const 
 int mchip_decode_blocks_host(mchip_ctx *ctx, const uint8_t *const *payloads, const uint32_t *lens,
                              const float *q, int n, uint8_t *out);
 
+/* Encode n dense 256^3 chunks resident in device memory (chunk i at
+ * d_in + i*in_stride, 16-byte aligned) into v7 blobs at d_out + out_off[i]
+ * (capacity out_cap[i]; the buffer must stay readable 16 bytes past each
+ * region).  q[i] selects the quantiser and priors (sanitised like a codec
+ * context); tau[i] > 0 adds max-error corrections (NULL = all 0); the blob
+ * header stores q_store[i] (NULL = q[i] as given).  On return blob_len[i]
+ * is the blob length (0 = all-air chunk, no blob) and status[i] (may be
+ * NULL) is 0, 1 (a block payload exceeded 17408 bytes) or 2 (out_cap too
+ * small; blob_len then holds the needed length).  Synchronous. */
+int mchip_encode_chunks(mchip_ctx *ctx, const uint8_t *d_in, uint64_t in_stride, const float *q,
+                        const int *tau, const float *q_store, int n, uint8_t *d_out,
+                        const uint64_t *out_off, const uint64_t *out_cap, uint64_t *blob_len,
+                        int *status);
+
+/* Host-memory convenience for mchip_encode_chunks: chunk i at
+ * chunks + i*16777216, blob i written to blobs[i] (capacity caps[i]). */
+int mchip_encode_chunks_host(mchip_ctx *ctx, const uint8_t *chunks, const float *q, const int *tau,
+                             const float *q_store, int n, uint8_t *const *blobs, const uint64_t *caps,
+                             uint64_t *blob_len, int *status);
+
 /* Kernel timing: when enabled, every kernel launched on the context is
  * bracketed by HIP events on the context's stream; stats accumulate per
  * kernel name ("index", "entropy", "recon", ...). */

@@ -35,7 +35,7 @@ HIP_FLAGS = [
 C_FLAGS = ["-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall",
            "-Wno-unused-function", "-I", INCLUDE, "-D_GNU_SOURCE"]
 
-HIP_SOURCES = ["mchip_decode.hip", "mchip_encode.hip", "mchip_api.hip"]
+HIP_SOURCES = ["mchip_kernels.hip", "mchip_api.hip"]
 HOST_SOURCES = ["mc_host.c"]
 
 

@@ -197,6 +197,16 @@ struct mchip_ctx {
     void *d_stage_in = nullptr; size_t cap_stage_in = 0;
     void *d_stage_out = nullptr; size_t cap_stage_out = 0;
     void *d_aux = nullptr; size_t cap_aux = 0;
+    // encode workspace
+    void *d_ecd = nullptr; size_t cap_ecd = 0;
+    void *d_einfo = nullptr; size_t cap_einfo = 0;
+    void *d_levels = nullptr; size_t cap_levels = 0;
+    void *d_corr = nullptr; size_t cap_corr = 0;
+    void *d_slots = nullptr; size_t cap_slots = 0;
+    void *d_plen = nullptr; size_t cap_plen = 0;
+    void *d_dst = nullptr; size_t cap_dst = 0;
+    void *d_res = nullptr; size_t cap_res = 0;
+    void *h_res = nullptr;
     // pinned descriptor staging, reusable once ev_desc has completed
     void *h_desc = nullptr; size_t cap_h_desc = 0;
     hipEvent_t ev_desc = nullptr;
@@ -318,10 +328,12 @@ void mchip_ctx_destroy(mchip_ctx *c) {
     hipStreamSynchronize(c->stream);
     collect_timing(c);
     void *bufs[] = {c->d_cd, c->d_rec, c->d_plist, c->d_cinfo, c->d_binfo, c->d_scratch,
-                    c->d_stage_in, c->d_stage_out, c->d_aux};
+                    c->d_stage_in, c->d_stage_out, c->d_aux, c->d_ecd, c->d_einfo, c->d_levels,
+                    c->d_corr, c->d_slots, c->d_plen, c->d_dst, c->d_res};
     for (void *p : bufs)
         if (p) hipFree(p);
     if (c->h_desc) hipHostFree(c->h_desc);
+    if (c->h_res) hipHostFree(c->h_res);
     for (hipEvent_t e : c->free_events) hipEventDestroy(e);
     hipEventDestroy(c->ev_desc);
     hipStreamDestroy(c->stream);
@@ -478,6 +490,118 @@ int mchip_decode_blocks_host(mchip_ctx *c, const uint8_t *const *payloads, const
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, c->d_stage_out, (size_t)n * kNVox, hipMemcpyDeviceToHost, c->stream));
     return mchip_ctx_sync(c);
+}
+
+// Chunks per encode launch: bounds the workspace (about 160 MiB per chunk of
+// levels, corrections and payload slots).
+constexpr int kEncBatch = 32;
+
+int mchip_encode_chunks(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride, const float *q, const int *tau,
+                        const float *q_store, int n, uint8_t *d_out, const uint64_t *out_off,
+                        const uint64_t *out_cap, uint64_t *blob_len, int *status) {
+    if (!c) return fail("null context");
+    if (n <= 0) return 0;
+    if (!d_in || !q || !d_out || !out_off || !out_cap || !blob_len) return fail("null argument");
+    if (in_stride % 16 || ((uintptr_t)d_in) % 16) return fail("input rasters must be 16-byte aligned");
+    for (int b0 = 0; b0 < n; b0 += kEncBatch) {
+        const int m = n - b0 < kEncBatch ? n - b0 : kEncBatch;
+        void *h = nullptr;
+        if (desc_buffer(c, sizeof(EncChunkDesc) * (size_t)m, &h)) return -1;
+        EncChunkDesc *cd = (EncChunkDesc *)h;
+        for (int i = 0; i < m; ++i) {
+            const int k = b0 + i;
+            uint32_t slot;
+            if (qtab_slot(q[k], &slot)) return -1;
+            cd[i].in_off = (uint64_t)i * in_stride;     // relative to the sub-batch base
+            cd[i].out_off = out_off[k];
+            cd[i].out_cap = out_cap[k];
+            cd[i].qtab = slot;
+            cd[i].tau = tau ? (tau[k] < 0 ? 0 : tau[k]) : 0;
+            cd[i].q_store = q_store ? q_store[k] : q[k];
+            cd[i].pad = 0;
+        }
+        const size_t nb = (size_t)m * kNVox;
+        if (ensure(c, &c->d_ecd, &c->cap_ecd, sizeof(EncChunkDesc) * m) ||
+            ensure(c, &c->d_einfo, &c->cap_einfo, sizeof(uint4) * nb) ||
+            ensure(c, &c->d_levels, &c->cap_levels, sizeof(int16_t) * nb * kNVox) ||
+            ensure(c, &c->d_corr, &c->cap_corr, sizeof(uint32_t) * nb * kNVox) ||
+            ensure(c, &c->d_slots, &c->cap_slots, (size_t)kSlotBytes * nb) ||
+            ensure(c, &c->d_plen, &c->cap_plen, sizeof(uint32_t) * nb) ||
+            ensure(c, &c->d_dst, &c->cap_dst, sizeof(uint32_t) * nb) ||
+            ensure(c, &c->d_res, &c->cap_res, sizeof(EncChunkResult) * kEncBatch))
+            return -1;
+        if (!c->h_res) HIPCHK(hipHostMalloc(&c->h_res, sizeof(EncChunkResult) * kEncBatch, hipHostMallocDefault));
+        HIPCHK(hipMemcpyAsync(c->d_ecd, cd, sizeof(EncChunkDesc) * m, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipEventRecord(c->ev_desc, c->stream));
+        c->desc_pending = true;
+        EncodeArgs a;
+        a.in = d_in + (size_t)b0 * in_stride;
+        a.cd = (const EncChunkDesc *)c->d_ecd;
+        a.qtabs = G().d_qtabs;
+        a.einfo = (uint4 *)c->d_einfo;
+        a.levels = (int16_t *)c->d_levels;
+        a.corr = (uint32_t *)c->d_corr;
+        a.slots = (uint32_t *)c->d_slots;
+        a.plen = (uint32_t *)c->d_plen;
+        a.dst = (uint32_t *)c->d_dst;
+        a.out = d_out;
+        a.res = (EncChunkResult *)c->d_res;
+        a.n = m;
+        timed(c, "enc_transform", [&] { launch_enc_transform(a, c->stream); });
+        timed(c, "enc_entropy", [&] { launch_enc_entropy(a, c->stream); });
+        timed(c, "enc_assemble", [&] { launch_enc_assemble(a, c->stream); });
+        timed(c, "enc_copy", [&] { launch_enc_copy(a, c->stream); });
+        timed(c, "enc_hash", [&] { launch_enc_hash(a, c->stream); });
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(c->h_res, c->d_res, sizeof(EncChunkResult) * m, hipMemcpyDeviceToHost, c->stream));
+        if (mchip_ctx_sync(c)) return -1;
+        const EncChunkResult *r = (const EncChunkResult *)c->h_res;
+        for (int i = 0; i < m; ++i) {
+            blob_len[b0 + i] = r[i].blob_len;
+            if (status) status[b0 + i] = (int)r[i].status;
+        }
+    }
+    return 0;
+}
+
+int mchip_encode_chunks_host(mchip_ctx *c, const uint8_t *chunks, const float *q, const int *tau,
+                             const float *q_store, int n, uint8_t *const *blobs, const uint64_t *caps,
+                             uint64_t *blob_len, int *status) {
+    if (!c) return fail("null context");
+    if (n <= 0) return 0;
+    const uint64_t CH = 16777216ull;
+    std::vector<uint64_t> off(n), cap(n);
+    uint64_t tot = 0;
+    for (int i = 0; i < n; ++i) {
+        off[i] = tot;
+        cap[i] = caps[i];
+        tot += (caps[i] + 255) & ~255ull;
+    }
+    std::vector<int> st(n, 0);
+    for (int b0 = 0; b0 < n; b0 += kEncBatch) {
+        const int m = n - b0 < kEncBatch ? n - b0 : kEncBatch;
+        uint64_t base = off[b0], span = 0;
+        for (int i = 0; i < m; ++i) span = off[b0 + i] + ((caps[b0 + i] + 255) & ~255ull) - base;
+        if (ensure(c, &c->d_stage_in, &c->cap_stage_in, (size_t)m * CH + 256) ||
+            ensure(c, &c->d_stage_out, &c->cap_stage_out, span + 256))
+            return -1;
+        HIPCHK(hipMemcpyAsync(c->d_stage_in, chunks + (size_t)b0 * CH, (size_t)m * CH, hipMemcpyHostToDevice,
+                              c->stream));
+        std::vector<uint64_t> ro(m);
+        for (int i = 0; i < m; ++i) ro[i] = off[b0 + i] - base;
+        if (mchip_encode_chunks(c, (const uint8_t *)c->d_stage_in, CH, q + b0, tau ? tau + b0 : nullptr,
+                                q_store ? q_store + b0 : nullptr, m, (uint8_t *)c->d_stage_out, ro.data(),
+                                cap.data() + b0, blob_len + b0, st.data() + b0))
+            return -1;
+        for (int i = 0; i < m; ++i)
+            if (st[b0 + i] == 0 && blob_len[b0 + i])
+                HIPCHK(hipMemcpyAsync(blobs[b0 + i], (const uint8_t *)c->d_stage_out + ro[i], blob_len[b0 + i],
+                                      hipMemcpyDeviceToHost, c->stream));
+        if (mchip_ctx_sync(c)) return -1;
+    }
+    if (status)
+        for (int i = 0; i < n; ++i) status[i] = st[i];
+    return 0;
 }
 
 int mchip_ctx_enable_timing(mchip_ctx *c, int on) {

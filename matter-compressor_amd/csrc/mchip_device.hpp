@@ -186,3 +186,75 @@ __device__ __forceinline__ void fdct16_line(float v[16], const float *cm) {
 }
 
 }  // namespace mchip
+
+namespace mchip {
+
+// Range encoder writing bytes into a 4-byte aligned buffer (global or LDS)
+// through a word accumulator (src/mc_codec.c:178-195, :209-233).  Bytes
+// beyond cap are counted but not stored; n is the full payload length.
+struct RcEnc {
+    uint32_t *out;
+    uint32_t cap, n, wbuf;
+    uint64_t low;
+    uint32_t range, cache, pending;
+
+    __device__ __forceinline__ void init(uint32_t *o, uint32_t capacity) {
+        out = o; cap = capacity; n = 0; wbuf = 0;
+        low = 0; range = 0xFFFFFFFFu; cache = 0; pending = 1;
+    }
+    __device__ __forceinline__ void put(uint32_t b) {
+        if (n < cap) {
+            wbuf |= (b & 0xFFu) << (8 * (n & 3));
+            if ((n & 3) == 3) { out[n >> 2] = wbuf; wbuf = 0; }
+        }
+        n++;
+    }
+    __device__ __forceinline__ void shift_low() {
+        uint32_t top = (uint32_t)(low >> 32);
+        if (top != 0 || low < 0xFF000000ull) {
+            put(cache + top);
+            while (--pending) put(0xFFu + top);
+            cache = (uint32_t)(low >> 24) & 0xFFu;
+        }
+        pending++;
+        low = (low << 8) & 0xFFFFFFFFull;
+    }
+    __device__ __forceinline__ void norm() {
+        #pragma unroll 1
+        for (int i = 0; i < 4 && range < kRcTop; ++i) { shift_low(); range <<= 8; }
+    }
+    __device__ __forceinline__ void bit(uint32_t &p0, int b) {
+        uint32_t r0 = __umul24(range >> 12, p0);
+        if (!b) { range = r0; p0 += (4096u - p0) >> 4; }
+        else    { low += r0; range -= r0; p0 -= p0 >> 4; }
+        norm();
+    }
+    __device__ __forceinline__ void bit_once(uint32_t p0, int b) { uint32_t p = p0; bit(p, b); }
+    __device__ __forceinline__ void bypass(int b) {
+        range >>= 1;
+        if (b) low += range;
+        norm();
+    }
+    __device__ __forceinline__ void bypass_k16(uint32_t v, int k) {   // 1 <= k <= 16
+        range >>= k;
+        low += (uint64_t)(v & ((1u << k) - 1u)) * range;
+        norm();
+    }
+    __device__ __forceinline__ void bypass_n(uint32_t v, int k) {
+        while (k > 16) { bypass_k16((v >> (k - 16)) & 0xFFFFu, 16); k -= 16; }
+        if (k > 0) bypass_k16(v, k);
+    }
+    __device__ __forceinline__ void expgolomb(uint32_t v) {
+        int nb = 31 - __clz(v + 1u);            // floor(log2(v+1))
+        for (int i = 0; i < nb; ++i) bypass(1);
+        bypass(0);
+        if (nb) bypass_n((v + 1u) & ((1u << nb) - 1u), nb);
+    }
+    __device__ __forceinline__ void finish() {
+        #pragma unroll 1
+        for (int i = 0; i < 5; ++i) shift_low();
+        if ((n & 3) && n < cap) out[n >> 2] = wbuf;
+    }
+};
+
+}  // namespace mchip

@@ -1,0 +1,543 @@
+// mchip_encode.hip -- CDNA4 encode path of the 16^3-block codec.
+//
+// A dense 256^3 chunk becomes a v7 chunk blob (SuperOptimizer/
+// matter-compressor src/mc_codec.c:809-845, src/mc_archive.c:822-901) in five
+// kernels:
+//
+//   k_enc_transform  one workgroup per block: gather the 16^3 tile, count and
+//                    sum (dc, material fraction), forward DCT in LDS (x, y, z
+//                    passes), dead-zone quantisation, levels in scan order
+//                    and the EOB; with tau > 0 also dequantise, inverse DCT
+//                    and list the voxels whose error exceeds tau
+//                    (src/mc_codec.c:629-709);
+//   k_enc_entropy    one lane per block: the range encoder writes header,
+//                    EOB, significance/magnitude/sign and correction bins
+//                    into the block's slot (src/mc_codec.c:711-737);
+//   k_enc_assemble   one workgroup per chunk: block bitmap, u16 length table
+//                    and payload offsets by prefix scans, the fraction map
+//                    (one lane, src/mc_codec.c:600-611), blob header;
+//   k_enc_copy       one workgroup per block: payload into its final place;
+//   k_enc_hash       4 lanes per chunk: the XXH64 chain over fraction map,
+//                    bitmap, lengths and payloads (src/mc_archive.c:76-88).
+#include "mchip_device.hpp"
+#include "mchip_internal.hpp"
+
+namespace mchip {
+
+// c_cmf, c_band_start and g_scan are defined in mchip_decode.hip; both files
+// are compiled as one translation unit (mchip_kernels.hip).
+
+constexpr int kPadE = 17;
+
+__device__ __forceinline__ uint32_t xcd_remap_e(uint32_t i, uint32_t n) {
+    if (n % 8u) return i;
+    return (i % 8u) * (n / 8u) + i / 8u;
+}
+
+// sum over a 256-thread workgroup (all threads get the result)
+__device__ __forceinline__ uint32_t wg_sum(uint32_t v, uint32_t *lds4) {
+    #pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    if ((threadIdx.x & 63) == 0) lds4[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t r = lds4[0] + lds4[1] + lds4[2] + lds4[3];
+    __syncthreads();
+    return r;
+}
+__device__ __forceinline__ uint32_t wg_max(uint32_t v, uint32_t *lds4) {
+    #pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = max(v, (uint32_t)__shfl_xor(v, d, 64));
+    if ((threadIdx.x & 63) == 0) lds4[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t r = max(max(lds4[0], lds4[1]), max(lds4[2], lds4[3]));
+    __syncthreads();
+    return r;
+}
+__device__ __forceinline__ uint32_t wg_excl_scan(uint32_t v, uint32_t *lds4, uint32_t &total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+    #pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) lds4[wave] = x;
+    __syncthreads();
+    uint32_t w0 = lds4[0], w1 = lds4[1], w2 = lds4[2], w3 = lds4[3];
+    uint32_t base = (wave > 0 ? w0 : 0) + (wave > 1 ? w1 : 0) + (wave > 2 ? w2 : 0);
+    total = w0 + w1 + w2 + w3;
+    __syncthreads();
+    return base + x - v;
+}
+
+// ---------------------------------------------------------------------------
+// k_enc_transform
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_enc_transform(const uint8_t *__restrict__ in,
+                                                       const EncChunkDesc *__restrict__ cd,
+                                                       const QTab *__restrict__ qtabs,
+                                                       uint4 *__restrict__ einfo,
+                                                       int16_t *__restrict__ levels,
+                                                       uint32_t *__restrict__ corr, int nchunks) {
+    __shared__ float tile[256 * kPadE];
+    __shared__ __attribute__((aligned(16))) uint8_t vox[kNVox];
+    __shared__ int16_t lvr[kNVox];
+    __shared__ uint32_t lds4[4];
+    const uint32_t g = xcd_remap_e(blockIdx.x, gridDim.x);
+    const int c = (int)(g >> 12), bi = (int)(g & 4095), t = threadIdx.x;
+    if (c >= nchunks) return;
+    const size_t gb = (size_t)c * kNVox + bi;
+    const uint8_t *src = in + cd[c].in_off + (size_t)(bi >> 8) * 16 * 65536 +
+                         (size_t)((bi >> 4) & 15) * 16 * 256 + (size_t)(bi & 15) * 16;
+    const uint4 row = *(const uint4 *)(src + (size_t)(t >> 4) * 65536 + (size_t)(t & 15) * 256);
+    ((uint4 *)vox)[t] = row;
+    uint32_t s = 0, n = 0;
+    {
+        const uint32_t w[4] = {row.x, row.y, row.z, row.w};
+        #pragma unroll
+        for (int j = 0; j < 4; ++j)
+            #pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                uint32_t v = (w[j] >> (8 * b)) & 255u;
+                s += v;
+                n += v != 0;
+            }
+    }
+    const uint32_t sum = wg_sum(s, lds4), cnt = wg_sum(n, lds4);
+    if (cnt == 0) {
+        if (t == 0) einfo[gb] = make_uint4(0, 0, 0, 0);
+        return;
+    }
+    const int dc = (int)((sum + 2048u) >> 12);
+    uint32_t frac = (cnt * 15u + 2048u) / 4096u;
+    if (!frac) frac = 1;
+    const QTab *qt = qtabs + cd[c].qtab;
+    const int tau = cd[c].tau;
+    const float *cmf = c_cmf;
+    float v[16];
+    {   // x pass on line (z,y) = t
+        #pragma unroll
+        for (int x = 0; x < 16; ++x) v[x] = (float)((int)vox[t * 16 + x] - dc);
+        fdct16_line(v, cmf);
+        float *ln = tile + t * kPadE;
+        #pragma unroll
+        for (int k = 0; k < 16; ++k) ln[k] = v[k];
+    }
+    __syncthreads();
+    {   // y pass on line (z,x)
+        float *ln = tile + (t >> 4) * 16 * kPadE + (t & 15);
+        #pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = ln[k * kPadE];
+        fdct16_line(v, cmf);
+        #pragma unroll
+        for (int k = 0; k < 16; ++k) ln[k * kPadE] = v[k];
+    }
+    __syncthreads();
+    int lv[16];
+    {   // z pass on line (y,x), then quantise
+        const int y = t >> 4, x = t & 15;
+        float *ln = tile + y * kPadE + x;
+        #pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = ln[k * 16 * kPadE];
+        fdct16_line(v, cmf);
+        #pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            lv[k] = quant(v[k], qt->rstep[k + y + x]);
+            lvr[k * 256 + t] = (int16_t)lv[k];
+        }
+    }
+    __syncthreads();
+    // levels in scan order for p in [16t, 16t+16) and the EOB
+    int16_t sp[16];
+    uint32_t last = 0, nz = 0;
+    #pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        sp[j] = lvr[g_scan[t * 16 + j]];
+        if (sp[j]) { last = t * 16 + j + 1; nz++; }
+    }
+    const uint32_t eob = wg_max(last, lds4);
+    const uint32_t nnz = wg_sum(nz, lds4);
+    if ((uint32_t)t * 16 < eob) {
+        int16_t *L = levels + gb * kNVox + t * 16;
+        uint4 a, b;
+        a.x = (uint16_t)sp[0] | ((uint32_t)(uint16_t)sp[1] << 16);
+        a.y = (uint16_t)sp[2] | ((uint32_t)(uint16_t)sp[3] << 16);
+        a.z = (uint16_t)sp[4] | ((uint32_t)(uint16_t)sp[5] << 16);
+        a.w = (uint16_t)sp[6] | ((uint32_t)(uint16_t)sp[7] << 16);
+        b.x = (uint16_t)sp[8] | ((uint32_t)(uint16_t)sp[9] << 16);
+        b.y = (uint16_t)sp[10] | ((uint32_t)(uint16_t)sp[11] << 16);
+        b.z = (uint16_t)sp[12] | ((uint32_t)(uint16_t)sp[13] << 16);
+        b.w = (uint16_t)sp[14] | ((uint32_t)(uint16_t)sp[15] << 16);
+        ((uint4 *)L)[0] = a;
+        ((uint4 *)L)[1] = b;
+    }
+    uint32_t ncorr = 0;
+    if (tau > 0) {
+        // local reconstruction exactly as the decoder computes it
+        {
+            const int y = t >> 4, x = t & 15;
+            float *ln = tile + y * kPadE + x;
+            #pragma unroll
+            for (int k = 0; k < 16; ++k) ln[k * 16 * kPadE] = dequant(lv[k], qt->step[k + y + x]);
+        }
+        __syncthreads();
+        {
+            float *ln = tile + t * kPadE;
+            #pragma unroll
+            for (int k = 0; k < 16; ++k) v[k] = ln[k];
+            idct16_line(v, cmf);
+            #pragma unroll
+            for (int k = 0; k < 16; ++k) ln[k] = v[k];
+        }
+        __syncthreads();
+        {
+            float *ln = tile + (t >> 4) * 16 * kPadE + (t & 15);
+            #pragma unroll
+            for (int k = 0; k < 16; ++k) v[k] = ln[k * kPadE];
+            idct16_line(v, cmf);
+            #pragma unroll
+            for (int k = 0; k < 16; ++k) ln[k * kPadE] = v[k];
+        }
+        __syncthreads();
+        {
+            float *ln = tile + (t >> 4) * kPadE + (t & 15);
+            #pragma unroll
+            for (int k = 0; k < 16; ++k) v[k] = ln[k * 16 * kPadE];
+            idct16_line(v, cmf);
+            #pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                int rv = (int)((uint32_t)lrint_low32(v[k]) + (uint32_t)dc);
+                rv = rv < 0 ? 0 : (rv > 255 ? 255 : rv);
+                int err = (int)vox[k * 256 + t] - rv;
+                int ae = err < 0 ? -err : err;
+                int d = ae > tau ? (err < 0 ? -(ae - tau) : (ae - tau)) : 0;
+                lvr[k * 256 + t] = (int16_t)d;     // reuse: error excess by raster
+            }
+        }
+        __syncthreads();
+        // compact the corrections in raster order
+        int16_t dv[16];
+        uint32_t cntc = 0;
+        #pragma unroll
+        for (int j = 0; j < 16; ++j) { dv[j] = lvr[t * 16 + j]; cntc += dv[j] != 0; }
+        uint32_t base = wg_excl_scan(cntc, lds4, ncorr);
+        uint32_t *C = corr + gb * kNVox;
+        #pragma unroll
+        for (int j = 0; j < 16; ++j)
+            if (dv[j]) C[base++] = (uint32_t)(t * 16 + j) | ((uint32_t)(uint16_t)dv[j] << 16);
+    }
+    if (t == 0) einfo[gb] = make_uint4((uint32_t)dc | (frac << 8) | (1u << 12), nnz, eob, ncorr);
+}
+
+// ---------------------------------------------------------------------------
+// k_enc_entropy: one lane per block (blockIdx.y = chunk)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void encode_magnitude(RcEnc &e, uint32_t (&m)[12], uint32_t mag) {
+    uint32_t rest = mag - 1;
+    if (rest == 0) { e.bit(m[0], 0); return; }
+    bool done = false;
+    #pragma unroll
+    for (int k = 0; k < 11; ++k) {
+        if (!done) {
+            e.bit(m[k], 1);
+            rest--;
+            if (rest == 0) { e.bit(m[k + 1], 0); done = true; }
+        }
+    }
+    if (!done) { e.bit(m[11], 1); e.expgolomb(rest); }
+}
+
+__global__ __launch_bounds__(64) void k_enc_entropy(const EncChunkDesc *__restrict__ cd,
+                                                    const QTab *__restrict__ qtabs,
+                                                    const uint4 *__restrict__ einfo,
+                                                    const int16_t *__restrict__ levels,
+                                                    const uint32_t *__restrict__ corr,
+                                                    uint32_t *__restrict__ slots,
+                                                    uint32_t *__restrict__ plen) {
+    const int c = blockIdx.y;
+    const int bi = blockIdx.x * 64 + threadIdx.x;
+    const size_t gb = (size_t)c * kNVox + bi;
+    const uint4 inf = einfo[gb];
+    if (!(inf.x & (1u << 12))) { plen[gb] = 0; return; }
+    const QTab *qt = qtabs + cd[c].qtab;
+    const uint32_t dc = inf.x & 255u, eob = inf.z, ncorr = inf.w;
+    RcEnc e;
+    e.init(slots + gb * (kSlotBytes / 4), kSlotBytes);
+    e.bit_once(qt->flag[0], 0);
+    e.bit_once(qt->flag[1], ncorr > 0);
+    #pragma unroll
+    for (int b = 7; b >= 0; --b) e.bit_once(qt->dc[7 - b], (dc >> b) & 1);
+    // EOB: bit-length prefix (no terminator at 13), then the bits below the MSB
+    {
+        const int k = eob ? 32 - __clz(eob) : 0;
+        #pragma unroll
+        for (int i = 0; i < 13; ++i) {
+            if (i < k) e.bit_once(qt->eob[i], 1);
+            else if (i == k) e.bit_once(qt->eob[i], 0);
+        }
+        if (k > 1) e.bypass_n(eob & ((1u << (k - 1)) - 1u), k - 1);
+    }
+    uint32_t m[12];
+    #pragma unroll
+    for (int i = 0; i < 12; ++i) m[i] = qt->mag[i];
+    uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int band = -1;
+    uint32_t next_band = 0, hist = 0;
+    const int16_t *L = levels + gb * kNVox;
+    for (uint32_t p = 0; p < eob; ++p) {
+        if (p >= next_band) {
+            band++;
+            s0 = qt->sig[band * 4 + 0]; s1 = qt->sig[band * 4 + 1];
+            s2 = qt->sig[band * 4 + 2]; s3 = qt->sig[band * 4 + 3];
+            next_band = c_band_start[band + 1];
+        }
+        const int v = L[p];
+        const int nzb = v != 0;
+        if (p + 1 != eob) {
+            uint32_t dens = __popc(hist & 0xFFFFu);
+            uint32_t p0 = dens == 0 ? s0 : dens == 1 ? s1 : dens == 2 ? s2 : s3;
+            e.bit(p0, nzb);
+            s0 = dens == 0 ? p0 : s0;
+            s1 = dens == 1 ? p0 : s1;
+            s2 = dens == 2 ? p0 : s2;
+            s3 = dens >= 3 ? p0 : s3;
+        }
+        hist = (hist << 1) | (uint32_t)nzb;
+        if (nzb) {
+            encode_magnitude(e, m, (uint32_t)(v < 0 ? -v : v));
+            e.bypass(v < 0);
+        }
+    }
+    if (ncorr) {
+        const uint32_t *C = corr + gb * kNVox;
+        e.expgolomb(ncorr - 1);
+        uint32_t prev = 0;
+        for (uint32_t i = 0; i < ncorr; ++i) {
+            const uint32_t w = C[i];
+            const uint32_t pos = w & 0xFFFFu;
+            const int d = (int)(int16_t)(w >> 16);
+            e.expgolomb(pos - prev);
+            prev = pos;
+            e.bypass(d < 0);
+            e.expgolomb((uint32_t)(d < 0 ? -d : d) - 1u);
+        }
+    }
+    e.finish();
+    plen[gb] = e.n;
+}
+
+// ---------------------------------------------------------------------------
+// k_enc_assemble: one workgroup per chunk
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_enc_assemble(const EncChunkDesc *__restrict__ cd,
+                                                      const uint4 *__restrict__ einfo,
+                                                      const uint32_t *__restrict__ plen,
+                                                      uint32_t *__restrict__ dst,
+                                                      uint8_t *__restrict__ out,
+                                                      EncChunkResult *__restrict__ res) {
+    __shared__ uint8_t frac[kNVox];
+    __shared__ __attribute__((aligned(16))) uint32_t fmap_w[(kFmapCap + 16) / 4];
+    __shared__ uint32_t lds4[4];
+    __shared__ uint32_t s_fml;
+    const int c = blockIdx.x, t = threadIdx.x;
+    const size_t cb = (size_t)c * kNVox;
+    uint32_t bits = 0, lsum = 0, bad = 0;
+    uint32_t lj[16];
+    #pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int bi = t * 16 + j;
+        const uint4 inf = einfo[cb + bi];
+        const uint32_t present = (inf.x >> 12) & 1u;
+        frac[bi] = (uint8_t)((inf.x >> 8) & 15u);
+        lj[j] = present ? plen[cb + bi] : 0u;
+        bits |= present << j;
+        lsum += lj[j];
+        bad |= lj[j] > (uint32_t)kSlotBytes;
+    }
+    uint32_t np;
+    const uint32_t slot0 = wg_excl_scan(__popc(bits), lds4, np);
+    uint32_t paytot;
+    const uint32_t pbase = wg_excl_scan(lsum, lds4, paytot);
+    const uint32_t nbad = wg_sum(bad, lds4);
+    // fraction map: one lane, into LDS (src/mc_codec.c:600-611; the
+    // parallel-append stitch clamps its length to the buffer, :885-887)
+    if (t == 0) {
+        RcEnc e;
+        e.init(fmap_w, kFmapCap);
+        uint32_t cx[3][4];
+        for (int b = 0; b < 3; ++b)
+            for (int i = 0; i < 4; ++i) cx[b][i] = 2048;
+        int prev = 0;
+        for (int i = 0; i < kNVox; ++i) {
+            const int v = frac[i] & 15, pb = prev == 0 ? 0 : (prev == 15 ? 2 : 1);
+            for (int b = 3; b >= 0; --b) e.bit(cx[pb][b], (v >> b) & 1);
+            prev = v;
+        }
+        e.finish();
+        s_fml = e.n < (uint32_t)kFmapCap ? e.n : (uint32_t)kFmapCap;
+    }
+    __syncthreads();
+    const uint32_t fml = s_fml;
+    const uint64_t bm_off = 14ull + fml, lens_off = bm_off + 512, pay0 = lens_off + 2ull * np;
+    const uint64_t total = np ? pay0 + paytot : 0;
+    const bool fits = total <= cd[c].out_cap && nbad == 0;
+    if (t == 0) {
+        res[c].blob_len = total;
+        res[c].status = nbad ? kEncSlotOverflow : (total > cd[c].out_cap ? kEncOutTooSmall : 0u);
+        res[c].npresent = np;
+        res[c].pay0 = (uint32_t)pay0;
+    }
+    if (!np || !fits) return;
+    uint8_t *o = out + cd[c].out_off;
+    if (t == 0) {
+        const uint32_t qb = __float_as_uint(cd[c].q_store);
+        for (int i = 0; i < 4; ++i) o[i] = (uint8_t)(qb >> (8 * i));
+        for (int i = 4; i < 12; ++i) o[i] = 0;
+        o[12] = (uint8_t)fml;
+        o[13] = (uint8_t)(fml >> 8);
+    }
+    const uint8_t *fm = (const uint8_t *)fmap_w;
+    for (uint32_t i = t; i < fml; i += 256) o[14 + i] = fm[i];
+    o[bm_off + 2 * t] = (uint8_t)bits;
+    o[bm_off + 2 * t + 1] = (uint8_t)(bits >> 8);
+    uint32_t s = slot0, at = pbase;
+    #pragma unroll
+    for (int j = 0; j < 16; ++j)
+        if ((bits >> j) & 1) {
+            o[lens_off + 2ull * s] = (uint8_t)lj[j];
+            o[lens_off + 2ull * s + 1] = (uint8_t)(lj[j] >> 8);
+            dst[cb + t * 16 + j] = (uint32_t)pay0 + at;
+            at += lj[j];
+            s++;
+        }
+}
+
+// ---------------------------------------------------------------------------
+// k_enc_copy: payload of block (c, bi) from its slot to the blob
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_enc_copy(const EncChunkDesc *__restrict__ cd,
+                                                 const uint4 *__restrict__ einfo,
+                                                 const uint32_t *__restrict__ plen,
+                                                 const uint32_t *__restrict__ dst,
+                                                 const uint32_t *__restrict__ slots,
+                                                 const EncChunkResult *__restrict__ res,
+                                                 uint8_t *__restrict__ out) {
+    const uint32_t g = blockIdx.x;
+    const int c = (int)(g >> 12), bi = (int)(g & 4095);
+    const size_t gb = (size_t)c * kNVox + bi;
+    if (res[c].status != 0 || res[c].blob_len == 0) return;
+    if (!((einfo[gb].x >> 12) & 1u)) return;
+    const uint32_t n = plen[gb];
+    const uint8_t *s = (const uint8_t *)(slots + gb * (kSlotBytes / 4));
+    uint8_t *o = out + cd[c].out_off + dst[gb];
+    for (uint32_t i = threadIdx.x; i < n; i += 64) o[i] = s[i];
+}
+
+// ---------------------------------------------------------------------------
+// k_enc_hash: XXH64 chain, 4 lanes per chunk (one wave per chunk)
+// ---------------------------------------------------------------------------
+#define XP1 0x9E3779B185EBCA87ULL
+#define XP2 0xC2B2AE3D27D4EB4FULL
+#define XP3 0x165667B19E3779F9ULL
+#define XP4 0x85EBCA77C2B2AE63ULL
+#define XP5 0x27D4EB2F165667C5ULL
+__device__ __forceinline__ uint64_t rotl64(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
+__device__ __forceinline__ uint64_t xround(uint64_t acc, uint64_t in) { return rotl64(acc + in * XP2, 31) * XP1; }
+__device__ __forceinline__ uint64_t ld64u(const uint8_t *p) {
+    uint64_t v = 0;
+    #pragma unroll
+    for (int i = 7; i >= 0; --i) v = (v << 8) | p[i];
+    return v;
+}
+__device__ __forceinline__ uint32_t ld32u(const uint8_t *p) {
+    return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+}
+// Unaligned 64-bit little-endian load from three aligned dwords.
+__device__ __forceinline__ uint64_t ld64a(const uint8_t *p) {
+    const uintptr_t a = (uintptr_t)p;
+    const uint32_t *w = (const uint32_t *)(a & ~(uintptr_t)3);
+    const uint32_t sh = (uint32_t)(a & 3) * 8;
+    const uint32_t w0 = w[0], w1 = w[1];
+    if (!sh) return (uint64_t)w0 | ((uint64_t)w1 << 32);
+    const uint32_t w2 = w[2];
+    const uint32_t lo = (w0 >> sh) | (w1 << (32 - sh));
+    const uint32_t hi = (w1 >> sh) | (w2 << (32 - sh));
+    return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
+// Finish an XXH64 after the stripe phase: p points at the unconsumed tail.
+__device__ uint64_t xxh_tail(uint64_t h, const uint8_t *p, const uint8_t *end) {
+    for (; p + 8 <= end; p += 8) { h ^= xround(0, ld64u(p)); h = rotl64(h, 27) * XP1 + XP4; }
+    if (p + 4 <= end) { h ^= (uint64_t)ld32u(p) * XP1; h = rotl64(h, 23) * XP2 + XP3; p += 4; }
+    for (; p < end; ++p) { h ^= (uint64_t)(*p) * XP5; h = rotl64(h, 11) * XP1; }
+    h ^= h >> 33; h *= XP2; h ^= h >> 29; h *= XP3; h ^= h >> 32;
+    return h;
+}
+
+// XXH64 of [p, p+len) computed by lanes 0..3 of the calling wave; every lane
+// of the wave must call it; the result is valid in lane 0.
+__device__ uint64_t xxh64_wave4(const uint8_t *p, uint64_t len, uint64_t seed) {
+    const int lane = threadIdx.x & 63;
+    uint64_t h = 0;
+    const uint64_t nstripe = len >= 32 ? len / 32 : 0;
+    uint64_t v = 0;
+    if (lane < 4) {
+        v = lane == 0 ? seed + XP1 + XP2 : lane == 1 ? seed + XP2 : lane == 2 ? seed : seed - XP1;
+        const uint8_t *q = p + 8 * lane;
+        for (uint64_t s = 0; s < nstripe; ++s) v = xround(v, ld64a(q + 32 * s));
+    }
+    const uint64_t v1 = __shfl(v, 0, 64), v2 = __shfl(v, 1, 64), v3 = __shfl(v, 2, 64), v4 = __shfl(v, 3, 64);
+    if (lane == 0) {
+        if (len >= 32) {
+            h = rotl64(v1, 1) + rotl64(v2, 7) + rotl64(v3, 12) + rotl64(v4, 18);
+            const uint64_t vv[4] = {v1, v2, v3, v4};
+            for (int i = 0; i < 4; ++i) { h ^= xround(0, vv[i]); h = h * XP1 + XP4; }
+        } else {
+            h = seed + XP5;
+        }
+        h += len;
+        h = xxh_tail(h, p + 32 * nstripe, p + len);
+    }
+    return h;
+}
+
+__global__ __launch_bounds__(64) void k_enc_hash(const EncChunkDesc *__restrict__ cd,
+                                                 const EncChunkResult *__restrict__ res,
+                                                 uint8_t *__restrict__ out) {
+    const int c = blockIdx.x;
+    if (res[c].status != 0 || res[c].blob_len == 0) return;
+    uint8_t *o = out + cd[c].out_off;
+    const uint64_t total = res[c].blob_len, pay0 = res[c].pay0, np = res[c].npresent;
+    const uint32_t fml = (uint32_t)o[12] | ((uint32_t)o[13] << 8);
+    uint64_t h = xxh64_wave4(o + 14, fml, 0x6D636368756E6BULL);
+    h = __shfl(h, 0, 64);
+    h ^= xxh64_wave4(o + 14 + fml, 512, h);
+    h = __shfl(h, 0, 64);
+    h ^= xxh64_wave4(o + 14 + fml + 512, 2 * np, h);
+    h = __shfl(h, 0, 64);
+    h ^= xxh64_wave4(o + pay0, total - pay0, h);
+    if (threadIdx.x == 0)
+        for (int i = 0; i < 8; ++i) o[4 + i] = (uint8_t)(h >> (8 * i));
+}
+
+// ---------------------------------------------------------------------------
+void launch_enc_transform(const EncodeArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_enc_transform, dim3((unsigned)a.n * kNVox), dim3(256), 0, s, a.in, a.cd, a.qtabs, a.einfo,
+                       a.levels, a.corr, a.n);
+}
+void launch_enc_entropy(const EncodeArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_enc_entropy, dim3(kNVox / 64, a.n), dim3(64), 0, s, a.cd, a.qtabs, a.einfo, a.levels,
+                       a.corr, a.slots, a.plen);
+}
+void launch_enc_assemble(const EncodeArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_enc_assemble, dim3(a.n), dim3(256), 0, s, a.cd, a.einfo, a.plen, a.dst, a.out, a.res);
+}
+void launch_enc_copy(const EncodeArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_enc_copy, dim3((unsigned)a.n * kNVox), dim3(64), 0, s, a.cd, a.einfo, a.plen, a.dst,
+                       a.slots, a.res, a.out);
+}
+void launch_enc_hash(const EncodeArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_enc_hash, dim3(a.n), dim3(64), 0, s, a.cd, a.res, a.out);
+}
+
+}  // namespace mchip

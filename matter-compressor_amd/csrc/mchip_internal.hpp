@@ -45,6 +45,51 @@ struct DecodeArgs {
     int n;
 };
 
+// ---- encode ----------------------------------------------------------------
+constexpr int kSlotBytes = 17408;          // per-block payload cap (src/mc_codec.c:442, :732)
+constexpr int kFmapCap = 4096 / 2 + 64;    // fraction-map buffer (src/mc_archive.c:884-887)
+constexpr uint32_t kEncSlotOverflow = 1;   // a block payload exceeded kSlotBytes
+constexpr uint32_t kEncOutTooSmall = 2;    // blob longer than the caller's capacity
+
+struct EncChunkDesc {
+    uint64_t in_off;        // byte offset of the 256^3 raster (16-byte aligned)
+    uint64_t out_off;       // byte offset of the blob in the output buffer
+    uint64_t out_cap;       // bytes available for the blob
+    uint32_t qtab;          // quality table slot (sanitised q)
+    int32_t tau;            // max-error bound, 0 = no corrections
+    float q_store;          // quality written into the blob header
+    uint32_t pad;
+};
+
+struct EncChunkResult {
+    uint64_t blob_len;      // 0 = all-air chunk (no blob)
+    uint32_t status;
+    uint32_t npresent;
+    uint32_t pay0;
+    uint32_t pad;
+};
+
+struct EncodeArgs {
+    const uint8_t *in;
+    const EncChunkDesc *cd;
+    const QTab *qtabs;
+    uint4 *einfo;           // [n*4096]
+    int16_t *levels;        // [n*4096*4096] scan-ordered levels
+    uint32_t *corr;         // [n*4096*4096] corrections (pos | delta << 16)
+    uint32_t *slots;        // [n*4096*kSlotBytes/4]
+    uint32_t *plen;         // [n*4096]
+    uint32_t *dst;          // [n*4096] payload offset in the blob
+    uint8_t *out;
+    EncChunkResult *res;    // [n]
+    int n;
+};
+
+void launch_enc_transform(const EncodeArgs &a, hipStream_t s);
+void launch_enc_entropy(const EncodeArgs &a, hipStream_t s);
+void launch_enc_assemble(const EncodeArgs &a, hipStream_t s);
+void launch_enc_copy(const EncodeArgs &a, hipStream_t s);
+void launch_enc_hash(const EncodeArgs &a, hipStream_t s);
+
 // Scratch words a chunk needs: 8 per payload byte + 32 per block.
 inline uint64_t scratch_words_for(uint64_t blob_len) { return 8ull * blob_len + 32ull * 4096ull; }
 

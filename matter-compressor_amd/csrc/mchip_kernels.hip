@@ -1,0 +1,5 @@
+// mchip_kernels.hip -- the single device translation unit: the decode and
+// encode kernels share the constant tables (DCT matrix, scan order, band
+// starts) uploaded once by mchip_init.
+#include "mchip_decode.hip"
+#include "mchip_encode.hip"

@@ -51,6 +51,8 @@ def load():
         "mchip_decode_chunks": (i, [vp, u8p, vp, vp, vp, i, u8p, C.c_uint64]),
         "mchip_decode_chunks_host": (i, [vp, vp, vp, i, u8p, C.c_uint64]),
         "mchip_decode_blocks_host": (i, [vp, vp, vp, vp, i, u8p]),
+        "mchip_encode_chunks": (i, [vp, u8p, C.c_uint64, vp, vp, vp, i, u8p, vp, vp, vp, vp]),
+        "mchip_encode_chunks_host": (i, [vp, vp, vp, vp, vp, i, vp, vp, vp, vp]),
         "mchip_ctx_enable_timing": (i, [vp, i]),
         "mchip_ctx_kernel_time": (i, [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)]),
         "mchip_ctx_reset_timing": (i, [vp]),
@@ -167,6 +169,49 @@ class Ctx:
         qs = np.ascontiguousarray(qs, np.float32)
         out = np.empty((n, 4096), np.uint8)
         check(load().mchip_decode_blocks_host(self.h, _ptr_array(arrs), _ptr(lens), _ptr(qs), n, _ptr(out)))
+        return out
+
+
+    # ---- encode ----------------------------------------------------------
+    def encode_chunks(self, d_in, qs, taus, d_out, out_offs, out_caps, q_store=None, in_stride=CHUNK_BYTES):
+        """Device-resident encode; returns (blob_len[n], status[n])."""
+        n = len(qs)
+        qs = np.ascontiguousarray(qs, np.float32)
+        taus = np.ascontiguousarray(taus, np.int32)
+        offs = np.ascontiguousarray(out_offs, np.uint64)
+        caps = np.ascontiguousarray(out_caps, np.uint64)
+        qst = None if q_store is None else np.ascontiguousarray(q_store, np.float32)
+        lens = np.zeros(n, np.uint64)
+        st = np.zeros(n, np.int32)
+        check(load().mchip_encode_chunks(self.h, C.c_void_p(d_in.ptr), in_stride, _ptr(qs), _ptr(taus),
+                                         None if qst is None else _ptr(qst), n, C.c_void_p(d_out.ptr),
+                                         _ptr(offs), _ptr(caps), _ptr(lens), _ptr(st)))
+        return lens, st
+
+    def encode_chunks_host(self, chunks, qs, taus=None, q_store=None, cap=20 << 20):
+        """Encode host chunks [n,256,256,256]; returns list of blob bytes."""
+        chunks = np.ascontiguousarray(chunks, np.uint8)
+        n = chunks.shape[0]
+        qs = np.ascontiguousarray(qs, np.float32)
+        taus = np.zeros(n, np.int32) if taus is None else np.ascontiguousarray(taus, np.int32)
+        qst = None if q_store is None else np.ascontiguousarray(q_store, np.float32)
+        bufs = [np.empty(cap, np.uint8) for _ in range(n)]
+        caps = np.full(n, cap, np.uint64)
+        lens = np.zeros(n, np.uint64)
+        st = np.zeros(n, np.int32)
+        check(load().mchip_encode_chunks_host(self.h, _ptr(chunks), _ptr(qs), _ptr(taus),
+                                              None if qst is None else _ptr(qst), n, _ptr_array(bufs),
+                                              _ptr(caps), _ptr(lens), _ptr(st)))
+        out = []
+        for k in range(n):
+            if st[k] == 2:          # capacity too small: retry this chunk with the reported length
+                out.append(self.encode_chunks_host(chunks[k:k + 1], qs[k:k + 1], taus[k:k + 1],
+                                                   None if qst is None else qst[k:k + 1],
+                                                   cap=int(lens[k]) + 4096)[0])
+                continue
+            if st[k] != 0:
+                raise MchipError(f"encode of chunk {k} failed with status {st[k]}")
+            out.append(bufs[k][: int(lens[k])].tobytes())
         return out
 
 
