@@ -1,0 +1,120 @@
+/*
+ * matter_compressor_amd.h -- drop-in C API of the matter-compressor codec and
+ * archive, implemented for one MI355X (gfx950).
+ *
+ * Every declaration below keeps the name, signature, argument meaning and
+ * error behaviour of the reference declaration it replaces
+ * (SuperOptimizer/matter-compressor src/matter_compressor.h; line ranges are
+ * given per entry).  Block and chunk coding run on the GPU through the mchip
+ * engine (include/mchip.h); the archive file format (v7 .mca) and its
+ * index, install and metadata logic are plain C on the host.  A program
+ * built against the reference's codec/archive section links against
+ * libmc_amd.so instead, unchanged.
+ */
+#ifndef MATTER_COMPRESSOR_AMD_H
+#define MATTER_COMPRESSOR_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef uint8_t mc_u8;
+typedef int32_t mc_i32;
+
+#define MC_BLK 16           /* block edge */
+#define MC_CHUNK 256        /* chunk edge (16 blocks) */
+#define MC_DZ_FRAC 0.80f    /* dead-zone width fraction */
+#define MC_HF_EXP 0.65f     /* step = quality * (1 + L1)^MC_HF_EXP */
+#define MC_BLOCKCODEC_CABAC 0u
+
+/* ---- codec context (src/matter_compressor.h:36-59) ----------------------- */
+typedef struct mc_codec_ctx mc_codec_ctx;
+mc_codec_ctx *mc_codec_ctx_new(void);                                /* :42 */
+void  mc_codec_ctx_free(mc_codec_ctx *ctx);                          /* :43 */
+void  mc_codec_ctx_set_quality(mc_codec_ctx *ctx, float quality);    /* :44 */
+float mc_codec_ctx_get_quality(mc_codec_ctx *ctx);                   /* :45 */
+void  mc_codec_ctx_set_max_error(mc_codec_ctx *ctx, int tau);        /* :58 */
+int   mc_codec_ctx_get_max_error(mc_codec_ctx *ctx);                 /* :59 */
+
+/* ---- preset ladder (src/matter_compressor.h:74-92) ------------------------ */
+typedef enum {
+    MC_PRESET_ARCHIVAL = 0, MC_PRESET_MASTER = 1, MC_PRESET_HIGH = 2, MC_PRESET_BALANCED = 3,
+    MC_PRESET_STREAMING = 4, MC_PRESET_FAST = 5, MC_PRESET_ULTRAFAST = 6, MC_PRESET_PREVIEW = 7,
+    MC_PRESET_COUNT = 8,
+} mc_preset;
+float       mc_apply_preset(mc_codec_ctx *ctx, mc_preset level);    /* :88 */
+float       mc_preset_quality(mc_preset level);                     /* :89 */
+int         mc_preset_tau(mc_preset level);                         /* :90 */
+const char *mc_preset_name(mc_preset level);                        /* :91 */
+void  mc_codec_init(void);                                          /* :92 */
+void  mc_codec_set_priors(const uint16_t *plo, const uint16_t *phi); /* :96 */
+
+/* ---- buffers and block codec (src/matter_compressor.h:100-118) ----------- */
+typedef struct { mc_u8 *p; size_t len, cap; } mc_buf;
+void  mc_buf_put(mc_buf *b, const void *s, size_t n);               /* :100 */
+/* encode one 16^3 block: appends the payload to out; returns 1 if coded, 0
+ * for an all-zero block (no payload).  Runs on the GPU. (:111) */
+int   mc_enc_block(mc_codec_ctx *ctx, const mc_u8 *vox, mc_buf *out, uint32_t *len_out);
+/* decode one block payload into dst (16^3).  Runs on the GPU. (:114) */
+void  mc_dec_block(mc_codec_ctx *ctx, const mc_u8 *payload, uint32_t plen, mc_u8 *dst);
+/* per-chunk material-fraction map (:117-118) */
+uint32_t mc_enc_fracmap(const mc_u8 *frac, mc_u8 *out, size_t cap);
+void     mc_dec_fracmap(const mc_u8 *in, uint32_t len, mc_u8 *frac);
+
+/* ---- archive (src/matter_compressor.h:166-312, :364-379) ------------------ */
+typedef struct mc_archive mc_archive;
+typedef enum { MC_ABSENT = 0, MC_PRESENT = 1, MC_ZERO = 2 } mc_cover;
+typedef struct { long z0, y0, x0; } mc_box;
+
+mc_archive *mc_archive_open(const char *path, int dim, float quality);                  /* :177 */
+mc_archive *mc_archive_open_dims(const char *path, int nx, int ny, int nz, float quality); /* :180 */
+int      mc_archive_set_block_codec(mc_archive *a, uint32_t codec);                     /* :188 */
+uint32_t mc_archive_block_codec(const mc_archive *a);                                   /* :189 */
+int mc_archive_block_blob(mc_archive *a, uint64_t chunk_off, int bz, int by, int bx,
+                          const uint8_t **ptr, uint32_t *len);                           /* :194 */
+int mc_archive_append_chunk_raw(mc_archive *a, int lod, int cz, int cy, int cx,
+                                const mc_u8 vox[256 * 256 * 256]);                       /* :201 */
+int mc_archive_append_chunk_raw_q(mc_archive *a, int lod, int cz, int cy, int cx,
+                                  const mc_u8 vox[256 * 256 * 256], float q);            /* :206 */
+int mc_archive_append_chunk_ctx(mc_archive *a, mc_codec_ctx *C, int lod, int cz, int cy, int cx,
+                                const mc_u8 vox[256 * 256 * 256]);                       /* :212 */
+int mc_archive_append_chunk_compressed(mc_archive *a, int lod, int cz, int cy, int cx,
+                                       const uint8_t *blob, size_t len);                 /* :227 */
+int mc_archive_reserve_index(mc_archive *a, int lod, int cz, int cy, int cx);           /* :235 */
+mc_cover mc_archive_chunk_coverage(mc_archive *a, int lod, int cz, int cy, int cx);     /* :238 */
+uint64_t mc_archive_data_len(mc_archive *a);                                            /* :243 */
+int mc_archive_set_metadata(mc_archive *a, const void *data, size_t len);               /* :250 */
+const char *mc_archive_metadata(mc_archive *a, size_t *out_len);                        /* :251 */
+uint64_t mc_archive_chunk_offset(mc_archive *a, int lod, int cz, int cy, int cx);       /* :255 */
+void mc_archive_decode_block(mc_archive *a, uint64_t chunk_off, int bz, int by, int bx,
+                             mc_u8 *dst);                                                /* :260 */
+/* decode a whole 256^3 chunk (z,y,x raster); nthreads is accepted and
+ * ignored: the work runs on the GPU.  chunk_off <= 1 zero-fills. (:265) */
+void mc_archive_decode_chunk(mc_archive *a, uint64_t chunk_off, mc_u8 *out, int nthreads);
+/* encode a chunk at q (q <= 0 -> the archive's quality) with tau = 0 and
+ * install the blob; an all-air chunk is left absent.  0 on success. (:269) */
+int mc_archive_append_chunk_par(mc_archive *a, int lod, int cz, int cy, int cx,
+                                const mc_u8 vox[256 * 256 * 256], float q, int nthreads);
+void mc_archive_read_region(mc_archive *a, int lod, long z0, long y0, long x0,
+                            long dz, long dy, long dx, mc_u8 *out, size_t sz, size_t sy,
+                            int nthreads);                                               /* :279 */
+int   mc_archive_block_present(mc_archive *a, int lod, int bz, int by, int bx);         /* :287 */
+float mc_archive_block_fraction(mc_archive *a, int lod, int bz, int by, int bx);        /* :292 */
+int mc_archive_sample_boxes(mc_archive *a, int lod, uint64_t seed, int count,
+                            long dz, long dy, long dx, float min_frac, mc_box *out);      /* :300 */
+void mc_archive_read_regions(mc_archive *a, int lod, const mc_box *boxes, int n,
+                             long dz, long dy, long dx, mc_u8 *out, size_t batch_stride,
+                             int nthreads);                                              /* :307 */
+void mc_archive_close(mc_archive *a);                                                   /* :312 */
+const char *mc_metadata(const uint8_t *arc, size_t *out_len);                           /* :364 */
+uint64_t mc_chunk_compute_hash(const uint8_t *blob, uint64_t blob_len);                 /* :369 */
+long mc_verify_archive(const uint8_t *arc, size_t len, int verbose);                    /* :370 */
+int mc_archive_set_priors(struct mc_archive *a, const uint16_t plo[8][32],
+                          const uint16_t phi[8][32]);                                   /* :377 */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -96,6 +96,40 @@ int mchip_encode_chunks_host(mchip_ctx *ctx, const uint8_t *chunks, const float 
                              const float *q_store, int n, uint8_t *const *blobs, const uint64_t *caps,
                              uint64_t *blob_len, int *status);
 
+/* Encode n independent 16^3 blocks (host memory, block i at vox + 4096*i,
+ * z,y,x) at (q[i], tau[i]) (tau may be NULL).  payloads[i] receives up to
+ * 17408 bytes; lens[i] = payload length (0 = all-zero block); status[i]
+ * (may be NULL) = 1 if the payload exceeded 17408 bytes.  Synchronous. */
+int mchip_encode_blocks_host(mchip_ctx *ctx, const uint8_t *vox, const float *q, const int *tau, int n,
+                             uint8_t *const *payloads, uint32_t *lens, int *status);
+
+/* Decode n block payloads resident in device memory into 4096-byte slots of
+ * a device arena: payload i at d_pay + pay_off[i] (pay_len[i] bytes,
+ * quality q[i]) becomes the 16^3 block at d_arena + 4096*slot[i].
+ * Asynchronous; host arrays are consumed before the call returns. */
+int mchip_decode_block_jobs(mchip_ctx *ctx, const uint8_t *d_pay, const uint64_t *pay_off,
+                            const uint32_t *pay_len, const float *q, const uint32_t *slot, int n,
+                            uint8_t *d_arena);
+
+/* Assemble n boxes of dz x dy x dx voxels from arena blocks into
+ * d_out + i*batch_stride (dense z,y,x).  Box i has origin[3i..3i+2] =
+ * (z0,y0,x0); its touched block grid starts at block origin>>4 and holds
+ * gz x gy x gx cells in bmap[i*gz*gy*gx ...] (z,y,x order), each an arena
+ * slot or -1 for zeros.  Asynchronous. */
+int mchip_crop_boxes(mchip_ctx *ctx, const uint8_t *d_arena, const int32_t *bmap, const int64_t *origin,
+                     int n, int gz, int gy, int gx, int64_t dz, int64_t dy, int64_t dx, uint8_t *d_out,
+                     uint64_t batch_stride);
+
+/* Host-memory region read: the nblk payloads (host pointers) are decoded
+ * into slots 0..nblk-1 of a device arena, the n boxes are assembled on the
+ * device (as mchip_crop_boxes) and copied to out + i*batch_stride.
+ * Synchronous.  Replaces the worker pool of mc_archive_read_regions
+ * (src/mc_archive.c:950-1042). */
+int mchip_read_regions_host(mchip_ctx *ctx, int nblk, const uint8_t *const *payloads, const uint32_t *lens,
+                            const float *q, const int32_t *bmap, const int64_t *origin, int n, int gz,
+                            int gy, int gx, int64_t dz, int64_t dy, int64_t dx, uint8_t *out,
+                            uint64_t batch_stride);
+
 /* Kernel timing: when enabled, every kernel launched on the context is
  * bracketed by HIP events on the context's stream; stats accumulate per
  * kernel name ("index", "entropy", "recon", ...). */

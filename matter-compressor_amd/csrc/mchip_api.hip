@@ -207,6 +207,15 @@ struct mchip_ctx {
     void *d_dst = nullptr; size_t cap_dst = 0;
     void *d_res = nullptr; size_t cap_res = 0;
     void *h_res = nullptr;
+    // block jobs / crops
+    void *d_jobs = nullptr; size_t cap_jobs = 0;
+    void *d_jscratch = nullptr; size_t cap_jscratch = 0;
+    void *d_jinfo = nullptr; size_t cap_jinfo = 0;
+    void *d_bmap = nullptr; size_t cap_bmap = 0;
+    void *d_arena = nullptr; size_t cap_arena = 0;
+    void *h_crop = nullptr; size_t cap_h_crop = 0;
+    hipEvent_t ev_crop = nullptr;
+    bool crop_pending = false;
     // pinned descriptor staging, reusable once ev_desc has completed
     void *h_desc = nullptr; size_t cap_h_desc = 0;
     hipEvent_t ev_desc = nullptr;
@@ -315,7 +324,8 @@ mchip_ctx *mchip_ctx_create(void) {
     if (mchip_init(-1)) return nullptr;
     mchip_ctx *c = new mchip_ctx();
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_desc, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->ev_desc, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_crop, hipEventDisableTiming) != hipSuccess) {
         fail("stream/event creation failed");
         delete c;
         return nullptr;
@@ -325,18 +335,21 @@ mchip_ctx *mchip_ctx_create(void) {
 
 void mchip_ctx_destroy(mchip_ctx *c) {
     if (!c) return;
-    hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->stream);
     collect_timing(c);
     void *bufs[] = {c->d_cd, c->d_rec, c->d_plist, c->d_cinfo, c->d_binfo, c->d_scratch,
                     c->d_stage_in, c->d_stage_out, c->d_aux, c->d_ecd, c->d_einfo, c->d_levels,
-                    c->d_corr, c->d_slots, c->d_plen, c->d_dst, c->d_res};
+                    c->d_corr, c->d_slots, c->d_plen, c->d_dst, c->d_res,
+                    c->d_jobs, c->d_jscratch, c->d_jinfo, c->d_bmap, c->d_arena};
     for (void *p : bufs)
-        if (p) hipFree(p);
-    if (c->h_desc) hipHostFree(c->h_desc);
-    if (c->h_res) hipHostFree(c->h_res);
-    for (hipEvent_t e : c->free_events) hipEventDestroy(e);
-    hipEventDestroy(c->ev_desc);
-    hipStreamDestroy(c->stream);
+        if (p) (void)hipFree(p);
+    if (c->h_desc) (void)hipHostFree(c->h_desc);
+    if (c->h_res) (void)hipHostFree(c->h_res);
+    if (c->h_crop) (void)hipHostFree(c->h_crop);
+    if (c->ev_crop) (void)hipEventDestroy(c->ev_crop);
+    for (hipEvent_t e : c->free_events) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(c->ev_desc);
+    (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -354,7 +367,7 @@ void *mchip_malloc(size_t bytes) {
     if (hipMalloc(&p, bytes + 256) != hipSuccess) { fail("hipMalloc(%zu) failed", bytes); return nullptr; }
     return p;
 }
-void mchip_free(void *p) { if (p) hipFree(p); }
+void mchip_free(void *p) { if (p) (void)hipFree(p); }
 
 int mchip_h2d(mchip_ctx *c, void *dst, const void *src, size_t n) {
     HIPCHK(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, c->stream));
@@ -602,6 +615,162 @@ int mchip_encode_chunks_host(mchip_ctx *c, const uint8_t *chunks, const float *q
     if (status)
         for (int i = 0; i < n; ++i) status[i] = st[i];
     return 0;
+}
+
+int mchip_encode_blocks_host(mchip_ctx *c, const uint8_t *vox, const float *q, const int *tau, int n,
+                             uint8_t *const *payloads, uint32_t *lens, int *status) {
+    if (!c) return fail("null context");
+    if (n <= 0) return 0;
+    if (!vox || !q || !payloads || !lens) return fail("null argument");
+    const int kMax = kEncBatch * kNVox;
+    for (int b0 = 0; b0 < n; b0 += kMax) {
+        const int m = n - b0 < kMax ? n - b0 : kMax;
+        void *h = nullptr;
+        if (desc_buffer(c, sizeof(EncChunkDesc) * (size_t)m, &h)) return -1;
+        EncChunkDesc *cd = (EncChunkDesc *)h;
+        for (int i = 0; i < m; ++i) {
+            uint32_t slot;
+            if (qtab_slot(q[b0 + i], &slot)) return -1;
+            cd[i] = EncChunkDesc{0, 0, 0, slot, tau ? (tau[b0 + i] < 0 ? 0 : tau[b0 + i]) : 0, q[b0 + i], 0};
+        }
+        if (ensure(c, &c->d_ecd, &c->cap_ecd, sizeof(EncChunkDesc) * m) ||
+            ensure(c, &c->d_einfo, &c->cap_einfo, sizeof(uint4) * m) ||
+            ensure(c, &c->d_levels, &c->cap_levels, sizeof(int16_t) * (size_t)m * kNVox) ||
+            ensure(c, &c->d_corr, &c->cap_corr, sizeof(uint32_t) * (size_t)m * kNVox) ||
+            ensure(c, &c->d_slots, &c->cap_slots, (size_t)kSlotBytes * m) ||
+            ensure(c, &c->d_plen, &c->cap_plen, sizeof(uint32_t) * m) ||
+            ensure(c, &c->d_stage_in, &c->cap_stage_in, (size_t)m * kNVox + 256))
+            return -1;
+        HIPCHK(hipMemcpyAsync(c->d_stage_in, vox + (size_t)b0 * kNVox, (size_t)m * kNVox, hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(c->d_ecd, cd, sizeof(EncChunkDesc) * m, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipEventRecord(c->ev_desc, c->stream));
+        c->desc_pending = true;
+        EncodeArgs a{};
+        a.in = (const uint8_t *)c->d_stage_in;
+        a.cd = (const EncChunkDesc *)c->d_ecd;
+        a.qtabs = G().d_qtabs;
+        a.einfo = (uint4 *)c->d_einfo;
+        a.levels = (int16_t *)c->d_levels;
+        a.corr = (uint32_t *)c->d_corr;
+        a.slots = (uint32_t *)c->d_slots;
+        a.plen = (uint32_t *)c->d_plen;
+        a.n = m;
+        timed(c, "enc_blocks", [&] { launch_enc_blocks(a, c->stream); });
+        HIPCHK(hipGetLastError());
+        std::vector<uint32_t> pl(m);
+        HIPCHK(hipMemcpyAsync(pl.data(), c->d_plen, sizeof(uint32_t) * m, hipMemcpyDeviceToHost, c->stream));
+        if (mchip_ctx_sync(c)) return -1;
+        for (int i = 0; i < m; ++i) {
+            const uint32_t len = pl[i];
+            lens[b0 + i] = len;
+            if (status) status[b0 + i] = len > (uint32_t)kSlotBytes ? 1 : 0;
+            if (len && len <= (uint32_t)kSlotBytes)
+                HIPCHK(hipMemcpyAsync(payloads[b0 + i], (const uint8_t *)c->d_slots + (size_t)i * kSlotBytes, len,
+                                      hipMemcpyDeviceToHost, c->stream));
+        }
+        if (mchip_ctx_sync(c)) return -1;
+    }
+    return 0;
+}
+
+int mchip_decode_block_jobs(mchip_ctx *c, const uint8_t *d_pay, const uint64_t *pay_off, const uint32_t *pay_len,
+                            const float *q, const uint32_t *slot, int n, uint8_t *d_arena) {
+    if (!c) return fail("null context");
+    if (n <= 0) return 0;
+    if (!d_pay || !pay_off || !pay_len || !q || !slot || !d_arena) return fail("null argument");
+    void *h = nullptr;
+    if (desc_buffer(c, sizeof(BlockJob) * (size_t)n, &h)) return -1;
+    BlockJob *jb = (BlockJob *)h;
+    uint64_t sw = 0;
+    for (int i = 0; i < n; ++i) {
+        uint32_t qs;
+        if (qtab_slot(q[i], &qs)) return -1;
+        jb[i] = BlockJob{pay_off[i], sw, pay_len[i], qs, slot[i], 0};
+        sw += 8ull * pay_len[i] + 32ull;
+    }
+    if (ensure(c, &c->d_jobs, &c->cap_jobs, sizeof(BlockJob) * n) ||
+        ensure(c, &c->d_jscratch, &c->cap_jscratch, sizeof(uint32_t) * sw) ||
+        ensure(c, &c->d_jinfo, &c->cap_jinfo, sizeof(uint2) * n))
+        return -1;
+    HIPCHK(hipMemcpyAsync(c->d_jobs, jb, sizeof(BlockJob) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->ev_desc, c->stream));
+    c->desc_pending = true;
+    timed(c, "block_jobs", [&] {
+        launch_jobs(d_pay, (const BlockJob *)c->d_jobs, n, G().d_qtabs, (uint32_t *)c->d_jscratch,
+                    (uint2 *)c->d_jinfo, d_arena, c->stream);
+    });
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mchip_crop_boxes(mchip_ctx *c, const uint8_t *d_arena, const int32_t *bmap, const int64_t *origin, int n, int gz,
+                     int gy, int gx, int64_t dz, int64_t dy, int64_t dx, uint8_t *d_out, uint64_t batch_stride) {
+    if (!c) return fail("null context");
+    if (n <= 0) return 0;
+    if (!d_arena || !bmap || !origin || !d_out || gz <= 0 || gy <= 0 || gx <= 0 || dz <= 0 || dy <= 0 || dx <= 0)
+        return fail("bad argument");
+    const size_t cells = (size_t)n * gz * gy * gx;
+    const size_t bytes = cells * 4 + (size_t)n * 24;
+    if (c->crop_pending) { HIPCHK(hipEventSynchronize(c->ev_crop)); c->crop_pending = false; }
+    if (bytes > c->cap_h_crop) {
+        if (c->h_crop) HIPCHK(hipHostFree(c->h_crop));
+        c->h_crop = nullptr;
+        HIPCHK(hipHostMalloc(&c->h_crop, bytes + bytes / 2 + 4096, hipHostMallocDefault));
+        c->cap_h_crop = bytes + bytes / 2 + 4096;
+    }
+    memcpy(c->h_crop, origin, (size_t)n * 24);
+    memcpy((uint8_t *)c->h_crop + (size_t)n * 24, bmap, cells * 4);
+    if (ensure(c, &c->d_bmap, &c->cap_bmap, bytes)) return -1;
+    HIPCHK(hipMemcpyAsync(c->d_bmap, c->h_crop, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->ev_crop, c->stream));
+    c->crop_pending = true;
+    const int64_t *dorig = (const int64_t *)c->d_bmap;
+    const int32_t *dbm = (const int32_t *)((const uint8_t *)c->d_bmap + (size_t)n * 24);
+    timed(c, "crop", [&] {
+        launch_crop(d_arena, dbm, dorig, n, gz, gy, gx, dz, dy, dx, d_out, batch_stride, c->stream);
+    });
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mchip_read_regions_host(mchip_ctx *c, int nblk, const uint8_t *const *payloads, const uint32_t *lens,
+                            const float *q, const int32_t *bmap, const int64_t *origin, int n, int gz, int gy,
+                            int gx, int64_t dz, int64_t dy, int64_t dx, uint8_t *out, uint64_t batch_stride) {
+    if (!c) return fail("null context");
+    if (n <= 0) return 0;
+    std::vector<uint64_t> po(nblk > 0 ? nblk : 1);
+    std::vector<uint32_t> sl(nblk > 0 ? nblk : 1);
+    uint64_t tot = 0;
+    for (int i = 0; i < nblk; ++i) { po[i] = tot; sl[i] = (uint32_t)i; tot += ((uint64_t)lens[i] + 15) & ~15ull; }
+    const uint64_t box = (uint64_t)dz * dy * dx;
+    if (ensure(c, &c->d_stage_in, &c->cap_stage_in, tot + 256) ||
+        ensure(c, &c->d_arena, &c->cap_arena, (size_t)(nblk > 0 ? nblk : 1) * kNVox) ||
+        ensure(c, &c->d_stage_out, &c->cap_stage_out, box * n))
+        return -1;
+    uint8_t *din = (uint8_t *)c->d_stage_in;
+    // gather payloads into one pinned upload
+    if (tot) {
+        void *h = nullptr;
+        if (desc_buffer(c, tot, &h)) return -1;
+        for (int i = 0; i < nblk; ++i) memcpy((uint8_t *)h + po[i], payloads[i], lens[i]);
+        HIPCHK(hipMemcpyAsync(din, h, tot, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipEventRecord(c->ev_desc, c->stream));
+        c->desc_pending = true;
+    }
+    if (nblk > 0 && mchip_decode_block_jobs(c, din, po.data(), lens, q, sl.data(), nblk, (uint8_t *)c->d_arena))
+        return -1;
+    if (mchip_crop_boxes(c, (const uint8_t *)c->d_arena, bmap, origin, n, gz, gy, gx, dz, dy, dx,
+                         (uint8_t *)c->d_stage_out, box))
+        return -1;
+    if (batch_stride == box) {
+        HIPCHK(hipMemcpyAsync(out, c->d_stage_out, box * n, hipMemcpyDeviceToHost, c->stream));
+    } else {
+        for (int i = 0; i < n; ++i)
+            HIPCHK(hipMemcpyAsync(out + (size_t)i * batch_stride, (const uint8_t *)c->d_stage_out + (size_t)i * box,
+                                  box, hipMemcpyDeviceToHost, c->stream));
+    }
+    return mchip_ctx_sync(c);
 }
 
 int mchip_ctx_enable_timing(mchip_ctx *c, int on) {

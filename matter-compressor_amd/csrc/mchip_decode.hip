@@ -176,6 +176,35 @@ __device__ __forceinline__ void decode_header(RcDec &d, const QTab *__restrict__
     dc = v; flags = f;
 }
 
+// Entropy-decode one payload into a (raster index, level) list at w (cap
+// words), corrections appended as (pos, delta) word pairs.  Returns the
+// block record: x = dc | flags << 8 | status << 16, y = nsig | ncorr << 16.
+__device__ __forceinline__ uint2 entropy_to_list(const uint8_t *payload, uint32_t len, const QTab *__restrict__ qt,
+                                                 uint32_t *w, uint32_t cap) {
+    RcDec d;
+    d.init(payload, 0, len);
+    int dc, flags;
+    decode_header(d, qt, dc, flags);
+    ListSink sink{w, 0u, cap};
+    decode_coefs(d, qt, sink);
+    uint32_t nc = 0;
+    if (flags & 2) {
+        uint32_t n = d.expgolomb() + 1, pos = 0;
+        if (n > (uint32_t)kNVox) n = kNVox;
+        for (uint32_t i = 0; i < n; ++i) {
+            pos += d.expgolomb();
+            int neg = d.bypass();
+            uint32_t m = d.expgolomb() + 1;
+            if (pos >= (uint32_t)kNVox) break;
+            uint32_t at = sink.n + 2 * nc;
+            if (at + 2 <= sink.cap) { sink.w[at] = pos; sink.w[at + 1] = neg ? 0u - m : m; }
+            nc++;
+        }
+    }
+    uint32_t status = (sink.n + 2 * nc > sink.cap) ? kStatusFallback : kStatusOk;
+    return make_uint2((uint32_t)dc | ((uint32_t)flags << 8) | (status << 16), sink.n | (nc << 16));
+}
+
 // ---------------------------------------------------------------------------
 // k_entropy: one lane per present block of one chunk (blockIdx.y = chunk).
 // ---------------------------------------------------------------------------
@@ -199,30 +228,8 @@ __global__ __launch_bounds__(64) void k_entropy(const uint8_t *__restrict__ blob
         return;
     }
     const QTab *qt = qtabs + cd[c].qtab;
-    RcDec d;
-    d.init(blobs, cd[c].blob_off + r.off, r.len);
-    int dc, flags;
-    decode_header(d, qt, dc, flags);
     uint64_t wofs = cd[c].scratch_off + 8ull * (r.off - cinfo[c].pay0) + 32ull * r.slot;
-    ListSink sink{scratch + wofs, 0u, 8u * r.len + 32u};
-    uint32_t nsig = decode_coefs(d, qt, sink);
-    uint32_t nc = 0;
-    if (flags & 2) {
-        uint32_t n = d.expgolomb() + 1, pos = 0;
-        if (n > (uint32_t)kNVox) n = kNVox;
-        for (uint32_t i = 0; i < n; ++i) {
-            pos += d.expgolomb();
-            int neg = d.bypass();
-            uint32_t m = d.expgolomb() + 1;
-            if (pos >= (uint32_t)kNVox) break;
-            uint32_t at = sink.n + 2 * nc;
-            if (at + 2 <= sink.cap) { sink.w[at] = pos; sink.w[at + 1] = neg ? 0u - m : m; }
-            nc++;
-        }
-    }
-    uint32_t status = (sink.n + 2 * nc > sink.cap) ? kStatusFallback : kStatusOk;
-    (void)nsig;  // equals sink.n: one list entry per significant position
-    binfo[cb + bi] = make_uint2((uint32_t)dc | ((uint32_t)flags << 8) | (status << 16), sink.n | (nc << 16));
+    binfo[cb + bi] = entropy_to_list(blobs + cd[c].blob_off + r.off, r.len, qt, scratch + wofs, 8u * r.len + 32u);
 }
 
 // ---------------------------------------------------------------------------
@@ -318,6 +325,62 @@ __device__ void recon_single_lane(const uint8_t *payload, uint32_t len, const QT
     __syncthreads();
 }
 
+// Dequantise a level list into the LDS tile, inverse transform, round, add
+// dc, clamp and apply the corrections; leaves the voxels in u8t.
+__device__ __forceinline__ void recon_from_list(const uint2 inf, const uint32_t *__restrict__ w,
+                                                const QTab *__restrict__ qt, float *tile, uint8_t *u8t) {
+    const int t = threadIdx.x;
+    const float *cmf = c_cmf;
+    const int dc = inf.x & 255, flags = (inf.x >> 8) & 255;
+    const uint32_t nw = inf.y & 0xFFFFu, nc = inf.y >> 16;
+    if (nw == 0) {
+        ((uint4 *)u8t)[t] = make_uint4(dc * 0x01010101u, dc * 0x01010101u, dc * 0x01010101u, dc * 0x01010101u);
+    } else {
+        for (int i = t; i < 256 * kPad; i += 256) tile[i] = 0.0f;
+        __syncthreads();
+        for (uint32_t i = t; i < nw; i += 256) {
+            uint32_t e = w[i];
+            int idx = e & 0xFFFF;
+            int l1 = (idx >> 8) + ((idx >> 4) & 15) + (idx & 15);
+            tile[(idx >> 4) * kPad + (idx & 15)] = dequant((int)(int16_t)(e >> 16), qt->step[l1]);
+        }
+        __syncthreads();
+        float v[16];
+        {   // x pass: line (z,y) = t
+            float *ln = tile + t * kPad;
+            #pragma unroll
+            for (int k = 0; k < 16; ++k) v[k] = ln[k];
+            idct16_line(v, cmf);
+            #pragma unroll
+            for (int k = 0; k < 16; ++k) ln[k] = v[k];
+        }
+        __syncthreads();
+        {   // y pass: line (z,x)
+            float *ln = tile + (t >> 4) * 16 * kPad + (t & 15);
+            #pragma unroll
+            for (int k = 0; k < 16; ++k) v[k] = ln[k * kPad];
+            idct16_line(v, cmf);
+            #pragma unroll
+            for (int k = 0; k < 16; ++k) ln[k * kPad] = v[k];
+        }
+        __syncthreads();
+        {   // z pass: line (y,x); round, + dc, clamp into the byte tile
+            float *ln = tile + (t >> 4) * kPad + (t & 15);
+            #pragma unroll
+            for (int k = 0; k < 16; ++k) v[k] = ln[k * 16 * kPad];
+            idct16_line(v, cmf);
+            #pragma unroll
+            for (int k = 0; k < 16; ++k) u8t[k * 256 + t] = recon_voxel(v[k], dc);
+        }
+    }
+    if (flags & 2) {
+        __syncthreads();
+        if (t == 0)
+            for (uint32_t i = 0; i < nc; ++i) apply_correction(u8t, w[nw + 2 * i], w[nw + 2 * i + 1]);
+    }
+    __syncthreads();
+}
+
 // Map the linear workgroup id so that consecutive blocks (which share 128-byte
 // output lines) run on the same XCD: hardware places workgroup i on XCD i % 8.
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t i, uint32_t n) {
@@ -349,58 +412,10 @@ __global__ __launch_bounds__(256) void k_recon(const uint8_t *__restrict__ blobs
     const uint2 inf = binfo[cb + bi];
     const QTab *qt = qtabs + cd[c].qtab;
     const uint32_t status = inf.x >> 16;
-    const float *cmf = c_cmf;
 
     if (status == kStatusOk) {
-        const int dc = inf.x & 255, flags = (inf.x >> 8) & 255;
-        const uint32_t nw = inf.y & 0xFFFFu, nc = inf.y >> 16;
         const uint32_t *w = scratch + cd[c].scratch_off + 8ull * (r.off - cinfo[c].pay0) + 32ull * r.slot;
-        if (nw == 0) {
-            ((uint4 *)u8t)[t] = make_uint4(dc * 0x01010101u, dc * 0x01010101u, dc * 0x01010101u, dc * 0x01010101u);
-        } else {
-            for (int i = t; i < 256 * kPad; i += 256) tile[i] = 0.0f;
-            __syncthreads();
-            for (uint32_t i = t; i < nw; i += 256) {
-                uint32_t e = w[i];
-                int idx = e & 0xFFFF;
-                int l1 = (idx >> 8) + ((idx >> 4) & 15) + (idx & 15);
-                tile[(idx >> 4) * kPad + (idx & 15)] = dequant((int)(int16_t)(e >> 16), qt->step[l1]);
-            }
-            __syncthreads();
-            float v[16];
-            {   // x pass: line (z,y) = t
-                float *ln = tile + t * kPad;
-                #pragma unroll
-                for (int k = 0; k < 16; ++k) v[k] = ln[k];
-                idct16_line(v, cmf);
-                #pragma unroll
-                for (int k = 0; k < 16; ++k) ln[k] = v[k];
-            }
-            __syncthreads();
-            {   // y pass: line (z,x)
-                float *ln = tile + (t >> 4) * 16 * kPad + (t & 15);
-                #pragma unroll
-                for (int k = 0; k < 16; ++k) v[k] = ln[k * kPad];
-                idct16_line(v, cmf);
-                #pragma unroll
-                for (int k = 0; k < 16; ++k) ln[k * kPad] = v[k];
-            }
-            __syncthreads();
-            {   // z pass: line (y,x); round, + dc, clamp into the byte tile
-                float *ln = tile + (t >> 4) * kPad + (t & 15);
-                #pragma unroll
-                for (int k = 0; k < 16; ++k) v[k] = ln[k * 16 * kPad];
-                idct16_line(v, cmf);
-                #pragma unroll
-                for (int k = 0; k < 16; ++k) u8t[k * 256 + t] = recon_voxel(v[k], dc);
-            }
-        }
-        if (flags & 2) {
-            __syncthreads();
-            if (t == 0)
-                for (uint32_t i = 0; i < nc; ++i) apply_correction(u8t, w[nw + 2 * i], w[nw + 2 * i + 1]);
-        }
-        __syncthreads();
+        recon_from_list(inf, w, qt, tile, u8t);
         *orow = ((const uint4 *)u8t)[t];
         return;
     }
@@ -431,6 +446,94 @@ void launch_entropy(const DecodeArgs &a, hipStream_t s) {
 void launch_recon(const DecodeArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_recon, dim3((unsigned)a.n * kNVox), dim3(256), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
                        a.cinfo, a.scratch, a.binfo, a.out, a.n);
+}
+
+// ---------------------------------------------------------------------------
+// Block jobs: decode an arbitrary list of payloads into 4096-byte arena slots
+// (region reads and the decoded-block arena).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_entropy_jobs(const uint8_t *__restrict__ base,
+                                                     const BlockJob *__restrict__ jobs, int n,
+                                                     const QTab *__restrict__ qtabs,
+                                                     uint32_t *__restrict__ scratch, uint2 *__restrict__ binfo) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    const BlockJob j = jobs[i];
+    if (j.len == 0) { binfo[i] = make_uint2(kStatusFallback << 16, 0); return; }
+    binfo[i] = entropy_to_list(base + j.pay_off, j.len, qtabs + j.qtab, scratch + j.scratch_off, 8u * j.len + 32u);
+}
+
+__global__ __launch_bounds__(256) void k_recon_jobs(const uint8_t *__restrict__ base,
+                                                    const BlockJob *__restrict__ jobs,
+                                                    const QTab *__restrict__ qtabs,
+                                                    const uint32_t *__restrict__ scratch,
+                                                    const uint2 *__restrict__ binfo, uint8_t *__restrict__ arena) {
+    __shared__ float tile[256 * kPad];
+    __shared__ __attribute__((aligned(16))) uint8_t u8t[kNVox];
+    const int i = blockIdx.x, t = threadIdx.x;
+    const BlockJob j = jobs[i];
+    const QTab *qt = qtabs + j.qtab;
+    const uint2 inf = binfo[i];
+    if ((inf.x >> 16) == kStatusOk) recon_from_list(inf, scratch + j.scratch_off, qt, tile, u8t);
+    else recon_single_lane(base + j.pay_off, j.len, qt, tile, u8t);
+    ((uint4 *)(arena + (size_t)j.slot * kNVox))[t] = ((const uint4 *)u8t)[t];
+}
+
+void launch_jobs(const uint8_t *base, const BlockJob *jobs, int n, const QTab *qtabs, uint32_t *scratch,
+                 uint2 *binfo, uint8_t *arena, hipStream_t s) {
+    hipLaunchKernelGGL(k_entropy_jobs, dim3((n + 63) / 64), dim3(64), 0, s, base, jobs, n, qtabs, scratch, binfo);
+    hipLaunchKernelGGL(k_recon_jobs, dim3(n), dim3(256), 0, s, base, jobs, qtabs, scratch, binfo, arena);
+}
+
+// Assemble n boxes of dz x dy x dx voxels from arena blocks.  Box b's touched
+// block grid starts at block (gz0,gy0,gx0) = box origin >> 4 and has
+// gdim = (gz,gy,gx) cells in bmap[b * gz*gy*gx ...] (arena slot or -1 = 0).
+// Each thread writes 16 consecutive voxels of one output row.
+__global__ __launch_bounds__(256) void k_crop(const uint8_t *__restrict__ arena, const int32_t *__restrict__ bmap,
+                                              const int64_t *__restrict__ origin, int gz, int gy, int gx,
+                                              int64_t dz, int64_t dy, int64_t dx, uint8_t *__restrict__ out,
+                                              uint64_t batch_stride) {
+    const int b = blockIdx.y;
+    const int64_t segs = (dx + 15) / 16;
+    const int64_t item = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (item >= dz * dy * segs) return;
+    const int64_t x = (item % segs) * 16, y = (item / segs) % dy, z = item / (segs * dy);
+    const int64_t z0 = origin[3 * b], y0 = origin[3 * b + 1], x0 = origin[3 * b + 2];
+    const int64_t Z = z0 + z, Y = y0 + y, X = x0 + x;
+    const int32_t *bm = bmap + (size_t)b * gz * gy * gx;
+    const int cz = (int)((Z >> 4) - (z0 >> 4)), cy = (int)((Y >> 4) - (y0 >> 4));
+    const int lz = (int)(Z & 15), ly = (int)(Y & 15);
+    uint8_t v[16];
+    #pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int64_t XX = X + k;
+        const int cx = (int)((XX >> 4) - (x0 >> 4));
+        uint8_t val = 0;
+        if (x + k < dx && cx < gx) {
+            const int32_t sl = bm[(cz * gy + cy) * gx + cx];
+            if (sl >= 0) val = arena[(size_t)sl * kNVox + lz * 256 + ly * 16 + (int)(XX & 15)];
+        }
+        v[k] = val;
+    }
+    uint8_t *o = out + (size_t)b * batch_stride + (size_t)(z * dy + y) * dx + x;
+    const int64_t n = dx - x < 16 ? dx - x : 16;
+    if (n == 16 && (((uintptr_t)o) & 15) == 0) {
+        uint4 w;
+        w.x = v[0] | (v[1] << 8) | (v[2] << 16) | ((uint32_t)v[3] << 24);
+        w.y = v[4] | (v[5] << 8) | (v[6] << 16) | ((uint32_t)v[7] << 24);
+        w.z = v[8] | (v[9] << 8) | (v[10] << 16) | ((uint32_t)v[11] << 24);
+        w.w = v[12] | (v[13] << 8) | (v[14] << 16) | ((uint32_t)v[15] << 24);
+        *(uint4 *)o = w;
+    } else {
+        for (int k = 0; k < n; ++k) o[k] = v[k];
+    }
+}
+
+void launch_crop(const uint8_t *arena, const int32_t *bmap, const int64_t *origin, int n, int gz, int gy, int gx,
+                 int64_t dz, int64_t dy, int64_t dx, uint8_t *out, uint64_t batch_stride, hipStream_t s) {
+    const int64_t items = dz * dy * ((dx + 15) / 16);
+    hipLaunchKernelGGL(k_crop, dim3((unsigned)((items + 255) / 256), n), dim3(256), 0, s, arena, bmap, origin, gz, gy,
+                       gx, dz, dy, dx, out, batch_stride);
 }
 
 // n independent payloads: block i at base + off[i] (len[i]; off[i] == ~0 ->

@@ -73,6 +73,10 @@ __device__ __forceinline__ uint32_t wg_excl_scan(uint32_t v, uint32_t *lds4, uin
 // ---------------------------------------------------------------------------
 // k_enc_transform
 // ---------------------------------------------------------------------------
+// kBlocks = false: chunk c's block bi is gathered from its 256^3 raster;
+// kBlocks = true : independent 16^3 blocks, block g at in + 4096*g with its
+//                  own descriptor cd[g] (single-block API).
+template <bool kBlocks>
 __global__ __launch_bounds__(256) void k_enc_transform(const uint8_t *__restrict__ in,
                                                        const EncChunkDesc *__restrict__ cd,
                                                        const QTab *__restrict__ qtabs,
@@ -83,13 +87,18 @@ __global__ __launch_bounds__(256) void k_enc_transform(const uint8_t *__restrict
     __shared__ __attribute__((aligned(16))) uint8_t vox[kNVox];
     __shared__ int16_t lvr[kNVox];
     __shared__ uint32_t lds4[4];
-    const uint32_t g = xcd_remap_e(blockIdx.x, gridDim.x);
-    const int c = (int)(g >> 12), bi = (int)(g & 4095), t = threadIdx.x;
+    const uint32_t g = kBlocks ? blockIdx.x : xcd_remap_e(blockIdx.x, gridDim.x);
+    const int c = kBlocks ? (int)g : (int)(g >> 12), bi = (int)(g & 4095), t = threadIdx.x;
     if (c >= nchunks) return;
-    const size_t gb = (size_t)c * kNVox + bi;
-    const uint8_t *src = in + cd[c].in_off + (size_t)(bi >> 8) * 16 * 65536 +
-                         (size_t)((bi >> 4) & 15) * 16 * 256 + (size_t)(bi & 15) * 16;
-    const uint4 row = *(const uint4 *)(src + (size_t)(t >> 4) * 65536 + (size_t)(t & 15) * 256);
+    const size_t gb = kBlocks ? (size_t)g : (size_t)c * kNVox + bi;
+    uint4 row;
+    if (kBlocks) {
+        row = *(const uint4 *)(in + (size_t)g * kNVox + (size_t)t * 16);
+    } else {
+        const uint8_t *src = in + cd[c].in_off + (size_t)(bi >> 8) * 16 * 65536 +
+                             (size_t)((bi >> 4) & 15) * 16 * 256 + (size_t)(bi & 15) * 16;
+        row = *(const uint4 *)(src + (size_t)(t >> 4) * 65536 + (size_t)(t & 15) * 256);
+    }
     ((uint4 *)vox)[t] = row;
     uint32_t s = 0, n = 0;
     {
@@ -247,16 +256,18 @@ __device__ __forceinline__ void encode_magnitude(RcEnc &e, uint32_t (&m)[12], ui
     if (!done) { e.bit(m[11], 1); e.expgolomb(rest); }
 }
 
+template <bool kBlocks>
 __global__ __launch_bounds__(64) void k_enc_entropy(const EncChunkDesc *__restrict__ cd,
                                                     const QTab *__restrict__ qtabs,
                                                     const uint4 *__restrict__ einfo,
                                                     const int16_t *__restrict__ levels,
                                                     const uint32_t *__restrict__ corr,
                                                     uint32_t *__restrict__ slots,
-                                                    uint32_t *__restrict__ plen) {
-    const int c = blockIdx.y;
+                                                    uint32_t *__restrict__ plen, int nblocks) {
     const int bi = blockIdx.x * 64 + threadIdx.x;
-    const size_t gb = (size_t)c * kNVox + bi;
+    if (kBlocks && bi >= nblocks) return;
+    const int c = kBlocks ? bi : (int)blockIdx.y;
+    const size_t gb = kBlocks ? (size_t)bi : (size_t)c * kNVox + bi;
     const uint4 inf = einfo[gb];
     if (!(inf.x & (1u << 12))) { plen[gb] = 0; return; }
     const QTab *qt = qtabs + cd[c].qtab;
@@ -522,12 +533,19 @@ __global__ __launch_bounds__(64) void k_enc_hash(const EncChunkDesc *__restrict_
 
 // ---------------------------------------------------------------------------
 void launch_enc_transform(const EncodeArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_enc_transform, dim3((unsigned)a.n * kNVox), dim3(256), 0, s, a.in, a.cd, a.qtabs, a.einfo,
-                       a.levels, a.corr, a.n);
+    hipLaunchKernelGGL(k_enc_transform<false>, dim3((unsigned)a.n * kNVox), dim3(256), 0, s, a.in, a.cd, a.qtabs,
+                       a.einfo, a.levels, a.corr, a.n);
 }
 void launch_enc_entropy(const EncodeArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_enc_entropy, dim3(kNVox / 64, a.n), dim3(64), 0, s, a.cd, a.qtabs, a.einfo, a.levels,
-                       a.corr, a.slots, a.plen);
+    hipLaunchKernelGGL(k_enc_entropy<false>, dim3(kNVox / 64, a.n), dim3(64), 0, s, a.cd, a.qtabs, a.einfo,
+                       a.levels, a.corr, a.slots, a.plen, 0);
+}
+// independent blocks: a.n blocks, descriptor per block
+void launch_enc_blocks(const EncodeArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_enc_transform<true>, dim3((unsigned)a.n), dim3(256), 0, s, a.in, a.cd, a.qtabs, a.einfo,
+                       a.levels, a.corr, a.n);
+    hipLaunchKernelGGL(k_enc_entropy<true>, dim3(((unsigned)a.n + 63) / 64), dim3(64), 0, s, a.cd, a.qtabs,
+                       a.einfo, a.levels, a.corr, a.slots, a.plen, a.n);
 }
 void launch_enc_assemble(const EncodeArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_enc_assemble, dim3(a.n), dim3(256), 0, s, a.cd, a.einfo, a.plen, a.dst, a.out, a.res);

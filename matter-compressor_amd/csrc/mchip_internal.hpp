@@ -89,6 +89,21 @@ void launch_enc_entropy(const EncodeArgs &a, hipStream_t s);
 void launch_enc_assemble(const EncodeArgs &a, hipStream_t s);
 void launch_enc_copy(const EncodeArgs &a, hipStream_t s);
 void launch_enc_hash(const EncodeArgs &a, hipStream_t s);
+void launch_enc_blocks(const EncodeArgs &a, hipStream_t s);
+
+// ---- block jobs (region reads, decoded-block arena) --------------------------
+struct BlockJob {
+    uint64_t pay_off;       // payload offset in the payload buffer
+    uint64_t scratch_off;   // word offset of this job's list slot (8*len + 32 words)
+    uint32_t len;           // payload length (0: decoded by the single-lane path)
+    uint32_t qtab;
+    uint32_t slot;          // destination arena slot (4096 bytes each)
+    uint32_t pad;
+};
+void launch_jobs(const uint8_t *base, const BlockJob *jobs, int n, const QTab *qtabs, uint32_t *scratch,
+                 uint2 *binfo, uint8_t *arena, hipStream_t s);
+void launch_crop(const uint8_t *arena, const int32_t *bmap, const int64_t *origin, int n, int gz, int gy, int gx,
+                 int64_t dz, int64_t dy, int64_t dx, uint8_t *out, uint64_t batch_stride, hipStream_t s);
 
 // Scratch words a chunk needs: 8 per payload byte + 32 per block.
 inline uint64_t scratch_words_for(uint64_t blob_len) { return 8ull * blob_len + 32ull * 4096ull; }

@@ -246,3 +246,77 @@ def synth_chunks(seeds, air, coords=None, nthreads=None):
     synth().mcs_chunks(n, _ptr(seeds), None if cz is None else _ptr(cz), None if cy is None else _ptr(cy),
                        None if cx is None else _ptr(cx), _ptr(air), _ptr(out), nt)
     return out
+
+
+# ---- drop-in matter-compressor API (include/matter_compressor_amd.h) ----------
+class McBuf(C.Structure):
+    _fields_ = [("p", C.c_void_p), ("len", C.c_size_t), ("cap", C.c_size_t)]
+
+
+class McBox(C.Structure):
+    _fields_ = [("z0", C.c_long), ("y0", C.c_long), ("x0", C.c_long)]
+
+
+_api_declared = False
+
+
+def api():
+    """libmc_amd.so with the drop-in mc_* entry points declared."""
+    global _api_declared
+    L = load()
+    if _api_declared:
+        return L
+    vp, i, f, u64, lng = C.c_void_p, C.c_int, C.c_float, C.c_uint64, C.c_long
+    sig = {
+        "mc_codec_init": (None, []),
+        "mc_codec_ctx_new": (vp, []),
+        "mc_codec_ctx_free": (None, [vp]),
+        "mc_codec_ctx_set_quality": (None, [vp, f]),
+        "mc_codec_ctx_get_quality": (f, [vp]),
+        "mc_codec_ctx_set_max_error": (None, [vp, i]),
+        "mc_codec_ctx_get_max_error": (i, [vp]),
+        "mc_apply_preset": (f, [vp, i]),
+        "mc_preset_quality": (f, [i]),
+        "mc_preset_tau": (i, [i]),
+        "mc_preset_name": (C.c_char_p, [i]),
+        "mc_codec_set_priors": (None, [vp, vp]),
+        "mc_buf_put": (None, [C.POINTER(McBuf), vp, C.c_size_t]),
+        "mc_enc_block": (i, [vp, vp, C.POINTER(McBuf), C.POINTER(C.c_uint32)]),
+        "mc_dec_block": (None, [vp, vp, C.c_uint32, vp]),
+        "mc_enc_fracmap": (C.c_uint32, [vp, vp, C.c_size_t]),
+        "mc_dec_fracmap": (None, [vp, C.c_uint32, vp]),
+        "mc_archive_open": (vp, [C.c_char_p, i, f]),
+        "mc_archive_open_dims": (vp, [C.c_char_p, i, i, i, f]),
+        "mc_archive_close": (None, [vp]),
+        "mc_archive_set_block_codec": (i, [vp, C.c_uint32]),
+        "mc_archive_block_codec": (C.c_uint32, [vp]),
+        "mc_archive_block_blob": (i, [vp, u64, i, i, i, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32)]),
+        "mc_archive_append_chunk_raw": (i, [vp, i, i, i, i, vp]),
+        "mc_archive_append_chunk_raw_q": (i, [vp, i, i, i, i, vp, f]),
+        "mc_archive_append_chunk_ctx": (i, [vp, vp, i, i, i, i, vp]),
+        "mc_archive_append_chunk_compressed": (i, [vp, i, i, i, i, vp, C.c_size_t]),
+        "mc_archive_append_chunk_par": (i, [vp, i, i, i, i, vp, f, i]),
+        "mc_archive_reserve_index": (i, [vp, i, i, i, i]),
+        "mc_archive_chunk_coverage": (i, [vp, i, i, i, i]),
+        "mc_archive_data_len": (u64, [vp]),
+        "mc_archive_set_metadata": (i, [vp, vp, C.c_size_t]),
+        "mc_archive_metadata": (C.c_void_p, [vp, C.POINTER(C.c_size_t)]),
+        "mc_archive_chunk_offset": (u64, [vp, i, i, i, i]),
+        "mc_archive_decode_block": (None, [vp, u64, i, i, i, vp]),
+        "mc_archive_decode_chunk": (None, [vp, u64, vp, i]),
+        "mc_archive_read_region": (None, [vp, i, lng, lng, lng, lng, lng, lng, vp, C.c_size_t, C.c_size_t, i]),
+        "mc_archive_read_regions": (None, [vp, i, vp, i, lng, lng, lng, vp, C.c_size_t, i]),
+        "mc_archive_block_present": (i, [vp, i, i, i, i]),
+        "mc_archive_block_fraction": (f, [vp, i, i, i, i]),
+        "mc_archive_sample_boxes": (i, [vp, i, u64, i, lng, lng, lng, f, vp]),
+        "mc_archive_set_priors": (i, [vp, vp, vp]),
+        "mc_metadata": (C.c_void_p, [vp, C.POINTER(C.c_size_t)]),
+        "mc_chunk_compute_hash": (u64, [vp, u64]),
+        "mc_verify_archive": (lng, [vp, C.c_size_t, i]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _api_declared = True
+    return L
