@@ -80,6 +80,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="plain64_q6")
     ap.add_argument("--batched", action="store_true", help="also time all chunks in one launch")
+    ap.add_argument("--streams", type=int, default=8, help="HIP streams the per-chunk launches rotate over")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -97,14 +98,49 @@ def main():
     for i in range(min(n, 4)):
         assert xxh64_hex(chunks[i]) == man[i]["input_xxh64"], f"generator drift on chunk {i}"
 
-    log("[bench] CPU leg: encode all chunks, time decode on a sample")
-    blobs, cpu_enc, cpu_dec, ns = cpu_encode_and_baseline(chunks, specs, nthreads)
+    ctx = mc_amd.Ctx(0)
+    # ---- GPU encode (device-resident rasters -> device blobs), bit-exact check
+    d_in = mc_amd.DeviceBuffer(n * mc_amd.CHUNK_BYTES)
+    ctx.h2d(d_in, chunks)
+    cap = 20 << 20
+    d_enc = mc_amd.DeviceBuffer(n * cap)
+    eoffs = np.arange(n, dtype=np.uint64) * cap
+    ecaps = np.full(n, cap, np.uint64)
+    qv = [s_["q"] for s_ in specs]
+    tv = [s_["tau"] for s_ in specs]
+    log("[bench] GPU encode warmup")
+    ctx.encode_chunks(d_in, qv, tv, d_enc, eoffs, ecaps)
+    ctx.enable_timing(True)
+    ctx.reset_timing()
+    t0 = time.perf_counter()
+    esteps = max(1, min(args.steps, 3))
+    for _ in range(esteps):
+        elens, est = ctx.encode_chunks(d_in, qv, tv, d_enc, eoffs, ecaps)
+    enc_dt = (time.perf_counter() - t0) / esteps
+    enc_k = {k: ctx.kernel_time(k) for k in ("enc_transform", "enc_entropy", "enc_assemble", "enc_copy",
+                                              "enc_hash")}
+    ctx.enable_timing(False)
+    assert (est == 0).all(), f"encode status {est}"
+    blobs = []
+    bad_enc = 0
     for i in range(n):
-        assert len(blobs[i]) == man[i]["blob_len"] and xxh64_hex(np.frombuffer(blobs[i], np.uint8)) == \
-            man[i]["blob_xxh64"], f"blob {i} differs from the reference"
+        b = np.empty(int(elens[i]), np.uint8)
+        ctx.d2h(b, d_enc, offset=int(eoffs[i]))
+        if len(b) != man[i]["blob_len"] or xxh64_hex(b) != man[i]["blob_xxh64"]:
+            bad_enc += 1
+        blobs.append(b.tobytes())
+    if bad_enc:
+        log(f"[bench] ERROR: {bad_enc} encoded blobs differ from the reference")
+    enc_gbs = n * mc_amd.CHUNK_BYTES / enc_dt / 1e9
+    log(f"[bench] GPU encode {enc_gbs:.2f} GB/s, bit-exact={bad_enc == 0}")
+    d_in.free()
+    d_enc.free()
+
+    log("[bench] CPU baseline leg")
+    _, cpu_enc, cpu_dec, ns = cpu_encode_and_baseline(chunks[: min(n, nthreads)], specs[: min(n, nthreads)],
+                                                       nthreads)
     del chunks
 
-    ctx = mc_amd.Ctx(0)
     offs = np.zeros(n, np.uint64)
     tot = 0
     for i, b in enumerate(blobs):
@@ -125,26 +161,37 @@ def main():
             self.ptr = buf.ptr + i * mc_amd.CHUNK_BYTES
 
     views = [_View(d_out, i) for i in range(n)]
+    ctxs = [ctx] + [mc_amd.Ctx(0) for _ in range(args.streams - 1)]
 
     def step():
         for i in range(n):
-            ctx.decode_chunks(d_blobs, offs[i:i + 1], lens[i:i + 1], qs[i:i + 1], views[i])
+            ctxs[i % len(ctxs)].decode_chunks(d_blobs, offs[i:i + 1], lens[i:i + 1], qs[i:i + 1], views[i])
 
     log(f"[bench] warmup {args.warmup}")
     for _ in range(args.warmup):
         step()
-    ctx.sync()
-    ctx.enable_timing(True)
-    ctx.reset_timing()
-    ctx.sync()
+    for c_ in ctxs:
+        c_.sync()
+    for c_ in ctxs:
+        c_.enable_timing(True)
+        c_.reset_timing()
     log(f"[bench] timing {args.steps} steps")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
-    ctx.sync()
+    for c_ in ctxs:
+        c_.sync()
     dt = time.perf_counter() - t0
-    ktimes = {k: ctx.kernel_time(k) for k in ("index", "entropy", "recon")}
-    ctx.enable_timing(False)
+    ktimes = {}
+    for k in ("index", "entropy", "recon"):
+        tot, cnt = 0.0, 0
+        for c_ in ctxs:
+            a_, b_ = c_.kernel_time(k)
+            tot += a_
+            cnt += b_
+        ktimes[k] = (tot, cnt)
+    for c_ in ctxs:
+        c_.enable_timing(False)
     ms_per_step = dt * 1000.0 / args.steps
     value = n * mc_amd.CHUNK_BYTES / (dt / args.steps) / 1e9
 
@@ -193,7 +240,7 @@ def main():
         "bit_exact": bad == 0,
         "config": {"workload": args.workload, "chunks": n, "chunk": "256^3 u8", "q": 6.0, "air": 0.419,
                    "decoded_bytes_per_step": n * mc_amd.CHUNK_BYTES, "blob_bytes": blob_bytes,
-                   "launch": "one chunk per launch"},
+                   "launch": "one chunk per launch", "streams": args.streams},
         "kernels_ms_avg": {k: round(v, 4) for k, v in kt.items()},
         "roofline": {"bound": "hbm", "kernel": "decode (index+entropy+recon per chunk)", "dominant": dom,
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -201,12 +248,16 @@ def main():
         "cpu_baseline": {"value": round(cpu_dec, 3), "unit": "GB/s", "cores": nthreads, "kind": "port",
                          "sample": f"oracle decode of {ns} of the {n} chunks, {nthreads} threads over blocks",
                          "encode_gbs": round(cpu_enc, 3)},
+        "encode": {"value": round(enc_gbs, 3), "unit": "GB/s", "byte_identical": bad_enc == 0,
+                   "launch": "32 chunks per launch",
+                   "kernels_ms_per_chunk": {k: round(v[0] / max(1, v[1]) / 32.0, 4) for k, v in enc_k.items()}},
     }
     rec.update(extra)
     print(json.dumps(rec))
     d_blobs.free()
     d_out.free()
-    ctx.close()
+    for c_ in ctxs:
+        c_.close()
 
 
 if __name__ == "__main__":

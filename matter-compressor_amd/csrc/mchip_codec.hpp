@@ -1,0 +1,204 @@
+// mchip_codec.hpp -- the block bitstream grammar on top of the range coder:
+// header bins, EOB, significance / magnitude / sign bins and max-error
+// corrections, for decoding (any sink) and encoding.  Shared by the HIP
+// kernels and, through a host shim, by the CPU test harness of the device
+// logic.  Follows SuperOptimizer/matter-compressor src/mc_codec.c:240-375,
+// :711-783.
+#pragma once
+#include "mchip_device.hpp"
+
+namespace mchip {
+
+// ---------------------------------------------------------------------------
+// Coefficient grammar (src/mc_codec.c:321-375), shared by the list decoder
+// and the single-lane fallback.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t decode_magnitude(RcDec &d, uint32_t (&m)[12]) {
+    uint32_t v = 0;
+    bool done = false;
+    #pragma unroll
+    for (int k = 0; k < 11; ++k) {
+        if (!done) {
+            if (d.bit(m[k])) v++;
+            else done = true;
+        }
+    }
+    if (done) return v + 1;
+    if (!d.bit(m[11])) return v + 1;
+    return v + d.expgolomb() + 1;
+}
+
+template <class Sink>
+__device__ __forceinline__ uint32_t decode_coefs(RcDec &d, const QTab *__restrict__ qt, const uint16_t *band_start,
+                                                 Sink &sink) {
+    int k = 0;
+    #pragma unroll
+    for (int i = 0; i < 13; ++i)
+        if (k == i && d.bit_once(qt->eob[i])) k = i + 1;
+    uint32_t eob = k == 0 ? 0u : (k == 1 ? 1u : ((1u << (k - 1)) | d.bypass_n(k - 1)));
+    if (eob > (uint32_t)kNVox) eob = kNVox;
+    uint32_t m[12];
+    #pragma unroll
+    for (int i = 0; i < 12; ++i) m[i] = qt->mag[i];
+    uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int band = -1;
+    uint32_t next_band = 0, hist = 0, nsig = 0;
+    for (uint32_t p = 0; p < eob; ++p) {
+        if (p >= next_band) {
+            band++;
+            s0 = qt->sig[band * 4 + 0]; s1 = qt->sig[band * 4 + 1];
+            s2 = qt->sig[band * 4 + 2]; s3 = qt->sig[band * 4 + 3];
+            next_band = band_start[band + 1];
+        }
+        int sig = 1;
+        if (p + 1 != eob) {
+            uint32_t dens = __popc(hist & 0xFFFFu);
+            uint32_t p0 = dens == 0 ? s0 : dens == 1 ? s1 : dens == 2 ? s2 : s3;
+            sig = d.bit(p0);
+            s0 = dens == 0 ? p0 : s0;
+            s1 = dens == 1 ? p0 : s1;
+            s2 = dens == 2 ? p0 : s2;
+            s3 = dens >= 3 ? p0 : s3;
+        }
+        hist = (hist << 1) | (uint32_t)sig;
+        if (sig) {
+            uint32_t mag = decode_magnitude(d, m);
+            int neg = d.bypass();
+            uint32_t lv = neg ? 0u - mag : mag;
+            sink.coef(p, (uint16_t)lv);
+            nsig++;
+        }
+    }
+    return nsig;
+}
+
+// List entries hold the scan position; the reconstruction maps it to the
+// raster index (keeps the table lookup off the serial decode chain).
+struct ListSink {
+    uint32_t *w;
+    uint32_t n, cap;
+    __device__ __forceinline__ void coef(uint32_t p, uint16_t lv) {
+        if (n < cap) w[n] = p | ((uint32_t)lv << 16);
+        n++;
+    }
+};
+
+__device__ __forceinline__ void decode_header(RcDec &d, const QTab *__restrict__ qt, int &dc, int &flags) {
+    int f = 0, v = 0;
+    f |= d.bit_once(qt->flag[0]) ? 1 : 0;
+    f |= d.bit_once(qt->flag[1]) ? 2 : 0;
+    #pragma unroll
+    for (int b = 0; b < 8; ++b) v = (v << 1) | d.bit_once(qt->dc[b]);
+    dc = v; flags = f;
+}
+
+// Entropy-decode one payload into a (raster index, level) list at w (cap
+// words), corrections appended as (pos, delta) word pairs.  Returns the
+// block record: x = dc | flags << 8 | status << 16, y = nsig | ncorr << 16.
+__device__ __forceinline__ uint2 entropy_to_list(const uint8_t *payload, uint32_t len, const QTab *__restrict__ qt,
+                                                 const uint16_t *band_start, uint32_t *w, uint32_t cap) {
+    RcDec d;
+    d.init(payload, 0, len);
+    int dc, flags;
+    decode_header(d, qt, dc, flags);
+    ListSink sink{w, 0u, cap};
+    decode_coefs(d, qt, band_start, sink);
+    uint32_t nc = 0;
+    if (flags & 2) {
+        uint32_t n = d.expgolomb() + 1, pos = 0;
+        if (n > (uint32_t)kNVox) n = kNVox;
+        for (uint32_t i = 0; i < n; ++i) {
+            pos += d.expgolomb();
+            int neg = d.bypass();
+            uint32_t m = d.expgolomb() + 1;
+            if (pos >= (uint32_t)kNVox) break;
+            uint32_t at = sink.n + 2 * nc;
+            if (at + 2 <= sink.cap) { sink.w[at] = pos; sink.w[at + 1] = neg ? 0u - m : m; }
+            nc++;
+        }
+    }
+    uint32_t status = (sink.n + 2 * nc > sink.cap) ? kStatusFallback : kStatusOk;
+    return make_uint2((uint32_t)dc | ((uint32_t)flags << 8) | (status << 16), sink.n | (nc << 16));
+}
+
+__device__ __forceinline__ void encode_magnitude(RcEnc &e, uint32_t (&m)[12], uint32_t mag) {
+    uint32_t rest = mag - 1;
+    if (rest == 0) { e.bit(m[0], 0); return; }
+    bool done = false;
+    #pragma unroll
+    for (int k = 0; k < 11; ++k) {
+        if (!done) {
+            e.bit(m[k], 1);
+            rest--;
+            if (rest == 0) { e.bit(m[k + 1], 0); done = true; }
+        }
+    }
+    if (!done) { e.bit(m[11], 1); e.expgolomb(rest); }
+}
+
+// Stream of one block: header bins, EOB, coefficient bins over the
+// scan-ordered levels L[0..eob), corrections C[0..ncorr) as (pos | delta<<16)
+// (src/mc_codec.c:711-729); the caller flushes.
+__device__ __forceinline__ void encode_block_stream(RcEnc &e, const QTab *__restrict__ qt, const uint16_t *band_start,
+                                                    uint32_t dc, uint32_t eob, const int16_t *__restrict__ L,
+                                                    uint32_t ncorr, const uint32_t *__restrict__ C) {
+    e.bit_once(qt->flag[0], 0);
+    e.bit_once(qt->flag[1], ncorr > 0);
+    #pragma unroll
+    for (int b = 7; b >= 0; --b) e.bit_once(qt->dc[7 - b], (dc >> b) & 1);
+    // EOB: bit-length prefix (no terminator at 13), then the bits below the MSB
+    {
+        const int k = eob ? 32 - __clz(eob) : 0;
+        #pragma unroll
+        for (int i = 0; i < 13; ++i) {
+            if (i < k) e.bit_once(qt->eob[i], 1);
+            else if (i == k) e.bit_once(qt->eob[i], 0);
+        }
+        if (k > 1) e.bypass_n(eob & ((1u << (k - 1)) - 1u), k - 1);
+    }
+    uint32_t m[12];
+    #pragma unroll
+    for (int i = 0; i < 12; ++i) m[i] = qt->mag[i];
+    uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int band = -1;
+    uint32_t next_band = 0, hist = 0;
+    for (uint32_t p = 0; p < eob; ++p) {
+        if (p >= next_band) {
+            band++;
+            s0 = qt->sig[band * 4 + 0]; s1 = qt->sig[band * 4 + 1];
+            s2 = qt->sig[band * 4 + 2]; s3 = qt->sig[band * 4 + 3];
+            next_band = band_start[band + 1];
+        }
+        const int v = L[p];
+        const int nzb = v != 0;
+        if (p + 1 != eob) {
+            uint32_t dens = __popc(hist & 0xFFFFu);
+            uint32_t p0 = dens == 0 ? s0 : dens == 1 ? s1 : dens == 2 ? s2 : s3;
+            e.bit(p0, nzb);
+            s0 = dens == 0 ? p0 : s0;
+            s1 = dens == 1 ? p0 : s1;
+            s2 = dens == 2 ? p0 : s2;
+            s3 = dens >= 3 ? p0 : s3;
+        }
+        hist = (hist << 1) | (uint32_t)nzb;
+        if (nzb) {
+            encode_magnitude(e, m, (uint32_t)(v < 0 ? -v : v));
+            e.bypass(v < 0);
+        }
+    }
+    if (ncorr) {
+        e.expgolomb(ncorr - 1);
+        uint32_t prev = 0;
+        for (uint32_t i = 0; i < ncorr; ++i) {
+            const uint32_t w = C[i];
+            const uint32_t pos = w & 0xFFFFu;
+            const int d = (int)(int16_t)(w >> 16);
+            e.expgolomb(pos - prev);
+            prev = pos;
+            e.bypass(d < 0);
+            e.expgolomb((uint32_t)(d < 0 ? -d : d) - 1u);
+        }
+    }
+}
+
+}  // namespace mchip

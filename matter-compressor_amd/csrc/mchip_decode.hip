@@ -18,6 +18,7 @@
 // Blocks whose list would not fit the per-block slot (possible only for
 // corrupt payloads) are decoded inside k_recon by a single lane straight into
 // the LDS tile, so every payload decodes exactly like the reference.
+#include "mchip_codec.hpp"
 #include "mchip_device.hpp"
 #include "mchip_internal.hpp"
 
@@ -97,115 +98,6 @@ __global__ __launch_bounds__(256) void k_index(const uint8_t *__restrict__ blobs
 }
 
 // ---------------------------------------------------------------------------
-// Coefficient grammar (src/mc_codec.c:321-375), shared by the list decoder
-// and the single-lane fallback.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t decode_magnitude(RcDec &d, uint32_t (&m)[12]) {
-    uint32_t v = 0;
-    bool done = false;
-    #pragma unroll
-    for (int k = 0; k < 11; ++k) {
-        if (!done) {
-            if (d.bit(m[k])) v++;
-            else done = true;
-        }
-    }
-    if (done) return v + 1;
-    if (!d.bit(m[11])) return v + 1;
-    return v + d.expgolomb() + 1;
-}
-
-template <class Sink>
-__device__ __forceinline__ uint32_t decode_coefs(RcDec &d, const QTab *__restrict__ qt, Sink &sink) {
-    int k = 0;
-    #pragma unroll
-    for (int i = 0; i < 13; ++i)
-        if (k == i && d.bit_once(qt->eob[i])) k = i + 1;
-    uint32_t eob = k == 0 ? 0u : (k == 1 ? 1u : ((1u << (k - 1)) | d.bypass_n(k - 1)));
-    if (eob > (uint32_t)kNVox) eob = kNVox;
-    uint32_t m[12];
-    #pragma unroll
-    for (int i = 0; i < 12; ++i) m[i] = qt->mag[i];
-    uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-    int band = -1;
-    uint32_t next_band = 0, hist = 0, nsig = 0;
-    for (uint32_t p = 0; p < eob; ++p) {
-        if (p >= next_band) {
-            band++;
-            s0 = qt->sig[band * 4 + 0]; s1 = qt->sig[band * 4 + 1];
-            s2 = qt->sig[band * 4 + 2]; s3 = qt->sig[band * 4 + 3];
-            next_band = c_band_start[band + 1];
-        }
-        int sig = 1;
-        if (p + 1 != eob) {
-            uint32_t dens = __popc(hist & 0xFFFFu);
-            uint32_t p0 = dens == 0 ? s0 : dens == 1 ? s1 : dens == 2 ? s2 : s3;
-            sig = d.bit(p0);
-            s0 = dens == 0 ? p0 : s0;
-            s1 = dens == 1 ? p0 : s1;
-            s2 = dens == 2 ? p0 : s2;
-            s3 = dens >= 3 ? p0 : s3;
-        }
-        hist = (hist << 1) | (uint32_t)sig;
-        if (sig) {
-            uint32_t mag = decode_magnitude(d, m);
-            int neg = d.bypass();
-            uint32_t lv = neg ? 0u - mag : mag;
-            sink.coef(p, (uint16_t)lv);
-            nsig++;
-        }
-    }
-    return nsig;
-}
-
-struct ListSink {
-    uint32_t *w;
-    uint32_t n, cap;
-    __device__ __forceinline__ void coef(uint32_t p, uint16_t lv) {
-        if (n < cap) w[n] = (uint32_t)g_scan[p] | ((uint32_t)lv << 16);
-        n++;
-    }
-};
-
-__device__ __forceinline__ void decode_header(RcDec &d, const QTab *__restrict__ qt, int &dc, int &flags) {
-    int f = 0, v = 0;
-    f |= d.bit_once(qt->flag[0]) ? 1 : 0;
-    f |= d.bit_once(qt->flag[1]) ? 2 : 0;
-    #pragma unroll
-    for (int b = 0; b < 8; ++b) v = (v << 1) | d.bit_once(qt->dc[b]);
-    dc = v; flags = f;
-}
-
-// Entropy-decode one payload into a (raster index, level) list at w (cap
-// words), corrections appended as (pos, delta) word pairs.  Returns the
-// block record: x = dc | flags << 8 | status << 16, y = nsig | ncorr << 16.
-__device__ __forceinline__ uint2 entropy_to_list(const uint8_t *payload, uint32_t len, const QTab *__restrict__ qt,
-                                                 uint32_t *w, uint32_t cap) {
-    RcDec d;
-    d.init(payload, 0, len);
-    int dc, flags;
-    decode_header(d, qt, dc, flags);
-    ListSink sink{w, 0u, cap};
-    decode_coefs(d, qt, sink);
-    uint32_t nc = 0;
-    if (flags & 2) {
-        uint32_t n = d.expgolomb() + 1, pos = 0;
-        if (n > (uint32_t)kNVox) n = kNVox;
-        for (uint32_t i = 0; i < n; ++i) {
-            pos += d.expgolomb();
-            int neg = d.bypass();
-            uint32_t m = d.expgolomb() + 1;
-            if (pos >= (uint32_t)kNVox) break;
-            uint32_t at = sink.n + 2 * nc;
-            if (at + 2 <= sink.cap) { sink.w[at] = pos; sink.w[at + 1] = neg ? 0u - m : m; }
-            nc++;
-        }
-    }
-    uint32_t status = (sink.n + 2 * nc > sink.cap) ? kStatusFallback : kStatusOk;
-    return make_uint2((uint32_t)dc | ((uint32_t)flags << 8) | (status << 16), sink.n | (nc << 16));
-}
-
-// ---------------------------------------------------------------------------
 // k_entropy: one lane per present block of one chunk (blockIdx.y = chunk).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_entropy(const uint8_t *__restrict__ blobs,
@@ -229,7 +121,8 @@ __global__ __launch_bounds__(64) void k_entropy(const uint8_t *__restrict__ blob
     }
     const QTab *qt = qtabs + cd[c].qtab;
     uint64_t wofs = cd[c].scratch_off + 8ull * (r.off - cinfo[c].pay0) + 32ull * r.slot;
-    binfo[cb + bi] = entropy_to_list(blobs + cd[c].blob_off + r.off, r.len, qt, scratch + wofs, 8u * r.len + 32u);
+    binfo[cb + bi] = entropy_to_list(blobs + cd[c].blob_off + r.off, r.len, qt, c_band_start, scratch + wofs,
+                                     8u * r.len + 32u);
 }
 
 // ---------------------------------------------------------------------------
@@ -274,7 +167,7 @@ __device__ void recon_single_lane(const uint8_t *payload, uint32_t len, const QT
         int dc, flags;
         decode_header(d, qt, dc, flags);
         TileSink sink{tile, qt};
-        s_nsig = (int)decode_coefs(d, qt, sink);
+        s_nsig = (int)decode_coefs(d, qt, c_band_start, sink);
         s_dc = dc; s_flags = flags;
     }
     __syncthreads();
@@ -340,7 +233,7 @@ __device__ __forceinline__ void recon_from_list(const uint2 inf, const uint32_t 
         __syncthreads();
         for (uint32_t i = t; i < nw; i += 256) {
             uint32_t e = w[i];
-            int idx = e & 0xFFFF;
+            int idx = g_scan[e & 0xFFFF];
             int l1 = (idx >> 8) + ((idx >> 4) & 15) + (idx & 15);
             tile[(idx >> 4) * kPad + (idx & 15)] = dequant((int)(int16_t)(e >> 16), qt->step[l1]);
         }
@@ -460,7 +353,8 @@ __global__ __launch_bounds__(64) void k_entropy_jobs(const uint8_t *__restrict__
     if (i >= n) return;
     const BlockJob j = jobs[i];
     if (j.len == 0) { binfo[i] = make_uint2(kStatusFallback << 16, 0); return; }
-    binfo[i] = entropy_to_list(base + j.pay_off, j.len, qtabs + j.qtab, scratch + j.scratch_off, 8u * j.len + 32u);
+    binfo[i] = entropy_to_list(base + j.pay_off, j.len, qtabs + j.qtab, c_band_start, scratch + j.scratch_off,
+                               8u * j.len + 32u);
 }
 
 __global__ __launch_bounds__(256) void k_recon_jobs(const uint8_t *__restrict__ base,

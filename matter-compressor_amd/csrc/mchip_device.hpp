@@ -15,6 +15,8 @@ namespace mchip {
 
 constexpr int kNVox = 4096;
 constexpr uint32_t kRcTop = 1u << 24;
+constexpr uint32_t kStatusOk = 0;
+constexpr uint32_t kStatusFallback = 1;   // block list overflowed: decode it with the single-lane path
 
 // Per-quality tables built on the host (libm powf/log are evaluated there,
 // exactly as the reference's context does: src/mc_codec.c:156-167, :541-552).
@@ -28,55 +30,75 @@ struct QTab {
     uint16_t dc[8];
 };
 
-// Byte source for the decoder: reads payload bytes through aligned 32-bit
-// words and returns 0 past the end of the payload (src/mc_codec.c:197-205).
-struct ByteSrc {
-    const uint32_t *words;   // 4-byte aligned base of the containing buffer
-    uint32_t start;          // byte offset of the payload in that buffer
-    uint32_t len;            // payload length
-    uint32_t pos;            // bytes consumed
-    uint32_t cur_w;          // index of the cached word (0xFFFFFFFF = none)
-    uint32_t cur;            // cached word
-
-    __device__ __forceinline__ void init(const uint8_t *base, uint64_t off, uint32_t n) {
-        uint64_t a = (uint64_t)(uintptr_t)(base + off);
-        words = (const uint32_t *)(uintptr_t)(a & ~3ull);
-        start = (uint32_t)(a & 3);
-        len = n; pos = 0; cur_w = 0xFFFFFFFFu; cur = 0;
-    }
-    __device__ __forceinline__ uint32_t next() {
-        if (pos >= len) return 0;
-        uint32_t b = start + pos;
-        uint32_t w = b >> 2;
-        if (w != cur_w) { cur = words[w]; cur_w = w; }
-        pos++;
-        return (cur >> ((b & 3) * 8)) & 0xFFu;
-    }
-};
-
+// Payload reader of the decoder: a 64-bit MSB-first bit buffer refilled one
+// aligned 32-bit word at a time, with the following word loaded one refill
+// ahead so the serial decode never waits on memory.  Bytes outside the
+// payload read as 0, as in the reference decoder (src/mc_codec.c:197-205).
 struct RcDec {
-    ByteSrc src;
+    const uint32_t *wp;      // aligned word holding payload byte 4*k - lead
+    int32_t k;               // index of the word in nextw
+    int32_t last;            // index of the last word that holds payload bytes
+    uint32_t lead;           // payload start offset inside word 0
+    uint32_t len;
+    uint32_t nextw;          // prefetched word k (masked, byte-swapped)
+    uint64_t nb;             // upcoming payload bits, MSB first
+    int32_t nbits;
     uint32_t code, range;
 
-    __device__ __forceinline__ void init(const uint8_t *base, uint64_t off, uint32_t n) {
-        src.init(base, off, n);
-        code = 0; range = 0xFFFFFFFFu;
-        #pragma unroll
-        for (int i = 0; i < 5; ++i) code = (code << 8) | src.next();
+    // word i with bytes outside [0, len) cleared, first payload byte on top
+    __device__ __forceinline__ uint32_t fetch(int32_t i) const {
+        const int32_t ii = i > last ? last : i;
+        uint32_t w = wp[ii];
+        const int32_t b0 = 4 * i - (int32_t)lead;           // payload index of byte 0 of word i
+        uint32_t keep = 0xFFFFFFFFu;
+        if (b0 < 0) keep &= 0xFFFFFFFFu << (8 * (-b0));
+        const int32_t room = (int32_t)len - b0;               // payload bytes from byte 0 on
+        if (room < 4) keep = room <= 0 ? 0u : keep & (0xFFFFFFFFu >> (8 * (4 - room)));
+        return __builtin_bswap32(w & keep);
     }
-    // Renormalise.  Any reachable state has range >= 1, which needs at most
-    // three byte shifts; the bound keeps a zero range (only possible with a
-    // corrupt prior override) from looping.
+    __device__ __forceinline__ void refill() {
+        nb |= (uint64_t)nextw << (32 - nbits);
+        nbits += 32;
+        k++;
+        nextw = fetch(k);
+    }
+    __device__ __forceinline__ void init(const uint8_t *base, uint64_t off, uint32_t n) {
+        const uint64_t a = (uint64_t)(uintptr_t)(base + off);
+        wp = (const uint32_t *)(uintptr_t)(a & ~3ull);
+        lead = (uint32_t)(a & 3);
+        len = n;
+        last = n ? (int32_t)((lead + n - 1) >> 2) : 0;
+        nb = (uint64_t)fetch(0) << (32 + 8 * lead);           // drop the bytes before the payload
+        nbits = 32 - 8 * (int32_t)lead;
+        k = 1;
+        nextw = fetch(1);
+        refill();                                             // >= 40 bits buffered
+        code = (uint32_t)(nb >> 32) << 8;                     // 5 bytes; the first one is shifted out
+        code |= (uint32_t)(nb >> 24) & 0xFFu;
+        nb <<= 40;
+        nbits -= 40;
+        range = 0xFFFFFFFFu;
+        if (nbits < 32) refill();
+    }
+    // Shift in as many bytes as the range needs (clz(range)/8 of them).
     __device__ __forceinline__ void norm() {
-        #pragma unroll 1
-        for (int i = 0; i < 4 && range < kRcTop; ++i) { code = (code << 8) | src.next(); range <<= 8; }
+        uint32_t kb = (uint32_t)__clz(range) >> 3;            // 0..3 for range >= 1
+        kb = kb > 3 ? 3 : kb;
+        const uint32_t sh = 8 * kb;
+        const uint32_t top = (uint32_t)((nb >> (63 - sh)) >> 1);
+        code = (code << sh) | top;
+        range <<= sh;
+        nb <<= sh;
+        nbits -= (int32_t)sh;
+        if (nbits < 32) refill();
     }
     // adaptive binary decision with P(0) = p0/4096 (src/mc_codec.c:198-204)
     __device__ __forceinline__ int bit(uint32_t &p0) {
         uint32_t r0 = __umul24(range >> 12, p0);
         int b = code >= r0;
-        if (b) { code -= r0; range -= r0; p0 -= p0 >> 4; }
-        else   { range = r0; p0 += (4096u - p0) >> 4; }
+        code = b ? code - r0 : code;
+        range = b ? range - r0 : r0;
+        p0 = b ? p0 - (p0 >> 4) : p0 + ((4096u - p0) >> 4);
         norm();
         return b;
     }
@@ -85,33 +107,33 @@ struct RcDec {
     __device__ __forceinline__ int bypass() {
         range >>= 1;
         int b = code >= range;
-        if (b) code -= range;
+        code = b ? code - range : code;
         norm();
         return b;
     }
-    __device__ __forceinline__ uint32_t bypass_k16(int k) {   // 1 <= k <= 16
-        range >>= k;
-        uint32_t q = code / range, m = (1u << k) - 1u;
+    __device__ __forceinline__ uint32_t bypass_k16(int kk) {   // 1 <= kk <= 16
+        range >>= kk;
+        uint32_t q = code / range, m = (1u << kk) - 1u;
         if (q > m) q = m;
         code -= q * range;
         norm();
         return q;
     }
-    // batched bypass of k bits (src/mc_codec.c:216-226)
-    __device__ __forceinline__ uint32_t bypass_n(int k) {
+    // batched bypass of kk bits (src/mc_codec.c:216-226)
+    __device__ __forceinline__ uint32_t bypass_n(int kk) {
         uint32_t v = 0;
-        while (k > 16) { v = (v << 16) | bypass_k16(16); k -= 16; }
-        if (k == 0) return v;
-        return (v << k) | bypass_k16(k);
+        while (kk > 16) { v = (v << 16) | bypass_k16(16); kk -= 16; }
+        if (kk == 0) return v;
+        return (v << kk) | bypass_k16(kk);
     }
     // order-0 Exp-Golomb in bypass bits (src/mc_codec.c:234-238).  A valid
     // stream never has more than 32 prefix ones; the cap only bounds the work
     // spent on corrupt input.
     __device__ __forceinline__ uint32_t expgolomb() {
-        uint32_t nb = 0;
-        while (nb < 64 && bypass()) nb++;
-        if (!nb) return 0;
-        return ((1u << (nb & 31)) | bypass_n((int)nb)) - 1u;
+        uint32_t nb1 = 0;
+        while (nb1 < 64 && bypass()) nb1++;
+        if (!nb1) return 0;
+        return ((1u << (nb1 & 31)) | bypass_n((int)nb1)) - 1u;
     }
 };
 

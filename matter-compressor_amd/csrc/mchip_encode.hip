@@ -19,6 +19,7 @@
 //   k_enc_copy       one workgroup per block: payload into its final place;
 //   k_enc_hash       4 lanes per chunk: the XXH64 chain over fraction map,
 //                    bitmap, lengths and payloads (src/mc_archive.c:76-88).
+#include "mchip_codec.hpp"
 #include "mchip_device.hpp"
 #include "mchip_internal.hpp"
 
@@ -241,21 +242,6 @@ __global__ __launch_bounds__(256) void k_enc_transform(const uint8_t *__restrict
 // ---------------------------------------------------------------------------
 // k_enc_entropy: one lane per block (blockIdx.y = chunk)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void encode_magnitude(RcEnc &e, uint32_t (&m)[12], uint32_t mag) {
-    uint32_t rest = mag - 1;
-    if (rest == 0) { e.bit(m[0], 0); return; }
-    bool done = false;
-    #pragma unroll
-    for (int k = 0; k < 11; ++k) {
-        if (!done) {
-            e.bit(m[k], 1);
-            rest--;
-            if (rest == 0) { e.bit(m[k + 1], 0); done = true; }
-        }
-    }
-    if (!done) { e.bit(m[11], 1); e.expgolomb(rest); }
-}
-
 template <bool kBlocks>
 __global__ __launch_bounds__(64) void k_enc_entropy(const EncChunkDesc *__restrict__ cd,
                                                     const QTab *__restrict__ qtabs,
@@ -274,65 +260,7 @@ __global__ __launch_bounds__(64) void k_enc_entropy(const EncChunkDesc *__restri
     const uint32_t dc = inf.x & 255u, eob = inf.z, ncorr = inf.w;
     RcEnc e;
     e.init(slots + gb * (kSlotBytes / 4), kSlotBytes);
-    e.bit_once(qt->flag[0], 0);
-    e.bit_once(qt->flag[1], ncorr > 0);
-    #pragma unroll
-    for (int b = 7; b >= 0; --b) e.bit_once(qt->dc[7 - b], (dc >> b) & 1);
-    // EOB: bit-length prefix (no terminator at 13), then the bits below the MSB
-    {
-        const int k = eob ? 32 - __clz(eob) : 0;
-        #pragma unroll
-        for (int i = 0; i < 13; ++i) {
-            if (i < k) e.bit_once(qt->eob[i], 1);
-            else if (i == k) e.bit_once(qt->eob[i], 0);
-        }
-        if (k > 1) e.bypass_n(eob & ((1u << (k - 1)) - 1u), k - 1);
-    }
-    uint32_t m[12];
-    #pragma unroll
-    for (int i = 0; i < 12; ++i) m[i] = qt->mag[i];
-    uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-    int band = -1;
-    uint32_t next_band = 0, hist = 0;
-    const int16_t *L = levels + gb * kNVox;
-    for (uint32_t p = 0; p < eob; ++p) {
-        if (p >= next_band) {
-            band++;
-            s0 = qt->sig[band * 4 + 0]; s1 = qt->sig[band * 4 + 1];
-            s2 = qt->sig[band * 4 + 2]; s3 = qt->sig[band * 4 + 3];
-            next_band = c_band_start[band + 1];
-        }
-        const int v = L[p];
-        const int nzb = v != 0;
-        if (p + 1 != eob) {
-            uint32_t dens = __popc(hist & 0xFFFFu);
-            uint32_t p0 = dens == 0 ? s0 : dens == 1 ? s1 : dens == 2 ? s2 : s3;
-            e.bit(p0, nzb);
-            s0 = dens == 0 ? p0 : s0;
-            s1 = dens == 1 ? p0 : s1;
-            s2 = dens == 2 ? p0 : s2;
-            s3 = dens >= 3 ? p0 : s3;
-        }
-        hist = (hist << 1) | (uint32_t)nzb;
-        if (nzb) {
-            encode_magnitude(e, m, (uint32_t)(v < 0 ? -v : v));
-            e.bypass(v < 0);
-        }
-    }
-    if (ncorr) {
-        const uint32_t *C = corr + gb * kNVox;
-        e.expgolomb(ncorr - 1);
-        uint32_t prev = 0;
-        for (uint32_t i = 0; i < ncorr; ++i) {
-            const uint32_t w = C[i];
-            const uint32_t pos = w & 0xFFFFu;
-            const int d = (int)(int16_t)(w >> 16);
-            e.expgolomb(pos - prev);
-            prev = pos;
-            e.bypass(d < 0);
-            e.expgolomb((uint32_t)(d < 0 ? -d : d) - 1u);
-        }
-    }
+    encode_block_stream(e, qt, c_band_start, dc, eob, levels + gb * kNVox, ncorr, corr + gb * kNVox);
     e.finish();
     plen[gb] = e.n;
 }

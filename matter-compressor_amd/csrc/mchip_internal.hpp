@@ -6,8 +6,6 @@
 
 namespace mchip {
 
-constexpr uint32_t kStatusOk = 0;
-constexpr uint32_t kStatusFallback = 1;   // decode this block in k_recon, single lane
 
 // One chunk of a decode batch.
 struct ChunkDesc {

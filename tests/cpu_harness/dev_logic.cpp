@@ -1,0 +1,58 @@
+// CPU harness of the device coder logic: the same mchip_codec.hpp the HIP
+// kernels use, compiled for the host through the shim in hip/hip_runtime.h.
+// Test infrastructure only.
+#include "mchip_codec.hpp"
+
+using namespace mchip;
+
+extern "C" {
+
+// Entropy-decode a payload with the device decoder; expands the list into
+// raster levels (via scan) and returns corrections.  Returns the status.
+int dl_decode(const QTab *qt, const uint16_t *band_start, const uint16_t *scan, const uint8_t *payload, uint32_t len,
+              int16_t *levels, int *dc, int *flags, uint32_t *ncorr, uint32_t *cpos, int32_t *cdel) {
+    static thread_local uint32_t w[8 * 17408 + 64 + 3 * 4096 * 2];
+    uint32_t cap = 8u * len + 32u;
+    uint2 r = entropy_to_list(payload, len, qt, band_start, w, cap);
+    memset(levels, 0, 4096 * sizeof(int16_t));
+    *dc = r.x & 255;
+    *flags = (r.x >> 8) & 255;
+    uint32_t nw = r.y & 0xFFFF, nc = r.y >> 16;
+    int status = (int)(r.x >> 16);
+    if (status != 0) { *ncorr = 0; return status; }
+    for (uint32_t i = 0; i < nw; ++i) levels[scan[w[i] & 0xFFFF]] = (int16_t)(w[i] >> 16);
+    for (uint32_t i = 0; i < nc; ++i) { cpos[i] = w[nw + 2 * i]; cdel[i] = (int32_t)w[nw + 2 * i + 1]; }
+    *ncorr = nc;
+    return 0;
+}
+
+// Full single-lane decode into a per-block level array (no capacity limit).
+struct ArraySink {
+    int16_t *lv;
+    const uint16_t *scan;
+    void coef(uint32_t p, uint16_t v) { lv[scan[p]] = (int16_t)v; }
+};
+uint32_t dl_decode_levels_full(const QTab *qt, const uint16_t *band_start, const uint16_t *scan,
+                               const uint8_t *payload, uint32_t len, int16_t *levels, int *dc, int *flags) {
+    RcDec d;
+    d.init(payload, 0, len);
+    decode_header(d, qt, *dc, *flags);
+    memset(levels, 0, 4096 * sizeof(int16_t));
+    ArraySink s{levels, scan};
+    return decode_coefs(d, qt, band_start, s);
+}
+
+// Encode a block stream from scan-ordered levels and corrections.
+uint32_t dl_encode(const QTab *qt, const uint16_t *band_start, uint32_t dc, uint32_t eob, const int16_t *L,
+                   uint32_t ncorr, const uint32_t *C, uint8_t *out, uint32_t cap) {
+    RcEnc e;
+    e.init((uint32_t *)out, cap);
+    encode_block_stream(e, qt, band_start, dc, eob, L, ncorr, C);
+    e.finish();
+    return e.n;
+}
+
+// IDCT of one 16-point line with the device kernel's arithmetic.
+void dl_idct_line(float *v, const float *cmf) { idct16_line(v, cmf); }
+void dl_fdct_line(float *v, const float *cmf) { fdct16_line(v, cmf); }
+}
