@@ -206,6 +206,7 @@ struct mchip_ctx {
     void *d_plen = nullptr; size_t cap_plen = 0;
     void *d_dst = nullptr; size_t cap_dst = 0;
     void *d_res = nullptr; size_t cap_res = 0;
+    void *d_fmap = nullptr; size_t cap_fmap = 0;
     void *h_res = nullptr;
     // block jobs / crops
     void *d_jobs = nullptr; size_t cap_jobs = 0;
@@ -339,7 +340,7 @@ void mchip_ctx_destroy(mchip_ctx *c) {
     collect_timing(c);
     void *bufs[] = {c->d_cd, c->d_rec, c->d_plist, c->d_cinfo, c->d_binfo, c->d_scratch,
                     c->d_stage_in, c->d_stage_out, c->d_aux, c->d_ecd, c->d_einfo, c->d_levels,
-                    c->d_corr, c->d_slots, c->d_plen, c->d_dst, c->d_res,
+                    c->d_corr, c->d_slots, c->d_plen, c->d_dst, c->d_res, c->d_fmap,
                     c->d_jobs, c->d_jscratch, c->d_jinfo, c->d_bmap, c->d_arena};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
@@ -541,7 +542,8 @@ int mchip_encode_chunks(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride, c
             ensure(c, &c->d_slots, &c->cap_slots, (size_t)kSlotBytes * nb) ||
             ensure(c, &c->d_plen, &c->cap_plen, sizeof(uint32_t) * nb) ||
             ensure(c, &c->d_dst, &c->cap_dst, sizeof(uint32_t) * nb) ||
-            ensure(c, &c->d_res, &c->cap_res, sizeof(EncChunkResult) * kEncBatch))
+            ensure(c, &c->d_res, &c->cap_res, sizeof(EncChunkResult) * kEncBatch) ||
+            ensure(c, &c->d_fmap, &c->cap_fmap, (size_t)kFmapStride * kEncBatch))
             return -1;
         if (!c->h_res) HIPCHK(hipHostMalloc(&c->h_res, sizeof(EncChunkResult) * kEncBatch, hipHostMallocDefault));
         HIPCHK(hipMemcpyAsync(c->d_ecd, cd, sizeof(EncChunkDesc) * m, hipMemcpyHostToDevice, c->stream));
@@ -559,6 +561,7 @@ int mchip_encode_chunks(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride, c
         a.dst = (uint32_t *)c->d_dst;
         a.out = d_out;
         a.res = (EncChunkResult *)c->d_res;
+        a.fmap = (uint32_t *)c->d_fmap;
         a.n = m;
         timed(c, "enc_transform", [&] { launch_enc_transform(a, c->stream); });
         timed(c, "enc_entropy", [&] { launch_enc_entropy(a, c->stream); });

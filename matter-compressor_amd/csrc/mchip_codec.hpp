@@ -72,15 +72,39 @@ __device__ __forceinline__ uint32_t decode_coefs(RcDec &d, const QTab *__restric
     return nsig;
 }
 
-// List entries hold the scan position; the reconstruction maps it to the
-// raster index (keeps the table lookup off the serial decode chain).
+// Word list of one block (coefficients as scan position | level << 16, then
+// corrections as pos, delta word pairs).  Words go through a small staging
+// area (LDS on the GPU, lane-interleaved) and reach global memory in 16-byte
+// stores every kStage words, so the serial decode never waits on its own
+// stores: on CDNA the vector-memory counter also counts stores, and a store
+// still in flight would stall the next payload-word wait.
+constexpr uint32_t kStage = 64;
 struct ListSink {
-    uint32_t *w;
-    uint32_t n, cap;
-    __device__ __forceinline__ void coef(uint32_t p, uint16_t lv) {
-        if (n < cap) w[n] = p | ((uint32_t)lv << 16);
+    uint32_t *w;          // global destination, 16-byte aligned
+    uint32_t *st;         // staging slot 0 of this lane
+    uint32_t stride;      // words between consecutive staging slots
+    uint32_t n, cap;      // words offered (may exceed cap) and capacity
+    uint32_t base, nst;   // first unflushed index and staged count
+    __device__ __forceinline__ void init(uint32_t *dst, uint32_t capacity, uint32_t *stage, uint32_t stage_stride) {
+        w = dst; cap = capacity; st = stage; stride = stage_stride; n = 0; base = 0; nst = 0;
+    }
+    __device__ __forceinline__ void flush() {
+        uint32_t i = 0;
+        for (; i + 4 <= nst; i += 4)
+            *(uint4 *)(w + base + i) = make_uint4(st[i * stride], st[(i + 1) * stride], st[(i + 2) * stride],
+                                                  st[(i + 3) * stride]);
+        for (; i < nst; ++i) w[base + i] = st[i * stride];
+        base += nst;
+        nst = 0;
+    }
+    __device__ __forceinline__ void put(uint32_t v) {
+        if (n < cap) {
+            st[nst * stride] = v;
+            if (++nst == kStage) flush();
+        }
         n++;
     }
+    __device__ __forceinline__ void coef(uint32_t p, uint16_t lv) { put(p | ((uint32_t)lv << 16)); }
 };
 
 __device__ __forceinline__ void decode_header(RcDec &d, const QTab *__restrict__ qt, int &dc, int &flags) {
@@ -96,13 +120,15 @@ __device__ __forceinline__ void decode_header(RcDec &d, const QTab *__restrict__
 // words), corrections appended as (pos, delta) word pairs.  Returns the
 // block record: x = dc | flags << 8 | status << 16, y = nsig | ncorr << 16.
 __device__ __forceinline__ uint2 entropy_to_list(const uint8_t *payload, uint32_t len, const QTab *__restrict__ qt,
-                                                 const uint16_t *band_start, uint32_t *w, uint32_t cap) {
+                                                 const uint16_t *band_start, uint32_t *w, uint32_t cap,
+                                                 uint32_t *stage, uint32_t stage_stride) {
     RcDec d;
     d.init(payload, 0, len);
     int dc, flags;
     decode_header(d, qt, dc, flags);
-    ListSink sink{w, 0u, cap};
-    decode_coefs(d, qt, band_start, sink);
+    ListSink sink;
+    sink.init(w, cap, stage, stage_stride);
+    const uint32_t nsig = decode_coefs(d, qt, band_start, sink);
     uint32_t nc = 0;
     if (flags & 2) {
         uint32_t n = d.expgolomb() + 1, pos = 0;
@@ -112,13 +138,14 @@ __device__ __forceinline__ uint2 entropy_to_list(const uint8_t *payload, uint32_
             int neg = d.bypass();
             uint32_t m = d.expgolomb() + 1;
             if (pos >= (uint32_t)kNVox) break;
-            uint32_t at = sink.n + 2 * nc;
-            if (at + 2 <= sink.cap) { sink.w[at] = pos; sink.w[at + 1] = neg ? 0u - m : m; }
+            sink.put(pos);
+            sink.put(neg ? 0u - m : m);
             nc++;
         }
     }
-    uint32_t status = (sink.n + 2 * nc > sink.cap) ? kStatusFallback : kStatusOk;
-    return make_uint2((uint32_t)dc | ((uint32_t)flags << 8) | (status << 16), sink.n | (nc << 16));
+    sink.flush();
+    uint32_t status = sink.n > sink.cap ? kStatusFallback : kStatusOk;
+    return make_uint2((uint32_t)dc | ((uint32_t)flags << 8) | (status << 16), nsig | (nc << 16));
 }
 
 __device__ __forceinline__ void encode_magnitude(RcEnc &e, uint32_t (&m)[12], uint32_t mag) {
@@ -135,6 +162,32 @@ __device__ __forceinline__ void encode_magnitude(RcEnc &e, uint32_t (&m)[12], ui
     }
     if (!done) { e.bit(m[11], 1); e.expgolomb(rest); }
 }
+
+// Sequential reader of a 16-byte aligned u32 array that keeps the next
+// 16 bytes in flight (the coding chain never waits on a load it just issued).
+struct Prefetch4 {
+    const uint4 *p;
+    uint4 cur, nxt;
+    uint32_t grp, ngrp;
+    __device__ __forceinline__ void init(const uint32_t *base, uint32_t nwords) {
+        p = (const uint4 *)base;
+        ngrp = (nwords + 3) / 4;
+        grp = 0;
+        cur = ngrp > 0 ? p[0] : make_uint4(0, 0, 0, 0);
+        nxt = ngrp > 1 ? p[1] : make_uint4(0, 0, 0, 0);
+    }
+    // word i; calls must come in non-decreasing i
+    __device__ __forceinline__ uint32_t get(uint32_t i) {
+        const uint32_t g = i >> 2;
+        if (g != grp) {
+            cur = g == grp + 1 ? nxt : p[g];
+            grp = g;
+            nxt = g + 1 < ngrp ? p[g + 1] : make_uint4(0, 0, 0, 0);
+        }
+        const uint32_t k = i & 3;
+        return k == 0 ? cur.x : k == 1 ? cur.y : k == 2 ? cur.z : cur.w;
+    }
+};
 
 // Stream of one block: header bins, EOB, coefficient bins over the
 // scan-ordered levels L[0..eob), corrections C[0..ncorr) as (pos | delta<<16)
@@ -162,6 +215,8 @@ __device__ __forceinline__ void encode_block_stream(RcEnc &e, const QTab *__rest
     uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
     int band = -1;
     uint32_t next_band = 0, hist = 0;
+    Prefetch4 lr;
+    lr.init((const uint32_t *)L, (eob + 1) / 2);
     for (uint32_t p = 0; p < eob; ++p) {
         if (p >= next_band) {
             band++;
@@ -169,7 +224,7 @@ __device__ __forceinline__ void encode_block_stream(RcEnc &e, const QTab *__rest
             s2 = qt->sig[band * 4 + 2]; s3 = qt->sig[band * 4 + 3];
             next_band = band_start[band + 1];
         }
-        const int v = L[p];
+        const int v = (int)(int16_t)(lr.get(p >> 1) >> (16 * (p & 1)));
         const int nzb = v != 0;
         if (p + 1 != eob) {
             uint32_t dens = __popc(hist & 0xFFFFu);
@@ -189,8 +244,10 @@ __device__ __forceinline__ void encode_block_stream(RcEnc &e, const QTab *__rest
     if (ncorr) {
         e.expgolomb(ncorr - 1);
         uint32_t prev = 0;
+        Prefetch4 cr;
+        cr.init(C, ncorr);
         for (uint32_t i = 0; i < ncorr; ++i) {
-            const uint32_t w = C[i];
+            const uint32_t w = cr.get(i);
             const uint32_t pos = w & 0xFFFFu;
             const int d = (int)(int16_t)(w >> 16);
             e.expgolomb(pos - prev);

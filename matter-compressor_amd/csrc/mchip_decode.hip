@@ -108,9 +108,20 @@ __global__ __launch_bounds__(64) void k_entropy(const uint8_t *__restrict__ blob
                                                 const ChunkInfo *__restrict__ cinfo,
                                                 uint32_t *__restrict__ scratch,
                                                 uint2 *__restrict__ binfo) {
+    __shared__ uint32_t stage[kStage * 64];
+    __shared__ QTab sq;
+    __shared__ uint16_t sband[9];
     const int c = blockIdx.y;
-    const uint32_t slot = blockIdx.x * 64 + threadIdx.x;
     const uint32_t np = cinfo[c].np;
+    if (blockIdx.x * 64 >= np) return;
+    {   // the chunk's priors and band starts into LDS (off the global-memory path)
+        const uint32_t *src = (const uint32_t *)(qtabs + cd[c].qtab);
+        uint32_t *dst = (uint32_t *)&sq;
+        for (uint32_t i = threadIdx.x; i < sizeof(QTab) / 4; i += 64) dst[i] = src[i];
+        if (threadIdx.x < 9) sband[threadIdx.x] = c_band_start[threadIdx.x];
+    }
+    __syncthreads();
+    const uint32_t slot = blockIdx.x * 64 + threadIdx.x;
     if (slot >= np) return;
     const size_t cb = (size_t)c * kNVox;
     const int bi = plist[cb + slot];
@@ -119,10 +130,9 @@ __global__ __launch_bounds__(64) void k_entropy(const uint8_t *__restrict__ blob
         binfo[cb + bi] = make_uint2(kStatusFallback << 16, 0);
         return;
     }
-    const QTab *qt = qtabs + cd[c].qtab;
     uint64_t wofs = cd[c].scratch_off + 8ull * (r.off - cinfo[c].pay0) + 32ull * r.slot;
-    binfo[cb + bi] = entropy_to_list(blobs + cd[c].blob_off + r.off, r.len, qt, c_band_start, scratch + wofs,
-                                     8u * r.len + 32u);
+    binfo[cb + bi] = entropy_to_list(blobs + cd[c].blob_off + r.off, r.len, &sq, sband, scratch + wofs,
+                                     8u * r.len + 32u, stage + threadIdx.x, 64);
 }
 
 // ---------------------------------------------------------------------------
@@ -353,8 +363,9 @@ __global__ __launch_bounds__(64) void k_entropy_jobs(const uint8_t *__restrict__
     if (i >= n) return;
     const BlockJob j = jobs[i];
     if (j.len == 0) { binfo[i] = make_uint2(kStatusFallback << 16, 0); return; }
+    __shared__ uint32_t stage[kStage * 64];
     binfo[i] = entropy_to_list(base + j.pay_off, j.len, qtabs + j.qtab, c_band_start, scratch + j.scratch_off,
-                               8u * j.len + 32u);
+                               8u * j.len + 32u, stage + threadIdx.x, 64);
 }
 
 __global__ __launch_bounds__(256) void k_recon_jobs(const uint8_t *__restrict__ base,

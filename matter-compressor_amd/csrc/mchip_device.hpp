@@ -211,23 +211,45 @@ __device__ __forceinline__ void fdct16_line(float v[16], const float *cm) {
 
 namespace mchip {
 
-// Range encoder writing bytes into a 4-byte aligned buffer (global or LDS)
-// through a word accumulator (src/mc_codec.c:178-195, :209-233).  Bytes
-// beyond cap are counted but not stored; n is the full payload length.
+// Range encoder (src/mc_codec.c:178-195, :209-233) writing bytes into a
+// 16-byte aligned buffer through a word accumulator.  With a staging area
+// (LDS on the GPU) completed words are staged and flushed in 16-byte stores
+// every kEncStage words, keeping stores off the serial coding chain; without
+// one (stage == nullptr) words go straight to out.  Bytes beyond cap are
+// counted but not stored; n is the full payload length.
+constexpr uint32_t kEncStage = 32;
 struct RcEnc {
     uint32_t *out;
+    uint32_t *st;
+    uint32_t stride, base, nst;
     uint32_t cap, n, wbuf;
     uint64_t low;
     uint32_t range, cache, pending;
 
-    __device__ __forceinline__ void init(uint32_t *o, uint32_t capacity) {
-        out = o; cap = capacity; n = 0; wbuf = 0;
+    __device__ __forceinline__ void init(uint32_t *o, uint32_t capacity, uint32_t *stage = nullptr,
+                                         uint32_t stage_stride = 1) {
+        out = o; st = stage; stride = stage_stride; base = 0; nst = 0;
+        cap = capacity; n = 0; wbuf = 0;
         low = 0; range = 0xFFFFFFFFu; cache = 0; pending = 1;
+    }
+    __device__ __forceinline__ void flush_stage() {
+        uint32_t i = 0;
+        for (; i + 4 <= nst; i += 4)
+            *(uint4 *)(out + base + i) = make_uint4(st[i * stride], st[(i + 1) * stride], st[(i + 2) * stride],
+                                                    st[(i + 3) * stride]);
+        for (; i < nst; ++i) out[base + i] = st[i * stride];
+        base += nst;
+        nst = 0;
+    }
+    __device__ __forceinline__ void put_word(uint32_t w) {
+        if (!st) { out[base++] = w; return; }
+        st[nst * stride] = w;
+        if (++nst == kEncStage) flush_stage();
     }
     __device__ __forceinline__ void put(uint32_t b) {
         if (n < cap) {
             wbuf |= (b & 0xFFu) << (8 * (n & 3));
-            if ((n & 3) == 3) { out[n >> 2] = wbuf; wbuf = 0; }
+            if ((n & 3) == 3) { put_word(wbuf); wbuf = 0; }
         }
         n++;
     }
@@ -247,8 +269,9 @@ struct RcEnc {
     }
     __device__ __forceinline__ void bit(uint32_t &p0, int b) {
         uint32_t r0 = __umul24(range >> 12, p0);
-        if (!b) { range = r0; p0 += (4096u - p0) >> 4; }
-        else    { low += r0; range -= r0; p0 -= p0 >> 4; }
+        low += b ? (uint64_t)r0 : 0ull;
+        range = b ? range - r0 : r0;
+        p0 = b ? p0 - (p0 >> 4) : p0 + ((4096u - p0) >> 4);
         norm();
     }
     __device__ __forceinline__ void bit_once(uint32_t p0, int b) { uint32_t p = p0; bit(p, b); }
@@ -275,7 +298,8 @@ struct RcEnc {
     __device__ __forceinline__ void finish() {
         #pragma unroll 1
         for (int i = 0; i < 5; ++i) shift_low();
-        if ((n & 3) && n < cap) out[n >> 2] = wbuf;
+        if ((n & 3) && n < cap) put_word(wbuf);
+        if (st) flush_stage();
     }
 };
 

@@ -242,6 +242,30 @@ __global__ __launch_bounds__(256) void k_enc_transform(const uint8_t *__restrict
 // ---------------------------------------------------------------------------
 // k_enc_entropy: one lane per block (blockIdx.y = chunk)
 // ---------------------------------------------------------------------------
+// Fraction map of one chunk (src/mc_codec.c:600-611): the 64 lanes stage the
+// 4096 nibbles in LDS, lane 0 range-codes them.  Output at fm: [u32 length]
+// [bytes]; the length is clamped to the buffer like the parallel-append
+// stitch does (src/mc_archive.c:885-887).
+__device__ void encode_fracmap_wg(const uint4 *__restrict__ ei, uint32_t *__restrict__ fm, uint32_t *lds) {
+    uint8_t *frac = (uint8_t *)lds;                 // 4096 bytes of the staging area
+    for (int i = threadIdx.x; i < kNVox; i += 64) frac[i] = (uint8_t)((ei[i].x >> 8) & 15u);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    RcEnc e;
+    e.init(fm + 1, kFmapCap);
+    uint32_t cx[3][4];
+    for (int b = 0; b < 3; ++b)
+        for (int i = 0; i < 4; ++i) cx[b][i] = 2048;
+    int prev = 0;
+    for (int i = 0; i < kNVox; ++i) {
+        const int v = frac[i], pb = prev == 0 ? 0 : (prev == 15 ? 2 : 1);
+        for (int b = 3; b >= 0; --b) e.bit(cx[pb][b], (v >> b) & 1);
+        prev = v;
+    }
+    e.finish();
+    fm[0] = e.n < (uint32_t)kFmapCap ? e.n : (uint32_t)kFmapCap;
+}
+
 template <bool kBlocks>
 __global__ __launch_bounds__(64) void k_enc_entropy(const EncChunkDesc *__restrict__ cd,
                                                     const QTab *__restrict__ qtabs,
@@ -249,18 +273,36 @@ __global__ __launch_bounds__(64) void k_enc_entropy(const EncChunkDesc *__restri
                                                     const int16_t *__restrict__ levels,
                                                     const uint32_t *__restrict__ corr,
                                                     uint32_t *__restrict__ slots,
-                                                    uint32_t *__restrict__ plen, int nblocks) {
+                                                    uint32_t *__restrict__ plen, uint32_t *__restrict__ fmapbuf,
+                                                    int nblocks) {
+    __shared__ uint32_t stage[kEncStage * 64];
+    __shared__ QTab sq;
+    __shared__ uint16_t sband[9];
+    if (!kBlocks && blockIdx.x == kNVox / 64) {   // one extra workgroup per chunk: the fraction map
+        encode_fracmap_wg(einfo + (size_t)blockIdx.y * kNVox, fmapbuf + (size_t)blockIdx.y * (kFmapStride / 4),
+                          stage);
+        return;
+    }
     const int bi = blockIdx.x * 64 + threadIdx.x;
+    const QTab *qt;
+    if (!kBlocks) {   // one chunk per workgroup: its priors and band starts into LDS
+        const uint32_t *src = (const uint32_t *)(qtabs + cd[blockIdx.y].qtab);
+        uint32_t *dst = (uint32_t *)&sq;
+        for (uint32_t i = threadIdx.x; i < sizeof(QTab) / 4; i += 64) dst[i] = src[i];
+        if (threadIdx.x < 9) sband[threadIdx.x] = c_band_start[threadIdx.x];
+        __syncthreads();
+    }
     if (kBlocks && bi >= nblocks) return;
     const int c = kBlocks ? bi : (int)blockIdx.y;
     const size_t gb = kBlocks ? (size_t)bi : (size_t)c * kNVox + bi;
     const uint4 inf = einfo[gb];
     if (!(inf.x & (1u << 12))) { plen[gb] = 0; return; }
-    const QTab *qt = qtabs + cd[c].qtab;
+    qt = kBlocks ? qtabs + cd[c].qtab : &sq;
     const uint32_t dc = inf.x & 255u, eob = inf.z, ncorr = inf.w;
     RcEnc e;
-    e.init(slots + gb * (kSlotBytes / 4), kSlotBytes);
-    encode_block_stream(e, qt, c_band_start, dc, eob, levels + gb * kNVox, ncorr, corr + gb * kNVox);
+    e.init(slots + gb * (kSlotBytes / 4), kSlotBytes, stage + threadIdx.x, 64);
+    encode_block_stream(e, qt, kBlocks ? c_band_start : sband, dc, eob, levels + gb * kNVox, ncorr,
+                        corr + gb * kNVox);
     e.finish();
     plen[gb] = e.n;
 }
@@ -271,13 +313,11 @@ __global__ __launch_bounds__(64) void k_enc_entropy(const EncChunkDesc *__restri
 __global__ __launch_bounds__(256) void k_enc_assemble(const EncChunkDesc *__restrict__ cd,
                                                       const uint4 *__restrict__ einfo,
                                                       const uint32_t *__restrict__ plen,
+                                                      const uint32_t *__restrict__ fmapbuf,
                                                       uint32_t *__restrict__ dst,
                                                       uint8_t *__restrict__ out,
                                                       EncChunkResult *__restrict__ res) {
-    __shared__ uint8_t frac[kNVox];
-    __shared__ __attribute__((aligned(16))) uint32_t fmap_w[(kFmapCap + 16) / 4];
     __shared__ uint32_t lds4[4];
-    __shared__ uint32_t s_fml;
     const int c = blockIdx.x, t = threadIdx.x;
     const size_t cb = (size_t)c * kNVox;
     uint32_t bits = 0, lsum = 0, bad = 0;
@@ -287,7 +327,6 @@ __global__ __launch_bounds__(256) void k_enc_assemble(const EncChunkDesc *__rest
         const int bi = t * 16 + j;
         const uint4 inf = einfo[cb + bi];
         const uint32_t present = (inf.x >> 12) & 1u;
-        frac[bi] = (uint8_t)((inf.x >> 8) & 15u);
         lj[j] = present ? plen[cb + bi] : 0u;
         bits |= present << j;
         lsum += lj[j];
@@ -298,25 +337,8 @@ __global__ __launch_bounds__(256) void k_enc_assemble(const EncChunkDesc *__rest
     uint32_t paytot;
     const uint32_t pbase = wg_excl_scan(lsum, lds4, paytot);
     const uint32_t nbad = wg_sum(bad, lds4);
-    // fraction map: one lane, into LDS (src/mc_codec.c:600-611; the
-    // parallel-append stitch clamps its length to the buffer, :885-887)
-    if (t == 0) {
-        RcEnc e;
-        e.init(fmap_w, kFmapCap);
-        uint32_t cx[3][4];
-        for (int b = 0; b < 3; ++b)
-            for (int i = 0; i < 4; ++i) cx[b][i] = 2048;
-        int prev = 0;
-        for (int i = 0; i < kNVox; ++i) {
-            const int v = frac[i] & 15, pb = prev == 0 ? 0 : (prev == 15 ? 2 : 1);
-            for (int b = 3; b >= 0; --b) e.bit(cx[pb][b], (v >> b) & 1);
-            prev = v;
-        }
-        e.finish();
-        s_fml = e.n < (uint32_t)kFmapCap ? e.n : (uint32_t)kFmapCap;
-    }
-    __syncthreads();
-    const uint32_t fml = s_fml;
+    const uint32_t *fm = fmapbuf + (size_t)c * (kFmapStride / 4);
+    const uint32_t fml = fm[0];
     const uint64_t bm_off = 14ull + fml, lens_off = bm_off + 512, pay0 = lens_off + 2ull * np;
     const uint64_t total = np ? pay0 + paytot : 0;
     const bool fits = total <= cd[c].out_cap && nbad == 0;
@@ -335,8 +357,8 @@ __global__ __launch_bounds__(256) void k_enc_assemble(const EncChunkDesc *__rest
         o[12] = (uint8_t)fml;
         o[13] = (uint8_t)(fml >> 8);
     }
-    const uint8_t *fm = (const uint8_t *)fmap_w;
-    for (uint32_t i = t; i < fml; i += 256) o[14 + i] = fm[i];
+    const uint8_t *fmb = (const uint8_t *)(fm + 1);
+    for (uint32_t i = t; i < fml; i += 256) o[14 + i] = fmb[i];
     o[bm_off + 2 * t] = (uint8_t)bits;
     o[bm_off + 2 * t + 1] = (uint8_t)(bits >> 8);
     uint32_t s = slot0, at = pbase;
@@ -423,7 +445,16 @@ __device__ uint64_t xxh64_wave4(const uint8_t *p, uint64_t len, uint64_t seed) {
     if (lane < 4) {
         v = lane == 0 ? seed + XP1 + XP2 : lane == 1 ? seed + XP2 : lane == 2 ? seed : seed - XP1;
         const uint8_t *q = p + 8 * lane;
-        for (uint64_t s = 0; s < nstripe; ++s) v = xround(v, ld64a(q + 32 * s));
+        uint64_t s = 0;
+        // eight stripes per step: the loads are independent of the chain
+        for (; s + 8 <= nstripe; s += 8) {
+            uint64_t in[8];
+            #pragma unroll
+            for (int u = 0; u < 8; ++u) in[u] = ld64a(q + 32 * (s + u));
+            #pragma unroll
+            for (int u = 0; u < 8; ++u) v = xround(v, in[u]);
+        }
+        for (; s < nstripe; ++s) v = xround(v, ld64a(q + 32 * s));
     }
     const uint64_t v1 = __shfl(v, 0, 64), v2 = __shfl(v, 1, 64), v3 = __shfl(v, 2, 64), v4 = __shfl(v, 3, 64);
     if (lane == 0) {
@@ -465,18 +496,19 @@ void launch_enc_transform(const EncodeArgs &a, hipStream_t s) {
                        a.einfo, a.levels, a.corr, a.n);
 }
 void launch_enc_entropy(const EncodeArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_enc_entropy<false>, dim3(kNVox / 64, a.n), dim3(64), 0, s, a.cd, a.qtabs, a.einfo,
-                       a.levels, a.corr, a.slots, a.plen, 0);
+    hipLaunchKernelGGL(k_enc_entropy<false>, dim3(kNVox / 64 + 1, a.n), dim3(64), 0, s, a.cd, a.qtabs, a.einfo,
+                       a.levels, a.corr, a.slots, a.plen, a.fmap, 0);
 }
 // independent blocks: a.n blocks, descriptor per block
 void launch_enc_blocks(const EncodeArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_enc_transform<true>, dim3((unsigned)a.n), dim3(256), 0, s, a.in, a.cd, a.qtabs, a.einfo,
                        a.levels, a.corr, a.n);
     hipLaunchKernelGGL(k_enc_entropy<true>, dim3(((unsigned)a.n + 63) / 64), dim3(64), 0, s, a.cd, a.qtabs,
-                       a.einfo, a.levels, a.corr, a.slots, a.plen, a.n);
+                       a.einfo, a.levels, a.corr, a.slots, a.plen, nullptr, a.n);
 }
 void launch_enc_assemble(const EncodeArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_enc_assemble, dim3(a.n), dim3(256), 0, s, a.cd, a.einfo, a.plen, a.dst, a.out, a.res);
+    hipLaunchKernelGGL(k_enc_assemble, dim3(a.n), dim3(256), 0, s, a.cd, a.einfo, a.plen, a.fmap, a.dst, a.out,
+                       a.res);
 }
 void launch_enc_copy(const EncodeArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_enc_copy, dim3((unsigned)a.n * kNVox), dim3(64), 0, s, a.cd, a.einfo, a.plen, a.dst,

@@ -46,6 +46,7 @@ struct DecodeArgs {
 // ---- encode ----------------------------------------------------------------
 constexpr int kSlotBytes = 17408;          // per-block payload cap (src/mc_codec.c:442, :732)
 constexpr int kFmapCap = 4096 / 2 + 64;    // fraction-map buffer (src/mc_archive.c:884-887)
+constexpr int kFmapStride = 2176;          // per-chunk fraction-map record: u32 length + bytes
 constexpr uint32_t kEncSlotOverflow = 1;   // a block payload exceeded kSlotBytes
 constexpr uint32_t kEncOutTooSmall = 2;    // blob longer than the caller's capacity
 
@@ -77,6 +78,7 @@ struct EncodeArgs {
     uint32_t *slots;        // [n*4096*kSlotBytes/4]
     uint32_t *plen;         // [n*4096]
     uint32_t *dst;          // [n*4096] payload offset in the blob
+    uint32_t *fmap;         // [n*kFmapStride/4] fraction-map records
     uint8_t *out;
     EncChunkResult *res;    // [n]
     int n;

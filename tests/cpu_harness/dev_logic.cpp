@@ -13,7 +13,8 @@ int dl_decode(const QTab *qt, const uint16_t *band_start, const uint16_t *scan, 
               int16_t *levels, int *dc, int *flags, uint32_t *ncorr, uint32_t *cpos, int32_t *cdel) {
     static thread_local uint32_t w[8 * 17408 + 64 + 3 * 4096 * 2];
     uint32_t cap = 8u * len + 32u;
-    uint2 r = entropy_to_list(payload, len, qt, band_start, w, cap);
+    uint32_t stage[kStage];
+    uint2 r = entropy_to_list(payload, len, qt, band_start, w, cap, stage, 1);
     memset(levels, 0, 4096 * sizeof(int16_t));
     *dc = r.x & 255;
     *flags = (r.x >> 8) & 255;
@@ -46,7 +47,8 @@ uint32_t dl_decode_levels_full(const QTab *qt, const uint16_t *band_start, const
 uint32_t dl_encode(const QTab *qt, const uint16_t *band_start, uint32_t dc, uint32_t eob, const int16_t *L,
                    uint32_t ncorr, const uint32_t *C, uint8_t *out, uint32_t cap) {
     RcEnc e;
-    e.init((uint32_t *)out, cap);
+    uint32_t stage[kEncStage];
+    e.init((uint32_t *)out, cap, stage, 1);
     encode_block_stream(e, qt, band_start, dc, eob, L, ncorr, C);
     e.finish();
     return e.n;

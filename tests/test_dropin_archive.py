@@ -196,7 +196,8 @@ def test_gpu_append_paths_byte_identical_to_compressed_install():
     print("append raw_q / ctx / par", flush=True)
     assert L.mc_archive_append_chunk_raw_q(a, 0, 0, 0, 0, ob.p(ch[0]), C.c_float(6.0)) == 0
     assert L.mc_archive_append_chunk_ctx(a, ctx, 0, 0, 0, 1, ob.p(ch[1])) == 0
-    assert L.mc_archive_append_chunk_par(a, 0, 0, 1, 1, 0, ob.p(ch[2]), C.c_float(16.0), 0) == 0 or True
+    assert L.mc_archive_append_chunk_par(a, 0, 0, 1, 1, ob.p(ch[2]), C.c_float(16.0), 0) == 0
+    co_par = L.mc_archive_chunk_offset(a, 0, 0, 1, 1)
     for (lod, cz, cy, cx), bl in zip(PLACES[:2], blobs[:2]):
         assert L.mc_archive_append_chunk_compressed(b, lod, cz, cy, cx, bl, len(bl)) == 0
     L.mc_codec_ctx_free(ctx)
@@ -208,6 +209,7 @@ def test_gpu_append_paths_byte_identical_to_compressed_install():
     da, db = open(p1, "rb").read(), open(p2, "rb").read()
     for ca, bl in zip(co_a, blobs[:2]):
         assert da[ca:ca + len(bl)] == bl
+    assert da[co_par:co_par + len(blobs[2])] == blobs[2]
 
 
 @pytest.mark.gpu
