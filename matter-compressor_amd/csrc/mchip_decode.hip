@@ -59,6 +59,7 @@ __global__ __launch_bounds__(256) void k_index(const uint8_t *__restrict__ blobs
                                                uint16_t *__restrict__ plist,
                                                ChunkInfo *__restrict__ cinfo) {
     __shared__ uint32_t lds4[4];
+    __shared__ uint32_t keys[kNVox];
     const int c = blockIdx.x, t = threadIdx.x;
     const uint8_t *b = blobs + cd[c].blob_off;
     const uint64_t L = cd[c].blob_len;
@@ -84,16 +85,35 @@ __global__ __launch_bounds__(256) void k_index(const uint8_t *__restrict__ blobs
     for (int j = 0, s = (int)slot0; j < 16; ++j) {
         int bi = t * 16 + j;
         BlkRec r = {0, 0, 0, 0};
+        uint32_t key = 0xFFFFFFFFu;                  // absent blocks sort last
         if ((bits >> j) & 1) {
             uint64_t a = lens + 2ull * s;
             uint32_t len = b[a] | ((uint32_t)b[a + 1] << 8);
             if (off + len <= L) { r.off = (uint32_t)off; r.len = len; r.slot = (uint32_t)s; }
-            plist[(size_t)c * kNVox + s] = (uint16_t)bi;
+            key = ((65535u - len) << 12) | (uint32_t)bi;   // longest payload first
             off += len;
             s++;
         }
+        keys[bi] = key;
         rec[(size_t)c * kNVox + bi] = r;
     }
+    // Lane order of the entropy kernel: present blocks by decreasing payload
+    // length, so each wave holds blocks of similar length and the longest
+    // chains start first (bitonic sort of 4096 keys in LDS).
+    __syncthreads();
+    for (uint32_t k = 2; k <= (uint32_t)kNVox; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = t; i < (uint32_t)kNVox; i += 256) {
+                const uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const uint32_t a = keys[i], bb = keys[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((a > bb) == up) { keys[i] = bb; keys[ixj] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    for (uint32_t i = t; i < np; i += 256) plist[(size_t)c * kNVox + i] = (uint16_t)(keys[i] & 4095u);
     if (t == 0) { cinfo[c].np = np; cinfo[c].pay0 = (uint32_t)pay0; }
 }
 
@@ -109,6 +129,7 @@ __global__ __launch_bounds__(64) void k_entropy(const uint8_t *__restrict__ blob
                                                 uint32_t *__restrict__ scratch,
                                                 uint2 *__restrict__ binfo) {
     __shared__ uint32_t stage[kStage * 64];
+    __shared__ uint32_t sctx[16 * 64];
     __shared__ QTab sq;
     __shared__ uint16_t sband[9];
     const int c = blockIdx.y;
@@ -132,7 +153,7 @@ __global__ __launch_bounds__(64) void k_entropy(const uint8_t *__restrict__ blob
     }
     uint64_t wofs = cd[c].scratch_off + 8ull * (r.off - cinfo[c].pay0) + 32ull * r.slot;
     binfo[cb + bi] = entropy_to_list(blobs + cd[c].blob_off + r.off, r.len, &sq, sband, scratch + wofs,
-                                     8u * r.len + 32u, stage + threadIdx.x, 64);
+                                     8u * r.len + 32u, stage + threadIdx.x, 64, sctx + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -364,8 +385,9 @@ __global__ __launch_bounds__(64) void k_entropy_jobs(const uint8_t *__restrict__
     const BlockJob j = jobs[i];
     if (j.len == 0) { binfo[i] = make_uint2(kStatusFallback << 16, 0); return; }
     __shared__ uint32_t stage[kStage * 64];
+    __shared__ uint32_t sctx[16 * 64];
     binfo[i] = entropy_to_list(base + j.pay_off, j.len, qtabs + j.qtab, c_band_start, scratch + j.scratch_off,
-                               8u * j.len + 32u, stage + threadIdx.x, 64);
+                               8u * j.len + 32u, stage + threadIdx.x, 64, sctx + threadIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_recon_jobs(const uint8_t *__restrict__ base,

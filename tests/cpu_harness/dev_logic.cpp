@@ -13,8 +13,8 @@ int dl_decode(const QTab *qt, const uint16_t *band_start, const uint16_t *scan, 
               int16_t *levels, int *dc, int *flags, uint32_t *ncorr, uint32_t *cpos, int32_t *cdel) {
     static thread_local uint32_t w[8 * 17408 + 64 + 3 * 4096 * 2];
     uint32_t cap = 8u * len + 32u;
-    uint32_t stage[kStage];
-    uint2 r = entropy_to_list(payload, len, qt, band_start, w, cap, stage, 1);
+    uint32_t stage[kStage], ctx[16];
+    uint2 r = entropy_to_list(payload, len, qt, band_start, w, cap, stage, 1, ctx);
     memset(levels, 0, 4096 * sizeof(int16_t));
     *dc = r.x & 255;
     *flags = (r.x >> 8) & 255;
