@@ -81,12 +81,16 @@ def main():
     ap.add_argument("--workload", default="plain64_q6")
     ap.add_argument("--batched", action="store_true", help="also time all chunks in one launch")
     ap.add_argument("--streams", type=int, default=8, help="HIP streams the per-chunk launches rotate over")
+    ap.add_argument("--tune", action="append", default=[], help="key=value launch tuning (mchip_set_tuning)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
               "the codec shards by chunk with no cross-GPU exchange", file=sys.stderr)
         sys.exit(2)
 
+    for kv in args.tune:
+        k, v = kv.split("=")
+        mc_amd.set_tuning(k, int(v))
     specs = workloads.WORKLOADS[args.workload]()
     man = json.load(open(os.path.join(ROOT, "tests", "golden", "chunk_manifest.json")))[args.workload]
     n = len(specs)
@@ -240,7 +244,7 @@ def main():
         "bit_exact": bad == 0,
         "config": {"workload": args.workload, "chunks": n, "chunk": "256^3 u8", "q": 6.0, "air": 0.419,
                    "decoded_bytes_per_step": n * mc_amd.CHUNK_BYTES, "blob_bytes": blob_bytes,
-                   "launch": "one chunk per launch", "streams": args.streams},
+                   "launch": "one chunk per launch", "streams": args.streams, "tuning": args.tune},
         "kernels_ms_avg": {k: round(v, 4) for k, v in kt.items()},
         "roofline": {"bound": "hbm", "kernel": "decode (index+entropy+recon per chunk)", "dominant": dom,
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -66,6 +66,12 @@ const int kHiN[8] = {12, 12, 11, 8, 4, 2, 4, 8};
 
 constexpr int kMaxQTabs = 4096;
 
+// Launch-shape tuning knobs (mchip_set_tuning); process-wide.
+struct Tuning {
+    int decoder_sm = 0;       // entropy decoder: 0 = per scan position, 1 = one bin per iteration
+};
+Tuning g_tune;
+
 struct Global {
     std::mutex mu;
     int device = -1;
@@ -321,6 +327,12 @@ int mchip_set_priors(const uint16_t *plo, const uint16_t *phi) {
     return 0;
 }
 
+int mchip_set_tuning(const char *key, int value) {
+    if (!key) return fail("null key");
+    if (!strcmp(key, "decoder_sm")) { g_tune.decoder_sm = value != 0; return 0; }
+    return fail("unknown tuning key '%s'", key);
+}
+
 mchip_ctx *mchip_ctx_create(void) {
     if (mchip_init(-1)) return nullptr;
     mchip_ctx *c = new mchip_ctx();
@@ -430,6 +442,7 @@ int mchip_decode_chunks(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *bl
     a.binfo = (uint2 *)c->d_binfo;
     a.out = d_out;
     a.n = n;
+    a.state_machine = g_tune.decoder_sm;
     timed(c, "index", [&] { launch_index(a, c->stream); });
     timed(c, "entropy", [&] { launch_entropy(a, c->stream); });
     timed(c, "recon", [&] { launch_recon(a, c->stream); });

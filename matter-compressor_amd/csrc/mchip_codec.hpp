@@ -201,16 +201,19 @@ __device__ __forceinline__ void decode_header(RcDec &d, const QTab *__restrict__
 // Entropy-decode one payload into a (raster index, level) list at w (cap
 // words), corrections appended as (pos, delta) word pairs.  Returns the
 // block record: x = dc | flags << 8 | status << 16, y = nsig | ncorr << 16.
+template <bool kSM>
 __device__ __forceinline__ uint2 entropy_to_list(const uint8_t *payload, uint32_t len, const QTab *__restrict__ qt,
                                                  const uint16_t *band_start, uint32_t *w, uint32_t cap,
-                                                 uint32_t *stage, uint32_t stage_stride, uint32_t *ctx) {
+                                                 uint32_t *stage, uint32_t stage_stride, uint32_t *ctx,
+                                                 const uint32_t *win = nullptr, int32_t nwin = 0) {
     RcDec d;
-    d.init(payload, 0, len);
+    d.init(payload, 0, len, win, nwin, (int32_t)stage_stride);
     int dc, flags;
     decode_header(d, qt, dc, flags);
     ListSink sink;
     sink.init(w, cap, stage, stage_stride);
-    const uint32_t nsig = decode_coefs_sm(d, qt, band_start, sink, ctx, stage_stride);
+    const uint32_t nsig = kSM ? decode_coefs_sm(d, qt, band_start, sink, ctx, stage_stride)
+                              : decode_coefs(d, qt, band_start, sink);
     uint32_t nc = 0;
     if (flags & 2) {
         uint32_t n = d.expgolomb() + 1, pos = 0;

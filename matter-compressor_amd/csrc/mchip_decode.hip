@@ -120,6 +120,28 @@ __global__ __launch_bounds__(256) void k_index(const uint8_t *__restrict__ blobs
 // ---------------------------------------------------------------------------
 // k_entropy: one lane per present block of one chunk (blockIdx.y = chunk).
 // ---------------------------------------------------------------------------
+// Stage the first min(words, kWinWords) aligned payload words of this lane
+// into its LDS window win[j * 64] (bulk loads, one wait); the decoder then
+// reads global memory only past the window.  In SIMT a per-lane prefetch
+// cannot hide latency (every refill of any lane would wait on the most
+// recent load of the wave), so the payload is fetched before decoding.
+constexpr int32_t kWinWords = 128;
+__device__ __forceinline__ int32_t stage_payload(const uint8_t *pay, uint32_t len, uint32_t *win) {
+    const uint32_t *aw = (const uint32_t *)((uintptr_t)pay & ~(uintptr_t)3);
+    int32_t nw = payload_words(pay, len);
+    nw = nw < kWinWords ? nw : kWinWords;
+    for (int32_t j = 0; j < nw; j += 16) {
+        uint32_t v[16];
+        #pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = j + u < nw ? aw[j + u] : 0u;
+        #pragma unroll
+        for (int u = 0; u < 16; ++u)
+            if (j + u < nw) win[(j + u) * 64] = v[u];
+    }
+    return nw;
+}
+
+template <bool kSM>
 __global__ __launch_bounds__(64) void k_entropy(const uint8_t *__restrict__ blobs,
                                                 const ChunkDesc *__restrict__ cd,
                                                 const QTab *__restrict__ qtabs,
@@ -129,7 +151,8 @@ __global__ __launch_bounds__(64) void k_entropy(const uint8_t *__restrict__ blob
                                                 uint32_t *__restrict__ scratch,
                                                 uint2 *__restrict__ binfo) {
     __shared__ uint32_t stage[kStage * 64];
-    __shared__ uint32_t sctx[16 * 64];
+    __shared__ uint32_t sctx[kSM ? 16 * 64 : 1];
+    __shared__ uint32_t pwin[kWinWords * 64];
     __shared__ QTab sq;
     __shared__ uint16_t sband[9];
     const int c = blockIdx.y;
@@ -151,9 +174,12 @@ __global__ __launch_bounds__(64) void k_entropy(const uint8_t *__restrict__ blob
         binfo[cb + bi] = make_uint2(kStatusFallback << 16, 0);
         return;
     }
+    const uint8_t *pay = blobs + cd[c].blob_off + r.off;
+    uint32_t *win = pwin + threadIdx.x;
+    const int32_t nw = stage_payload(pay, r.len, win);
     uint64_t wofs = cd[c].scratch_off + 8ull * (r.off - cinfo[c].pay0) + 32ull * r.slot;
-    binfo[cb + bi] = entropy_to_list(blobs + cd[c].blob_off + r.off, r.len, &sq, sband, scratch + wofs,
-                                     8u * r.len + 32u, stage + threadIdx.x, 64, sctx + threadIdx.x);
+    binfo[cb + bi] = entropy_to_list<kSM>(pay, r.len, &sq, sband, scratch + wofs, 8u * r.len + 32u,
+                                          stage + threadIdx.x, 64, kSM ? sctx + threadIdx.x : nullptr, win, nw);
 }
 
 // ---------------------------------------------------------------------------
@@ -364,8 +390,12 @@ void launch_index(const DecodeArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_index, dim3(a.n), dim3(256), 0, s, a.blobs, a.cd, a.rec, a.plist, a.cinfo);
 }
 void launch_entropy(const DecodeArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_entropy, dim3(kNVox / 64, a.n), dim3(64), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
-                       a.plist, a.cinfo, a.scratch, a.binfo);
+    if (a.state_machine)
+        hipLaunchKernelGGL(k_entropy<true>, dim3(kNVox / 64, a.n), dim3(64), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
+                           a.plist, a.cinfo, a.scratch, a.binfo);
+    else
+        hipLaunchKernelGGL(k_entropy<false>, dim3(kNVox / 64, a.n), dim3(64), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
+                           a.plist, a.cinfo, a.scratch, a.binfo);
 }
 void launch_recon(const DecodeArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_recon, dim3((unsigned)a.n * kNVox), dim3(256), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
@@ -385,9 +415,12 @@ __global__ __launch_bounds__(64) void k_entropy_jobs(const uint8_t *__restrict__
     const BlockJob j = jobs[i];
     if (j.len == 0) { binfo[i] = make_uint2(kStatusFallback << 16, 0); return; }
     __shared__ uint32_t stage[kStage * 64];
-    __shared__ uint32_t sctx[16 * 64];
-    binfo[i] = entropy_to_list(base + j.pay_off, j.len, qtabs + j.qtab, c_band_start, scratch + j.scratch_off,
-                               8u * j.len + 32u, stage + threadIdx.x, 64, sctx + threadIdx.x);
+    __shared__ uint32_t pwin[kWinWords * 64];
+    const uint8_t *pay = base + j.pay_off;
+    uint32_t *win = pwin + threadIdx.x;
+    const int32_t nw = stage_payload(pay, j.len, win);
+    binfo[i] = entropy_to_list<false>(pay, j.len, qtabs + j.qtab, c_band_start, scratch + j.scratch_off,
+                                      8u * j.len + 32u, stage + threadIdx.x, 64, nullptr, win, nw);
 }
 
 __global__ __launch_bounds__(256) void k_recon_jobs(const uint8_t *__restrict__ base,

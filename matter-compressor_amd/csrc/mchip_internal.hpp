@@ -41,6 +41,7 @@ struct DecodeArgs {
     uint2 *binfo;           // [n*4096]
     uint8_t *out;
     int n;
+    int state_machine;      // entropy decoder variant (tuning)
 };
 
 // ---- encode ----------------------------------------------------------------

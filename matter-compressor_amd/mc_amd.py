@@ -39,6 +39,7 @@ def load():
         "mchip_init": (i, [i]),
         "mchip_last_error": (C.c_char_p, []),
         "mchip_set_priors": (i, [vp, vp]),
+        "mchip_set_tuning": (i, [C.c_char_p, i]),
         "mchip_ctx_create": (vp, []),
         "mchip_ctx_destroy": (None, [vp]),
         "mchip_ctx_stream": (vp, [vp]),
@@ -63,6 +64,10 @@ def load():
         f.argtypes = args
     _lib = L
     return L
+
+
+def set_tuning(key, value):
+    check(load().mchip_set_tuning(key.encode(), int(value)))
 
 
 def check(rc):

@@ -38,6 +38,8 @@ def harness():
     H.dl_encode.restype = C.c_uint32
     H.dl_idct_line.argtypes = [vp, vp]
     H.dl_fdct_line.argtypes = [vp, vp]
+    H.dl_set_sm.argtypes = [C.c_int]
+    H.dl_set_window.argtypes = [C.c_int]
     _h = H
     return H
 
@@ -94,22 +96,34 @@ def oracle_levels(payload, q):
 def device_decode(payload, q, scan, band_start):
     H = harness()
     t = qtab(q)
-    buf = np.zeros(len(payload) + 16, np.uint8)
-    buf[: len(payload)] = np.frombuffer(payload, np.uint8)
+    lead = len(payload) % 4                      # exercise unaligned payload starts
+    buf = np.zeros(len(payload) + 32, np.uint8)
+    buf[lead:lead + len(payload)] = np.frombuffer(payload, np.uint8)
+    pp = C.c_void_p(buf.ctypes.data + lead)
     lv = np.zeros(4096, np.int16)
     dc, fl, nc = C.c_int(), C.c_int(), C.c_uint32()
     cp = np.zeros(4096, np.uint32)
     cd = np.zeros(4096, np.int32)
-    st = H.dl_decode(ob.p(t), ob.p(band_start), ob.p(scan), ob.p(buf), len(payload), ob.p(lv), C.byref(dc),
+    st = H.dl_decode(ob.p(t), ob.p(band_start), ob.p(scan), pp, len(payload), ob.p(lv), C.byref(dc),
                      C.byref(fl), C.byref(nc), ob.p(cp), ob.p(cd))
     if st != 0:   # list overflow: full decode (corrections are then checked by the GPU tests)
-        H.dl_decode_levels_full(ob.p(t), ob.p(band_start), ob.p(scan), ob.p(buf), len(payload), ob.p(lv),
+        H.dl_decode_levels_full(ob.p(t), ob.p(band_start), ob.p(scan), pp, len(payload), ob.p(lv),
                                 C.byref(dc), C.byref(fl))
         return lv, dc.value, fl.value, None, None
     return lv, dc.value, fl.value, cp[: nc.value].copy(), cd[: nc.value].copy()
 
 
-def test_device_decoder_matches_oracle_on_reference_kat(kat):
+@pytest.fixture(params=[(1, 0), (0, 0), (1, 7), (0, 3)],
+                ids=["state_machine", "per_position", "state_machine_window", "per_position_window"])
+def decoder_variant(request):
+    harness().dl_set_sm(request.param[0])
+    harness().dl_set_window(request.param[1])
+    yield request.param
+    harness().dl_set_sm(1)
+    harness().dl_set_window(0)
+
+
+def test_device_decoder_matches_oracle_on_reference_kat(kat, decoder_variant):
     scan, bs = tables()
     for i in range(len(kat["q"])):
         pay = kat["payload"][kat["offs"][i]:kat["offs"][i + 1]].tobytes()
@@ -121,7 +135,7 @@ def test_device_decoder_matches_oracle_on_reference_kat(kat):
         assert np.array_equal(got[3], want[3]) and np.array_equal(got[4], want[4]), f"case {i} corrections"
 
 
-def test_device_decoder_matches_oracle_on_corrupt_payloads(kat):
+def test_device_decoder_matches_oracle_on_corrupt_payloads(kat, decoder_variant):
     scan, bs = tables()
     rng = np.random.default_rng(21)
     for trial in range(400):
