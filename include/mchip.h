@@ -43,9 +43,10 @@ const char *mchip_last_error(void);
  * (src/mc_codec.c:141-154, :533). */
 int         mchip_set_priors(const uint16_t *plo, const uint16_t *phi);
 
-/* Launch-shape tuning (process-wide): "decoder_sm" selects the entropy
- * decoder variant (0 = one loop step per scan position, 1 = one bin per
- * loop step).  Results are identical; only speed differs. */
+/* Launch-shape tuning (process-wide): "decoder" selects the entropy decoder
+ * variant (0 = one loop step per scan position, 1 = one bin per loop step
+ * with branches, 2 = one bin per loop step, branch-free, payload in an LDS
+ * window; the default).  Results are identical; only speed differs. */
 int         mchip_set_tuning(const char *key, int value);
 
 /* A context owns one HIP stream and a growable device workspace. */

@@ -68,7 +68,8 @@ constexpr int kMaxQTabs = 4096;
 
 // Launch-shape tuning knobs (mchip_set_tuning); process-wide.
 struct Tuning {
-    int decoder_sm = 0;       // entropy decoder: 0 = per scan position, 1 = one bin per iteration
+    int decoder = 2;          // entropy decoder: 0 per scan position, 1 branching state machine,
+                              // 2 branch-free lane decoder over an LDS payload window
 };
 Tuning g_tune;
 
@@ -329,7 +330,11 @@ int mchip_set_priors(const uint16_t *plo, const uint16_t *phi) {
 
 int mchip_set_tuning(const char *key, int value) {
     if (!key) return fail("null key");
-    if (!strcmp(key, "decoder_sm")) { g_tune.decoder_sm = value != 0; return 0; }
+    if (!strcmp(key, "decoder")) {
+        if (value < 0 || value > 2) return fail("decoder must be 0, 1 or 2");
+        g_tune.decoder = value;
+        return 0;
+    }
     return fail("unknown tuning key '%s'", key);
 }
 
@@ -442,7 +447,7 @@ int mchip_decode_chunks(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *bl
     a.binfo = (uint2 *)c->d_binfo;
     a.out = d_out;
     a.n = n;
-    a.state_machine = g_tune.decoder_sm;
+    a.decoder = g_tune.decoder;
     timed(c, "index", [&] { launch_index(a, c->stream); });
     timed(c, "entropy", [&] { launch_entropy(a, c->stream); });
     timed(c, "recon", [&] { launch_recon(a, c->stream); });
@@ -714,7 +719,7 @@ int mchip_decode_block_jobs(mchip_ctx *c, const uint8_t *d_pay, const uint64_t *
     c->desc_pending = true;
     timed(c, "block_jobs", [&] {
         launch_jobs(d_pay, (const BlockJob *)c->d_jobs, n, G().d_qtabs, (uint32_t *)c->d_jscratch,
-                    (uint2 *)c->d_jinfo, d_arena, c->stream);
+                    (uint2 *)c->d_jinfo, d_arena, g_tune.decoder, c->stream);
     });
     HIPCHK(hipGetLastError());
     return 0;

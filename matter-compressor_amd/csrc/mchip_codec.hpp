@@ -154,6 +154,7 @@ __device__ __forceinline__ uint32_t decode_coefs_sm(RcDec &d, const QTab *__rest
     return nsig;
 }
 
+
 // Word list of one block (coefficients as scan position | level << 16, then
 // corrections as pos, delta word pairs).  Words go through a small staging
 // area (LDS on the GPU, lane-interleaved) and reach global memory in 16-byte
@@ -196,6 +197,282 @@ __device__ __forceinline__ void decode_header(RcDec &d, const QTab *__restrict__
     #pragma unroll
     for (int b = 0; b < 8; ++b) v = (v << 1) | d.bit_once(qt->dc[b]);
     dc = v; flags = f;
+}
+
+// ---------------------------------------------------------------------------
+// Lane decoder: one bin per loop iteration for every lane, with the bin's
+// arithmetic, context adaptation, renormalisation and state transition
+// written as selects, so a wave pays one bin body per iteration instead of
+// the union of its lanes' branch bodies.  Only rare events branch (band
+// change, last position, Exp-Golomb suffix, window refill, list flush).
+// Same bins, same order, same results as decode_coefs.
+//
+// The payload is read from a per-lane window of kWin words (LDS on the GPU,
+// word j at win[(kWinAlloc - 1 - j) * kLaneStride], descending so that one
+// paired read yields words j + 1 and j as the low and high halves) holding the payload's aligned words
+// byte-swapped to MSB-first, zero past the payload end (the reference reads
+// zeros there, src/mc_codec.c:197).  rd_bits is the bit position of the next
+// unread byte inside the window.
+// ---------------------------------------------------------------------------
+constexpr int32_t kWin = 64;
+constexpr int32_t kWinAlloc = kWin + 1;   // words allocated per lane (see LaneCoder::norm_fast)
+
+#ifndef MCHIP_LANE_STRIDE
+#define MCHIP_LANE_STRIDE 64
+#endif
+constexpr uint32_t kLaneStride = MCHIP_LANE_STRIDE;   // lanes sharing one LDS array
+
+// Fill win[0, kWin) with the payload's aligned words first .. first + kWin.
+// Out of line: it runs once per kWin words.
+__device__ __noinline__ void window_fill(const uint8_t *p, uint32_t len, int32_t first, uint32_t *win) {
+    const uint32_t lead = (uint32_t)((uintptr_t)p & 3);
+    const uint32_t *aw = (const uint32_t *)((uintptr_t)p - lead);
+    const int32_t nwords = (int32_t)((lead + len + 3) >> 2);   // aligned words holding payload bytes
+    #pragma unroll 1
+    for (int32_t j0 = 0; j0 < kWin; j0 += 16) {
+        if (first + j0 >= nwords) {                            // rest is past the payload
+            #pragma unroll 1
+            for (int32_t j = j0; j < kWin; ++j) win[(kWinAlloc - 1 - j) * kLaneStride] = 0u;
+            break;
+        }
+        uint32_t a[16];
+        #pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int32_t i = first + j0 + u;
+            a[u] = i < nwords ? aw[i] : 0u;
+        }
+        #pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            // aligned word i holds payload bytes 4i - lead .. 4i - lead + 3 (little-endian)
+            const int64_t room = (int64_t)len + lead - 4ll * (first + j0 + u);
+            const uint32_t m = room >= 4 ? 0xFFFFFFFFu : (room <= 0 ? 0u : 0xFFFFFFFFu >> (8 * (4 - (int32_t)room)));
+            win[(kWinAlloc - 1 - (j0 + u)) * kLaneStride] = __builtin_bswap32(a[u] & m);
+        }
+    }
+}
+
+struct LaneCoder {
+    const uint8_t *pay;
+    uint32_t len;
+    uint32_t *win;
+    int32_t first;           // aligned payload word held in win[0]
+    uint32_t rd_bits;        // bit position of the next unread byte in the window
+    uint32_t w_hi, w_lo;     // window words rd_bits / 32 and the one after it
+    uint32_t code, range;
+
+    // Issue the window reads for the current position; they are consumed by
+    // the next renormalisation, so their latency overlaps a whole bin.
+    __device__ __forceinline__ void fetch() {
+        const uint32_t *q = win + (kWinAlloc - 2 - (rd_bits >> 5)) * kLaneStride;
+        w_lo = q[0];                      // word j + 1
+        w_hi = q[kLaneStride];            // word j
+    }
+    __device__ __forceinline__ uint32_t next32() const {    // 32 bits (MSB first) at rd_bits
+        return (uint32_t)(((((uint64_t)w_hi) << 32) | w_lo) << (rd_bits & 24) >> 32);
+    }
+    __device__ __forceinline__ bool need_slide() const { return rd_bits >= 32u * (kWin - 1); }
+    __device__ __forceinline__ void slide() {
+        first += (int32_t)(rd_bits >> 5);
+        rd_bits &= 31u;
+        window_fill(pay, len, first, win);
+        fetch();
+    }
+    __device__ __forceinline__ void init(const uint8_t *p, uint32_t n, uint32_t *w) {
+        pay = p; len = n; win = w; first = 0;
+        window_fill(p, n, 0, win);
+        // code = payload bytes 1..4 (byte 0 is the encoder's leading cache byte)
+        rd_bits = 8u * (uint32_t)((uintptr_t)p & 3) + 8u;
+        fetch();
+        code = next32();
+        rd_bits += 32;
+        fetch();
+        range = 0xFFFFFFFFu;
+    }
+    // Renormalise: shift in 0..3 bytes until range >= 2^24.  The _fast form
+    // leaves a window slide to the caller: with rd_bits below the slide
+    // threshold on entry it reads at most word kWin (kWinAlloc words are
+    // allocated), and the words are valid again after slide().
+    __device__ __forceinline__ void norm_fast() {
+        const uint32_t sh = (uint32_t)__builtin_clz(range) & 24u;   // range >= 1
+        code = (uint32_t)(((((uint64_t)code) << 32) | next32()) << sh >> 32);
+        range <<= sh;
+        rd_bits += sh;
+        fetch();
+    }
+    __device__ __forceinline__ void norm() {
+        norm_fast();
+        if (need_slide()) slide();
+    }
+    __device__ __forceinline__ uint32_t bit(uint32_t p0) {      // fixed-probability bin
+        const uint32_t r0 = __umul24(range >> 12, p0);
+        const uint32_t b = code >= r0;
+        code = b ? code - r0 : code;
+        range = b ? range - r0 : r0;
+        norm();
+        return b;
+    }
+    __device__ __forceinline__ uint32_t bypass() {
+        range >>= 1;
+        const uint32_t b = code >= range;
+        code = b ? code - range : code;
+        norm();
+        return b;
+    }
+    __device__ __forceinline__ uint32_t bypass_k16(int kk) {
+        range >>= kk;
+        uint32_t q = code / range, m = (1u << kk) - 1u;
+        if (q > m) q = m;
+        code -= q * range;
+        norm();
+        return q;
+    }
+    __device__ __forceinline__ uint32_t bypass_n(int kk) {
+        uint32_t v = 0;
+        while (kk > 16) { v = (v << 16) | bypass_k16(16); kk -= 16; }
+        if (kk == 0) return v;
+        return (v << kk) | bypass_k16(kk);
+    }
+    __device__ __forceinline__ uint32_t expgolomb() {
+        uint32_t n1 = 0;
+        while (n1 < 64 && bypass()) n1++;
+        if (!n1) return 0;
+        return ((1u << (n1 & 31)) | bypass_n((int)n1)) - 1u;
+    }
+};
+
+// List sink of the lane decoder: every loop iteration stores its candidate
+// entry at w[n] and advances n only when the entry is kept, so the serial
+// loop has no branch and no LDS staging (the decode loop itself issues no
+// global loads, so nothing ever waits on these stores).  Stores clip at
+// the capacity; overflow is reported through count().
+struct LaneSink {
+    uint32_t *w;
+    uint32_t cap, n;
+    __device__ __forceinline__ void init(uint32_t *dst, uint32_t capacity) { w = dst; cap = capacity; n = 0; }
+    __device__ __forceinline__ void emit(uint32_t v, bool keep) {
+        w[n < cap ? n : cap - 1u] = v;
+        n += keep ? 1u : 0u;
+    }
+    __device__ __forceinline__ uint32_t count() const { return n; }
+};
+
+// Coefficients of one block; ctx[s * kLaneStride] holds slot s: 0-3 the
+// significance contexts of the current band, 4-15 the magnitude rungs, 16 a
+// dummy written by bypass bins.
+__device__ __forceinline__ uint32_t decode_coefs_lane(LaneCoder &d, const QTab *__restrict__ qt,
+                                                      const uint16_t *band_start, LaneSink &sink, uint32_t *ctx) {
+    uint32_t k0 = 0;
+    for (uint32_t i = 0; i < 13; ++i) {
+        if (!d.bit(qt->eob[i])) break;
+        k0 = i + 1;
+    }
+    uint32_t eob = k0 == 0 ? 0u : (k0 == 1 ? 1u : ((1u << (k0 - 1)) | d.bypass_n((int)k0 - 1)));
+    if (eob > (uint32_t)kNVox) eob = kNVox;
+    if (eob == 0) return 0;
+    for (int i = 0; i < 12; ++i) ctx[(4 + i) * kLaneStride] = qt->mag[i];
+    for (int i = 0; i < 4; ++i) ctx[i * kLaneStride] = qt->sig[i];
+    uint32_t band = 0, next_band = band_start[1];
+    // phases: 0 significance, 1 magnitude rung (slot ms = 4 + rung), 2 Exp-Golomb prefix
+    // (ms = 16 + ones so far), 3 sign
+    uint32_t ph = 0, p = 0, hist = 0, ms = 4, mag = 0;
+    uint32_t evp = next_band < eob - 1 ? next_band : eob - 1;   // next position needing the slow path
+    if (eob == 1) { hist = 1; ph = 1; }                          // position 0 is the last: implied
+    uint32_t slot = ph == 0 ? 0u : 4u;                           // context slot of the coming bin
+    // The coming bin's context is read one bin ahead (pnext); if the bin in
+    // between adapted the same slot, its updated value (p0n) is taken.
+    uint32_t pnext = ctx[slot * kLaneStride], p0n = 0;
+    bool same = false;
+    while (p < eob) {
+        const uint32_t p0 = same ? p0n : pnext;
+        const bool byp = ph >= 2;
+        const uint32_t rb = d.range >> 1, rm = __umul24(d.range >> 12, p0);
+        const uint32_t r0 = byp ? rb : rm;
+        const bool b = d.code >= r0;
+        d.code -= b ? r0 : 0u;
+        d.range = (b && !byp) ? d.range - r0 : r0;
+        // state transition
+        const bool isSig = ph == 0, isMag = ph == 1, isEgp = ph == 2, isSgn = ph == 3;
+        const uint32_t entry = p | ((b ? 0u - mag : mag) << 16);
+        hist = isSig ? (hist << 1) | (b ? 1u : 0u) : hist;
+        p += ((isSig && !b) || isSgn) ? 1u : 0u;
+        const uint32_t nph = isSig ? (b ? 1u : 0u) : (isSgn ? 0u : (b ? ph + (ms == 15u ? 1u : 0u) : 3u));
+        mag = isMag ? ms - 3u : (isEgp ? 12u : mag);
+        ms = isSig ? 4u : ms + (((isMag || isEgp) && b) ? 1u : 0u);
+        ph = nph;
+        // context of the next bin, read now so its latency overlaps the rest
+        uint32_t dens = (uint32_t)__popc(hist & 0xFFFFu);
+        dens = dens < 3u ? dens : 3u;
+        const uint32_t nslot = nph == 0 ? dens : (nph == 1 ? ms : 16u);
+        pnext = ctx[nslot * kLaneStride];
+        // adapt this bin's context: p0 - p0/16 after a 1, p0 + (4096 - p0)/16
+        // after a 0 (both floored), p0 + floor((4096 - p0)/16) = p0 - floor((p0 - 4081)/16)
+        p0n = p0 - (uint32_t)((int32_t)(p0 - (b ? 0u : 4081u)) >> 4);
+        ctx[slot * kLaneStride] = p0n;
+        same = nslot == slot;
+        slot = nslot;
+        d.norm_fast();
+        sink.emit(entry, isSgn);
+        // One test for every rare event: window slide, an Exp-Golomb prefix
+        // bin (the suffix may follow), a band change or the last (implied)
+        // position.
+        const bool evt = nph == 0 && p >= evp;
+        if (d.need_slide() || isEgp || evt) {
+            if (d.need_slide()) d.slide();
+            if (isEgp && (b ? ms == 16u + 64u : ms != 16u)) {
+                const uint32_t nb = ms - 16u;
+                mag = 12u + ((1u << (nb & 31)) | d.bypass_n((int)nb)) - 1u;
+                ph = 3;
+            }
+            if (evt) {
+                if (p >= next_band && band < 7) {
+                    band++;
+                    next_band = band_start[band + 1];
+                    for (int i = 0; i < 4; ++i) ctx[i * kLaneStride] = qt->sig[band * 4 + i];
+                }
+                if (p + 1 == eob) { hist = (hist << 1) | 1u; ph = 1; ms = 4; }
+                evp = next_band < eob - 1 ? next_band : eob - 1;
+            }
+            uint32_t dn = (uint32_t)__popc(hist & 0xFFFFu);
+            dn = dn < 3u ? dn : 3u;
+            slot = ph == 0 ? dn : (ph == 1 ? ms : 16u);
+            pnext = ctx[slot * kLaneStride];
+            same = false;
+        }
+    }
+    return sink.count();
+}
+
+// Lane decoder entry: header, coefficients and corrections into the list at
+// w (cap words).  Returns the block record like entropy_to_list.
+__device__ __forceinline__ uint2 entropy_to_list_lane(const uint8_t *payload, uint32_t len, const QTab *__restrict__ qt,
+                                                      const uint16_t *band_start, uint32_t *w, uint32_t cap,
+                                                      uint32_t *ctx, uint32_t *win) {
+    LaneCoder d;
+    d.init(payload, len, win);
+    uint32_t f = 0, dcv = 0;
+    f |= d.bit(qt->flag[0]) ? 1u : 0u;
+    f |= d.bit(qt->flag[1]) ? 2u : 0u;
+    for (int b = 0; b < 8; ++b) dcv = (dcv << 1) | d.bit(qt->dc[b]);
+    LaneSink sink;
+    sink.init(w, cap);
+    const uint32_t nsig = decode_coefs_lane(d, qt, band_start, sink, ctx);
+    uint32_t nc = 0;
+    if (f & 2) {
+        uint32_t n = d.expgolomb() + 1, pos = 0;
+        if (n > (uint32_t)kNVox) n = kNVox;
+        for (uint32_t i = 0; i < n; ++i) {
+            pos += d.expgolomb();
+            const uint32_t neg = d.bypass();
+            const uint32_t m = d.expgolomb() + 1;
+            if (pos >= (uint32_t)kNVox) break;
+            sink.emit(pos, true);
+            sink.emit(neg ? 0u - m : m, true);
+            nc++;
+        }
+    }
+    const uint32_t total = sink.count();
+    const uint32_t status = total > cap ? kStatusFallback : kStatusOk;
+    return make_uint2(dcv | (f << 8) | (status << 16), nsig | (nc << 16));
 }
 
 // Entropy-decode one payload into a (raster index, level) list at w (cap

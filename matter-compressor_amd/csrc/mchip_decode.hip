@@ -120,28 +120,29 @@ __global__ __launch_bounds__(256) void k_index(const uint8_t *__restrict__ blobs
 // ---------------------------------------------------------------------------
 // k_entropy: one lane per present block of one chunk (blockIdx.y = chunk).
 // ---------------------------------------------------------------------------
-// Stage the first min(words, kWinWords) aligned payload words of this lane
-// into its LDS window win[j * 64] (bulk loads, one wait); the decoder then
-// reads global memory only past the window.  In SIMT a per-lane prefetch
-// cannot hide latency (every refill of any lane would wait on the most
-// recent load of the wave), so the payload is fetched before decoding.
-constexpr int32_t kWinWords = 128;
-__device__ __forceinline__ int32_t stage_payload(const uint8_t *pay, uint32_t len, uint32_t *win) {
-    const uint32_t *aw = (const uint32_t *)((uintptr_t)pay & ~(uintptr_t)3);
-    int32_t nw = payload_words(pay, len);
-    nw = nw < kWinWords ? nw : kWinWords;
-    for (int32_t j = 0; j < nw; j += 16) {
-        uint32_t v[16];
-        #pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = j + u < nw ? aw[j + u] : 0u;
-        #pragma unroll
-        for (int u = 0; u < 16; ++u)
-            if (j + u < nw) win[(j + u) * 64] = v[u];
-    }
-    return nw;
+// Decoder variants (a tuning knob; all give identical results):
+//   kDecPosition  one loop iteration per scan position (mchip_codec.hpp
+//                 decode_coefs), payload read through a prefetched bit buffer;
+//   kDecState     one bin per iteration with branches per phase;
+//   kDecLane      one bin per iteration, branch-free, payload staged in a
+//                 per-lane LDS window (LaneDec).  In SIMT every branch body
+//                 taken by any lane costs the whole wave, so the branch-free
+//                 loop pays one bin body per bin instead of the union of the
+//                 lanes' ladders.
+constexpr int kDecPosition = 0, kDecState = 1, kDecLane = 2;
+
+template <int kMode>
+__device__ __forceinline__ uint2 entropy_lane(const uint8_t *pay, uint32_t len, const QTab *qt, const uint16_t *band,
+                                              uint32_t *w, uint32_t cap, uint32_t *stage, uint32_t *ctx,
+                                              uint32_t *win) {
+    if constexpr (kMode == kDecLane)
+        return entropy_to_list_lane(pay, len, qt, band, w, cap, ctx, win);
+    else
+        return entropy_to_list<kMode == kDecState>(pay, len, qt, band, w, cap, stage, 64,
+                                                   kMode == kDecState ? ctx : nullptr, nullptr, 0);
 }
 
-template <bool kSM>
+template <int kMode>
 __global__ __launch_bounds__(64) void k_entropy(const uint8_t *__restrict__ blobs,
                                                 const ChunkDesc *__restrict__ cd,
                                                 const QTab *__restrict__ qtabs,
@@ -150,9 +151,9 @@ __global__ __launch_bounds__(64) void k_entropy(const uint8_t *__restrict__ blob
                                                 const ChunkInfo *__restrict__ cinfo,
                                                 uint32_t *__restrict__ scratch,
                                                 uint2 *__restrict__ binfo) {
-    __shared__ uint32_t stage[kStage * 64];
-    __shared__ uint32_t sctx[kSM ? 16 * 64 : 1];
-    __shared__ uint32_t pwin[kWinWords * 64];
+    __shared__ uint32_t stage[kMode == kDecLane ? 1 : kStage * 64];
+    __shared__ uint32_t sctx[kMode == kDecPosition ? 1 : 17 * 64];
+    __shared__ uint32_t pwin[kMode == kDecLane ? kWinAlloc * 64 : 1];
     __shared__ QTab sq;
     __shared__ uint16_t sband[9];
     const int c = blockIdx.y;
@@ -175,11 +176,9 @@ __global__ __launch_bounds__(64) void k_entropy(const uint8_t *__restrict__ blob
         return;
     }
     const uint8_t *pay = blobs + cd[c].blob_off + r.off;
-    uint32_t *win = pwin + threadIdx.x;
-    const int32_t nw = stage_payload(pay, r.len, win);
     uint64_t wofs = cd[c].scratch_off + 8ull * (r.off - cinfo[c].pay0) + 32ull * r.slot;
-    binfo[cb + bi] = entropy_to_list<kSM>(pay, r.len, &sq, sband, scratch + wofs, 8u * r.len + 32u,
-                                          stage + threadIdx.x, 64, kSM ? sctx + threadIdx.x : nullptr, win, nw);
+    binfo[cb + bi] = entropy_lane<kMode>(pay, r.len, &sq, sband, scratch + wofs, 8u * r.len + 32u,
+                                         stage + threadIdx.x, sctx + threadIdx.x, pwin + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -390,12 +389,15 @@ void launch_index(const DecodeArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_index, dim3(a.n), dim3(256), 0, s, a.blobs, a.cd, a.rec, a.plist, a.cinfo);
 }
 void launch_entropy(const DecodeArgs &a, hipStream_t s) {
-    if (a.state_machine)
-        hipLaunchKernelGGL(k_entropy<true>, dim3(kNVox / 64, a.n), dim3(64), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
+    if (a.decoder == kDecState)
+        hipLaunchKernelGGL(k_entropy<kDecState>, dim3(kNVox / 64, a.n), dim3(64), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
+                           a.plist, a.cinfo, a.scratch, a.binfo);
+    else if (a.decoder == kDecLane)
+        hipLaunchKernelGGL(k_entropy<kDecLane>, dim3(kNVox / 64, a.n), dim3(64), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
                            a.plist, a.cinfo, a.scratch, a.binfo);
     else
-        hipLaunchKernelGGL(k_entropy<false>, dim3(kNVox / 64, a.n), dim3(64), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
-                           a.plist, a.cinfo, a.scratch, a.binfo);
+        hipLaunchKernelGGL(k_entropy<kDecPosition>, dim3(kNVox / 64, a.n), dim3(64), 0, s, a.blobs, a.cd, a.qtabs,
+                           a.rec, a.plist, a.cinfo, a.scratch, a.binfo);
 }
 void launch_recon(const DecodeArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_recon, dim3((unsigned)a.n * kNVox), dim3(256), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
@@ -406,21 +408,20 @@ void launch_recon(const DecodeArgs &a, hipStream_t s) {
 // Block jobs: decode an arbitrary list of payloads into 4096-byte arena slots
 // (region reads and the decoded-block arena).
 // ---------------------------------------------------------------------------
+template <int kMode>
 __global__ __launch_bounds__(64) void k_entropy_jobs(const uint8_t *__restrict__ base,
                                                      const BlockJob *__restrict__ jobs, int n,
                                                      const QTab *__restrict__ qtabs,
                                                      uint32_t *__restrict__ scratch, uint2 *__restrict__ binfo) {
+    __shared__ uint32_t stage[kMode == kDecLane ? 1 : kStage * 64];
+    __shared__ uint32_t sctx[kMode == kDecPosition ? 1 : 17 * 64];
+    __shared__ uint32_t pwin[kMode == kDecLane ? kWinAlloc * 64 : 1];
     const int i = blockIdx.x * 64 + threadIdx.x;
     if (i >= n) return;
     const BlockJob j = jobs[i];
     if (j.len == 0) { binfo[i] = make_uint2(kStatusFallback << 16, 0); return; }
-    __shared__ uint32_t stage[kStage * 64];
-    __shared__ uint32_t pwin[kWinWords * 64];
-    const uint8_t *pay = base + j.pay_off;
-    uint32_t *win = pwin + threadIdx.x;
-    const int32_t nw = stage_payload(pay, j.len, win);
-    binfo[i] = entropy_to_list<false>(pay, j.len, qtabs + j.qtab, c_band_start, scratch + j.scratch_off,
-                                      8u * j.len + 32u, stage + threadIdx.x, 64, nullptr, win, nw);
+    binfo[i] = entropy_lane<kMode>(base + j.pay_off, j.len, qtabs + j.qtab, c_band_start, scratch + j.scratch_off,
+                                   8u * j.len + 32u, stage + threadIdx.x, sctx + threadIdx.x, pwin + threadIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_recon_jobs(const uint8_t *__restrict__ base,
@@ -440,8 +441,16 @@ __global__ __launch_bounds__(256) void k_recon_jobs(const uint8_t *__restrict__ 
 }
 
 void launch_jobs(const uint8_t *base, const BlockJob *jobs, int n, const QTab *qtabs, uint32_t *scratch,
-                 uint2 *binfo, uint8_t *arena, hipStream_t s) {
-    hipLaunchKernelGGL(k_entropy_jobs, dim3((n + 63) / 64), dim3(64), 0, s, base, jobs, n, qtabs, scratch, binfo);
+                 uint2 *binfo, uint8_t *arena, int decoder, hipStream_t s) {
+    if (decoder == kDecLane)
+        hipLaunchKernelGGL(k_entropy_jobs<kDecLane>, dim3((n + 63) / 64), dim3(64), 0, s, base, jobs, n, qtabs,
+                           scratch, binfo);
+    else if (decoder == kDecState)
+        hipLaunchKernelGGL(k_entropy_jobs<kDecState>, dim3((n + 63) / 64), dim3(64), 0, s, base, jobs, n, qtabs,
+                           scratch, binfo);
+    else
+        hipLaunchKernelGGL(k_entropy_jobs<kDecPosition>, dim3((n + 63) / 64), dim3(64), 0, s, base, jobs, n, qtabs,
+                           scratch, binfo);
     hipLaunchKernelGGL(k_recon_jobs, dim3(n), dim3(256), 0, s, base, jobs, qtabs, scratch, binfo, arena);
 }
 

@@ -41,7 +41,7 @@ struct DecodeArgs {
     uint2 *binfo;           // [n*4096]
     uint8_t *out;
     int n;
-    int state_machine;      // entropy decoder variant (tuning)
+    int decoder;            // entropy decoder variant (tuning: 0 position, 1 state, 2 lane)
 };
 
 // ---- encode ----------------------------------------------------------------
@@ -102,7 +102,7 @@ struct BlockJob {
     uint32_t pad;
 };
 void launch_jobs(const uint8_t *base, const BlockJob *jobs, int n, const QTab *qtabs, uint32_t *scratch,
-                 uint2 *binfo, uint8_t *arena, hipStream_t s);
+                 uint2 *binfo, uint8_t *arena, int decoder, hipStream_t s);
 void launch_crop(const uint8_t *arena, const int32_t *bmap, const int64_t *origin, int n, int gz, int gy, int gx,
                  int64_t dz, int64_t dy, int64_t dx, uint8_t *out, uint64_t batch_stride, hipStream_t s);
 

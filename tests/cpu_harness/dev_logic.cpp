@@ -1,6 +1,7 @@
 // CPU harness of the device coder logic: the same mchip_codec.hpp the HIP
 // kernels use, compiled for the host through the shim in hip/hip_runtime.h.
 // Test infrastructure only.
+#define MCHIP_LANE_STRIDE 1
 #include "mchip_codec.hpp"
 
 using namespace mchip;
@@ -17,7 +18,7 @@ int dl_decode(const QTab *qt, const uint16_t *band_start, const uint16_t *scan, 
               int16_t *levels, int *dc, int *flags, uint32_t *ncorr, uint32_t *cpos, int32_t *cdel) {
     static thread_local uint32_t w[8 * 17408 + 64 + 3 * 4096 * 2];
     uint32_t cap = 8u * len + 32u;
-    uint32_t stage[kStage], ctx[16];
+    uint32_t stage[kStage], ctx[17];
     // optional window: the first use_win aligned payload words staged separately
     uint32_t win[64];
     int32_t nwin = 0;
@@ -29,7 +30,9 @@ int dl_decode(const QTab *qt, const uint16_t *band_start, const uint16_t *scan, 
         for (int j = 0; j < nwin; ++j) win[j] = aw[j];
     }
     const uint32_t *wp = use_win ? win : nullptr;
-    uint2 r = use_sm ? entropy_to_list<true>(payload, len, qt, band_start, w, cap, stage, 1, ctx, wp, nwin)
+    static thread_local uint32_t lwin[kWinAlloc];
+    uint2 r = use_sm == 2 ? entropy_to_list_lane(payload, len, qt, band_start, w, cap, ctx, lwin)
+            : use_sm ? entropy_to_list<true>(payload, len, qt, band_start, w, cap, stage, 1, ctx, wp, nwin)
                      : entropy_to_list<false>(payload, len, qt, band_start, w, cap, stage, 1, ctx, wp, nwin);
     memset(levels, 0, 4096 * sizeof(int16_t));
     *dc = r.x & 255;

@@ -113,8 +113,8 @@ def device_decode(payload, q, scan, band_start):
     return lv, dc.value, fl.value, cp[: nc.value].copy(), cd[: nc.value].copy()
 
 
-@pytest.fixture(params=[(1, 0), (0, 0), (1, 7), (0, 3)],
-                ids=["state_machine", "per_position", "state_machine_window", "per_position_window"])
+@pytest.fixture(params=[(2, 0), (1, 0), (0, 0), (1, 7), (0, 3)],
+                ids=["branch_free", "state_machine", "per_position", "state_machine_window", "per_position_window"])
 def decoder_variant(request):
     harness().dl_set_sm(request.param[0])
     harness().dl_set_window(request.param[1])
