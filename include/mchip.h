@@ -61,6 +61,8 @@ void        mchip_free(void *dptr);
 int         mchip_h2d(mchip_ctx *ctx, void *dst, const void *src, size_t bytes);
 int         mchip_d2h(mchip_ctx *ctx, void *dst, const void *src, size_t bytes);
 int         mchip_memset(mchip_ctx *ctx, void *dst, int value, size_t bytes);
+/* device-to-device copy on the context's stream (asynchronous) */
+int         mchip_d2d(mchip_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* Decode n chunk blobs (format v7) that are resident in device memory.
  * Blob i lives at d_blobs + blob_off[i] with blob_len[i] readable bytes; q[i]

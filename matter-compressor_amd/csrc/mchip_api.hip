@@ -397,6 +397,10 @@ int mchip_d2h(mchip_ctx *c, void *dst, const void *src, size_t n) {
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }
+int mchip_d2d(mchip_ctx *c, void *dst, const void *src, size_t n) {
+    HIPCHK(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, c->stream));
+    return 0;
+}
 int mchip_memset(mchip_ctx *c, void *dst, int v, size_t n) {
     HIPCHK(hipMemsetAsync(dst, v, n, c->stream));
     return 0;
@@ -409,9 +413,16 @@ int mchip_decode_chunks(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *bl
     if (n <= 0) return 0;
     if (!d_blobs || !blob_off || !blob_len || !q || !d_out) return fail("null argument");
     if (out_stride < (uint64_t)16777216) return fail("out_stride %llu < 256^3", (unsigned long long)out_stride);
-    void *h = nullptr;
-    if (desc_buffer(c, sizeof(ChunkDesc) * (size_t)n, &h)) return -1;
-    ChunkDesc *cd = (ChunkDesc *)h;
+    // descriptors: inside the kernel arguments for small calls, else staged
+    // through pinned memory and copied on the stream
+    const bool inl = n <= kInlineDesc;
+    ChunkDesc local[kInlineDesc];
+    ChunkDesc *cd = local;
+    if (!inl) {
+        void *h = nullptr;
+        if (desc_buffer(c, sizeof(ChunkDesc) * (size_t)n, &h)) return -1;
+        cd = (ChunkDesc *)h;
+    }
     uint64_t sw = 0;
     for (int i = 0; i < n; ++i) {
         uint32_t slot;
@@ -426,19 +437,26 @@ int mchip_decode_chunks(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *bl
         sw += scratch_words_for(blob_len[i]);
     }
     size_t nb = (size_t)n * kNVox;
-    if (ensure(c, &c->d_cd, &c->cap_cd, sizeof(ChunkDesc) * n) ||
+    if ((!inl && ensure(c, &c->d_cd, &c->cap_cd, sizeof(ChunkDesc) * n)) ||
         ensure(c, &c->d_rec, &c->cap_rec, sizeof(BlkRec) * nb) ||
         ensure(c, &c->d_plist, &c->cap_plist, sizeof(uint16_t) * nb) ||
         ensure(c, &c->d_cinfo, &c->cap_cinfo, sizeof(ChunkInfo) * n) ||
         ensure(c, &c->d_binfo, &c->cap_binfo, sizeof(uint2) * nb) ||
         ensure(c, &c->d_scratch, &c->cap_scratch, sizeof(uint32_t) * sw))
         return -1;
-    HIPCHK(hipMemcpyAsync(c->d_cd, cd, sizeof(ChunkDesc) * n, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipEventRecord(c->ev_desc, c->stream));
-    c->desc_pending = true;
+    DescRef dref{};
+    if (inl) {
+        dref.ptr = nullptr;
+        for (int i = 0; i < n; ++i) dref.inl[i] = cd[i];
+    } else {
+        HIPCHK(hipMemcpyAsync(c->d_cd, cd, sizeof(ChunkDesc) * n, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipEventRecord(c->ev_desc, c->stream));
+        c->desc_pending = true;
+        dref.ptr = (const ChunkDesc *)c->d_cd;
+    }
     DecodeArgs a;
     a.blobs = d_blobs;
-    a.cd = (const ChunkDesc *)c->d_cd;
+    a.cd = dref;
     a.qtabs = G().d_qtabs;
     a.rec = (BlkRec *)c->d_rec;
     a.plist = (uint16_t *)c->d_plist;

@@ -54,7 +54,7 @@ __device__ __forceinline__ uint32_t wg_exclusive_scan(uint32_t v, uint32_t *lds4
 }
 
 __global__ __launch_bounds__(256) void k_index(const uint8_t *__restrict__ blobs,
-                                               const ChunkDesc *__restrict__ cd,
+                                               const DescRef cd,
                                                BlkRec *__restrict__ rec,
                                                uint16_t *__restrict__ plist,
                                                ChunkInfo *__restrict__ cinfo) {
@@ -144,7 +144,7 @@ __device__ __forceinline__ uint2 entropy_lane(const uint8_t *pay, uint32_t len, 
 
 template <int kMode>
 __global__ __launch_bounds__(64) void k_entropy(const uint8_t *__restrict__ blobs,
-                                                const ChunkDesc *__restrict__ cd,
+                                                const DescRef cd,
                                                 const QTab *__restrict__ qtabs,
                                                 const BlkRec *__restrict__ rec,
                                                 const uint16_t *__restrict__ plist,
@@ -338,7 +338,7 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t i, uint32_t n) {
 }
 
 __global__ __launch_bounds__(256) void k_recon(const uint8_t *__restrict__ blobs,
-                                               const ChunkDesc *__restrict__ cd,
+                                               const DescRef cd,
                                                const QTab *__restrict__ qtabs,
                                                const BlkRec *__restrict__ rec,
                                                const ChunkInfo *__restrict__ cinfo,

@@ -17,6 +17,17 @@ struct ChunkDesc {
     uint32_t pad;
 };
 
+// Chunk descriptors as the decode kernels see them: a device array, or (for
+// launches of at most kInlineDesc chunks, the one-chunk-per-call path) the
+// descriptors themselves inside the kernel arguments, so that such a call
+// needs no host-to-device copy before its kernels.
+constexpr int kInlineDesc = 4;
+struct DescRef {
+    const ChunkDesc *ptr;   // null: use inl
+    ChunkDesc inl[kInlineDesc];
+    __device__ __forceinline__ ChunkDesc operator[](int i) const { return ptr ? ptr[i] : inl[i]; }
+};
+
 // Per block of a chunk, written by k_index.  off == 0: absent (decodes to 0).
 struct BlkRec {
     uint32_t off;           // payload offset inside the blob
@@ -32,7 +43,7 @@ struct ChunkInfo {
 
 struct DecodeArgs {
     const uint8_t *blobs;
-    const ChunkDesc *cd;
+    DescRef cd;
     const QTab *qtabs;
     BlkRec *rec;            // [n*4096]
     uint16_t *plist;        // [n*4096]

@@ -49,6 +49,7 @@ def load():
         "mchip_h2d": (i, [vp, vp, vp, C.c_size_t]),
         "mchip_d2h": (i, [vp, vp, vp, C.c_size_t]),
         "mchip_memset": (i, [vp, vp, i, C.c_size_t]),
+        "mchip_d2d": (i, [vp, vp, vp, C.c_size_t]),
         "mchip_decode_chunks": (i, [vp, u8p, vp, vp, vp, i, u8p, C.c_uint64]),
         "mchip_decode_chunks_host": (i, [vp, vp, vp, i, u8p, C.c_uint64]),
         "mchip_decode_blocks_host": (i, [vp, vp, vp, vp, i, u8p]),
@@ -137,6 +138,9 @@ class Ctx:
 
     def d2h(self, arr, dbuf, offset=0):
         check(load().mchip_d2h(self.h, _ptr(arr), C.c_void_p(dbuf.ptr + offset), arr.nbytes))
+
+    def d2d(self, dst, src, nbytes, dst_off=0, src_off=0):
+        check(load().mchip_d2d(self.h, C.c_void_p(dst.ptr + dst_off), C.c_void_p(src.ptr + src_off), nbytes))
 
     def enable_timing(self, on=True):
         check(load().mchip_ctx_enable_timing(self.h, 1 if on else 0))
