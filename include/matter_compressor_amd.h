@@ -59,6 +59,8 @@ void  mc_buf_put(mc_buf *b, const void *s, size_t n);               /* :100 */
 int   mc_enc_block(mc_codec_ctx *ctx, const mc_u8 *vox, mc_buf *out, uint32_t *len_out);
 /* decode one block payload into dst (16^3).  Runs on the GPU. (:114) */
 void  mc_dec_block(mc_codec_ctx *ctx, const mc_u8 *payload, uint32_t plen, mc_u8 *dst);
+/* decode by archive block codec id (CABAC only; src/mc_codec.h:20) */
+void  mc_dec_block_codec(mc_codec_ctx *ctx, uint32_t codec, const mc_u8 *payload, uint32_t plen, mc_u8 *dst);
 /* per-chunk material-fraction map (:117-118) */
 uint32_t mc_enc_fracmap(const mc_u8 *frac, mc_u8 *out, size_t cap);
 void     mc_dec_fracmap(const mc_u8 *in, uint32_t len, mc_u8 *frac);
@@ -113,6 +115,9 @@ uint64_t mc_chunk_compute_hash(const uint8_t *blob, uint64_t blob_len);         
 long mc_verify_archive(const uint8_t *arc, size_t len, int verbose);                    /* :370 */
 int mc_archive_set_priors(struct mc_archive *a, const uint16_t plo[8][32],
                           const uint16_t phi[8][32]);                                   /* :377 */
+/* install an archive's prior override as the process priors (used by the
+ * reference's reader, src/mc_stream.c; declared in src/mc_archive.h:27) */
+void priors_load(const uint8_t *base);
 
 #ifdef __cplusplus
 }

@@ -28,6 +28,13 @@
 #include "mc_host_internal.h"
 #include "mchip.h"
 
+/* Decode dispatch by the archive's block codec id; CABAC is the only codec
+ * (src/mc_codec.c:788-792). */
+void mc_dec_block_codec(mc_codec_ctx *c, uint32_t codec, const mc_u8 *p, uint32_t plen, mc_u8 *dst) {
+    (void)codec;
+    mc_dec_block(c, p, plen, dst);
+}
+
 /* ------------------------------------------------------------------------
  * Device context per host thread (reentrancy: concurrent callers each get
  * their own HIP stream and workspace).
@@ -395,7 +402,10 @@ static int mark_zero(mc_archive *a, int lod, int cz, int cy, int cx) {
     return 0;
 }
 
-static void load_priors(const uint8_t *base) {
+/* The archive's prior override (header @128 -> "PRIS" blob) becomes the
+ * process-wide priors; absent or invalid restores the built-in tables
+ * (src/mc_archive.c:666-672). */
+void priors_load(const uint8_t *base) {
     uint64_t off; memcpy(&off, base + MCH_PRIOROFF, 8);
     uint32_t magic = 0;
     if (off) memcpy(&magic, base + off, 4);
@@ -425,7 +435,7 @@ mc_archive *mc_archive_open_dims(const char *path, int nx, int ny, int nz, float
     if (base == MAP_FAILED) { perror("mmap"); close(fd); return NULL; }
     mc_archive *a = calloc(1, sizeof *a);
     a->fd = fd; a->base = base; a->reserve = reserve;
-    a->nx = nx; a->ny = ny; a->nz = nz; a->quality = quality;
+    a->nx = nx; a->ny = ny; a->nz = nz; a->dim = nx; a->quality = quality; a->cov = NULL;
     pthread_mutex_init(&a->grow_mu, NULL);
     pthread_mutex_init(&a->write_mu, NULL);
     pthread_mutex_init(&a->aux_mu, NULL);
@@ -456,7 +466,7 @@ mc_archive *mc_archive_open_dims(const char *path, int nx, int ny, int nz, float
         if (tot < MCH_META_END) tot = MCH_META_END;
         atomic_store(&a->cursor, tot);
         a->block_codec = MC_BLOCKCODEC_CABAC;
-        load_priors(base);
+        priors_load(base);
     }
     return a;
 }

@@ -32,19 +32,25 @@ typedef struct {
 
 struct mch_region;          /* device-resident reader state (mc_host_region.c) */
 
+/* The leading fields keep the order and types of the reference's private
+ * struct (src/mc_archive.h:12-25), so that modules compiled against that
+ * header (mc_cache.c and mc_volume.c read a->gen) see the same offsets when
+ * linked with this library; the fields after gen are this library's own. */
 struct mc_archive {
     int fd;
     uint8_t *base;                 /* fixed mmap base */
     _Atomic uint64_t cursor;       /* append EOF */
     _Atomic uint64_t file_len;     /* allocated file size */
-    uint64_t reserve;              /* mmap reservation */
-    int nx, ny, nz;
+    int dim;                       /* nx (legacy single-dimension field) */
     float quality;
     uint32_t block_codec;
+    uint64_t reserve;              /* mmap reservation */
     pthread_mutex_t grow_mu;       /* file growth */
     pthread_mutex_t write_mu;      /* index mutation + install */
-    pthread_mutex_t aux_mu;        /* fraction-map cache, region reader */
+    _Atomic uint64_t *cov;         /* unused here (always NULL) */
     _Atomic uint64_t gen;          /* bumped on every publish */
+    int nx, ny, nz;
+    pthread_mutex_t aux_mu;        /* fraction-map cache, region reader */
     mch_frac_entry *fr_cache;
     uint32_t fr_cap;
     struct mch_region *region;
