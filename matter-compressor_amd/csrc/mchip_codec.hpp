@@ -214,7 +214,10 @@ __device__ __forceinline__ void decode_header(RcDec &d, const QTab *__restrict__
 // zeros there, src/mc_codec.c:197).  rd_bits is the bit position of the next
 // unread byte inside the window.
 // ---------------------------------------------------------------------------
-constexpr int32_t kWin = 64;
+#ifndef MCHIP_LANE_WIN
+#define MCHIP_LANE_WIN 64
+#endif
+constexpr int32_t kWin = MCHIP_LANE_WIN;   // window words per lane (multiple of 16)
 constexpr int32_t kWinAlloc = kWin + 1;   // words allocated per lane (see LaneCoder::norm_fast)
 
 #ifndef MCHIP_LANE_STRIDE
@@ -372,56 +375,55 @@ __device__ __forceinline__ uint32_t decode_coefs_lane(LaneCoder &d, const QTab *
     for (int i = 0; i < 12; ++i) ctx[(4 + i) * kLaneStride] = qt->mag[i];
     for (int i = 0; i < 4; ++i) ctx[i * kLaneStride] = qt->sig[i];
     uint32_t band = 0, next_band = band_start[1];
-    // phases: 0 significance, 1 magnitude rung (slot ms = 4 + rung), 2 Exp-Golomb prefix
-    // (ms = 16 + ones so far), 3 sign
-    uint32_t ph = 0, p = 0, hist = 0, ms = 4, mag = 0;
+    // Bin state st: 0 significance, 1..12 magnitude rung st-1 (context slot
+    // st + 3), 13 Exp-Golomb prefix (nb ones so far), 14 sign.  The context
+    // slot of a bin is dens for st = 0, else min(st + 3, 16) (16: the dummy
+    // slot of bypass bins).
+    constexpr uint32_t kEgp = 13, kSgn = 14;
+    uint32_t st = 0, p = 0, hist = 0, nb = 0, mag = 0;
     uint32_t evp = next_band < eob - 1 ? next_band : eob - 1;   // next position needing the slow path
-    if (eob == 1) { hist = 1; ph = 1; }                          // position 0 is the last: implied
-    uint32_t slot = ph == 0 ? 0u : 4u;                           // context slot of the coming bin
-    // The coming bin's context is read one bin ahead (pnext); if the bin in
-    // between adapted the same slot, its updated value (p0n) is taken.
-    uint32_t pnext = ctx[slot * kLaneStride], p0n = 0;
-    bool same = false;
+    if (eob == 1) { hist = 1; st = 1; }                          // position 0 is the last: implied
+    uint32_t slot = st == 0 ? 0u : 4u;                           // context slot of the coming bin
+    uint32_t p0 = ctx[slot * kLaneStride];                       // its value, read one bin ahead
     while (p < eob) {
-        const uint32_t p0 = same ? p0n : pnext;
-        const bool byp = ph >= 2;
+        const bool byp = st >= kEgp;
         const uint32_t rb = d.range >> 1, rm = __umul24(d.range >> 12, p0);
         const uint32_t r0 = byp ? rb : rm;
         const bool b = d.code >= r0;
         d.code -= b ? r0 : 0u;
         d.range = (b && !byp) ? d.range - r0 : r0;
+        // adapt this bin's context before the next read, so that read sees it:
+        // p0 - p0/16 after a 1, p0 + (4096 - p0)/16 after a 0 (both floored),
+        // and p0 + floor((4096 - p0)/16) = p0 - floor((p0 - 4081)/16)
+        ctx[slot * kLaneStride] = p0 - (uint32_t)((int32_t)(p0 - (b ? 0u : 4081u)) >> 4);
+        d.norm_fast();     // issues the window read for the next bin
         // state transition
-        const bool isSig = ph == 0, isMag = ph == 1, isEgp = ph == 2, isSgn = ph == 3;
+        const bool isSig = st == 0, isEgp = st == kEgp, isSgn = st == kSgn;
         const uint32_t entry = p | ((b ? 0u - mag : mag) << 16);
         hist = isSig ? (hist << 1) | (b ? 1u : 0u) : hist;
         p += ((isSig && !b) || isSgn) ? 1u : 0u;
-        const uint32_t nph = isSig ? (b ? 1u : 0u) : (isSgn ? 0u : (b ? ph + (ms == 15u ? 1u : 0u) : 3u));
-        mag = isMag ? ms - 3u : (isEgp ? 12u : mag);
-        ms = isSig ? 4u : ms + (((isMag || isEgp) && b) ? 1u : 0u);
-        ph = nph;
-        // context of the next bin, read now so its latency overlaps the rest
+        mag = isSgn ? mag : (st < 12u ? st : 12u);
+        nb = isEgp ? nb + (b ? 1u : 0u) : 0u;
+        const uint32_t nx = st + 1u < kEgp ? st + 1u : kEgp;
+        const uint32_t nst = isSig ? (b ? 1u : 0u) : (isSgn ? 0u : (b ? nx : kSgn));
+        // the next bin's context, read now so its latency overlaps the rest
         uint32_t dens = (uint32_t)__popc(hist & 0xFFFFu);
         dens = dens < 3u ? dens : 3u;
-        const uint32_t nslot = nph == 0 ? dens : (nph == 1 ? ms : 16u);
-        pnext = ctx[nslot * kLaneStride];
-        // adapt this bin's context: p0 - p0/16 after a 1, p0 + (4096 - p0)/16
-        // after a 0 (both floored), p0 + floor((4096 - p0)/16) = p0 - floor((p0 - 4081)/16)
-        p0n = p0 - (uint32_t)((int32_t)(p0 - (b ? 0u : 4081u)) >> 4);
-        ctx[slot * kLaneStride] = p0n;
-        same = nslot == slot;
-        slot = nslot;
-        d.norm_fast();
+        const uint32_t sl = nst + 3u < 16u ? nst + 3u : 16u;
+        slot = nst == 0 ? dens : sl;
+        p0 = ctx[slot * kLaneStride];
+        st = nst;
+        MCHIP_SCHED_FENCE();
         sink.emit(entry, isSgn);
         // One test for every rare event: window slide, an Exp-Golomb prefix
         // bin (the suffix may follow), a band change or the last (implied)
         // position.
-        const bool evt = nph == 0 && p >= evp;
+        const bool evt = nst == 0 && p >= evp;
         if (d.need_slide() || isEgp || evt) {
             if (d.need_slide()) d.slide();
-            if (isEgp && (b ? ms == 16u + 64u : ms != 16u)) {
-                const uint32_t nb = ms - 16u;
+            if (isEgp && (b ? nb == 64u : nb != 0u)) {           // Exp-Golomb suffix
                 mag = 12u + ((1u << (nb & 31)) | d.bypass_n((int)nb)) - 1u;
-                ph = 3;
+                st = kSgn;
             }
             if (evt) {
                 if (p >= next_band && band < 7) {
@@ -429,14 +431,15 @@ __device__ __forceinline__ uint32_t decode_coefs_lane(LaneCoder &d, const QTab *
                     next_band = band_start[band + 1];
                     for (int i = 0; i < 4; ++i) ctx[i * kLaneStride] = qt->sig[band * 4 + i];
                 }
-                if (p + 1 == eob) { hist = (hist << 1) | 1u; ph = 1; ms = 4; }
+                if (p + 1 == eob) { hist = (hist << 1) | 1u; st = 1; }
                 evp = next_band < eob - 1 ? next_band : eob - 1;
             }
             uint32_t dn = (uint32_t)__popc(hist & 0xFFFFu);
             dn = dn < 3u ? dn : 3u;
-            slot = ph == 0 ? dn : (ph == 1 ? ms : 16u);
-            pnext = ctx[slot * kLaneStride];
-            same = false;
+            slot = st == 0 ? dn : (st + 3u < 16u ? st + 3u : 16u);
+            // a distinct (volatile) read keeps the compiler from merging it
+            // with the hot path's read and sinking both to the loop top
+            p0 = *(volatile uint32_t *)(ctx + slot * kLaneStride);
         }
     }
     return sink.count();

@@ -11,6 +11,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Scheduling fence for the instruction scheduler (no code): keeps the
+// serial decoders' long-latency reads issued early in each bin.
+#ifndef MCHIP_SCHED_FENCE
+#define MCHIP_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+#endif
+
 namespace mchip {
 
 constexpr int kNVox = 4096;

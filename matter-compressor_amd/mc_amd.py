@@ -22,7 +22,8 @@ class MchipError(RuntimeError):
 
 
 def lib_path():
-    return os.path.join(LIB_DIR, "libmc_amd.so")
+    """The in-tree build, or the library named by MC_AMD_LIB."""
+    return os.environ.get("MC_AMD_LIB") or os.path.join(LIB_DIR, "libmc_amd.so")
 
 
 def load():

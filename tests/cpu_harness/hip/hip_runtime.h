@@ -11,6 +11,7 @@
 #define __global__
 #define __forceinline__ inline
 #define __noinline__ __attribute__((noinline))
+#define MCHIP_SCHED_FENCE() ((void)0)
 struct uint2 { uint32_t x, y; };
 struct uint4 { uint32_t x, y, z, w; };
 static inline uint2 make_uint2(uint32_t x, uint32_t y) { uint2 r = {x, y}; return r; }
