@@ -119,6 +119,27 @@ int mc_archive_set_priors(struct mc_archive *a, const uint16_t plo[8][32],
  * reference's reader, src/mc_stream.c; declared in src/mc_archive.h:27) */
 void priors_load(const uint8_t *base);
 
+/* ---- HBM-resident serving (additive; replaces the reference's decoded-block
+ * cache for box batches, mc_cache_new_archive + mc_cache_get_copy,
+ * src/matter_compressor.h:425-444, src/mc_cache.c:737-741) -------------------
+ * mc_serve_open uploads the present chunks of one LOD to the GPU and starts a
+ * serving loop with an HBM arena of arena_bytes (0: half of the free device
+ * memory) of decoded blocks and up to `lanes` requests in flight.  A request
+ * is a batch of n boxes of dz x dy x dx voxels written to device memory at
+ * d_out + i*batch_stride; its voxels equal mc_archive_read_regions' for the
+ * same boxes.  Requests submitted while others run are admitted as lanes and
+ * arena slots free up and overlap with them on the device.  Chunks appended
+ * after mc_serve_open are not seen. */
+typedef struct mc_serve mc_serve;
+mc_serve *mc_serve_open(mc_archive *a, int lod, uint64_t arena_bytes, int lanes);
+int64_t   mc_serve_submit(mc_serve *s, const mc_box *boxes, int n, long dz, long dy, long dx,
+                          void *d_out, size_t batch_stride);          /* ticket >= 0, or -1 */
+int       mc_serve_poll(mc_serve *s, int64_t ticket);                 /* 1 done, 0 pending, -1 error */
+int       mc_serve_wait(mc_serve *s, int64_t ticket, double *latency_ms);
+void      mc_serve_close(mc_serve *s);
+/* the device serving loop behind s (for mchip_server_get_stats, include/mchip.h) */
+struct mchip_server *mc_serve_device(mc_serve *s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -145,6 +145,59 @@ int mchip_ctx_enable_timing(mchip_ctx *ctx, int on);
 int mchip_ctx_kernel_time(mchip_ctx *ctx, const char *kernel, double *total_ms, long *launches);
 int mchip_ctx_reset_timing(mchip_ctx *ctx);
 
+/* ---- Device-resident block store and serving loop ------------------------
+ * Replaces the reference's decoded-block cache (mc_cache, S3-FIFO over host
+ * memory, src/mc_cache.c:287-384) for box-batch serving: compressed blocks
+ * stay resident in HBM, decoded blocks live in an HBM arena, and requests
+ * are admitted, decoded and assembled on the device. */
+typedef struct mchip_store mchip_store;
+typedef struct mchip_server mchip_server;
+
+/* A store over a dense grid of gz*gy*gx 16^3 blocks; block (z,y,x) has
+ * index (z*gy + y)*gx + x.  The nbytes host bytes at payload are uploaded
+ * once; block i's payload is payload[off[i] .. off[i]+len[i]) coded at
+ * quality q[i]; len[i] == 0 marks an absent block (reads as zeros). */
+mchip_store *mchip_store_create(const uint8_t *payload, uint64_t nbytes, const uint64_t *off,
+                                const uint32_t *len, const float *q, int gz, int gy, int gx);
+void         mchip_store_destroy(mchip_store *st);
+uint64_t     mchip_store_device_bytes(const mchip_store *st);
+
+typedef struct {
+    uint64_t requests;        /* submitted */
+    uint64_t completed;
+    uint64_t boxes;
+    uint64_t block_refs;      /* distinct present blocks per request, summed */
+    uint64_t hits;            /* of those, already resident (or being decoded) */
+    uint64_t misses;          /* decoded into the arena */
+    uint64_t evictions;
+    uint64_t deferred;        /* admissions postponed for lanes or arena slots */
+    uint64_t arena_slots;
+} mchip_server_stats;
+
+/* A serving loop over st with an arena of arena_bytes / 4096 decoded-block
+ * slots (0: half of the device memory free at creation) and up to lanes
+ * requests in flight, each on its own stream.  A worker thread admits the
+ * submitted requests in arrival order as lanes and arena slots free up
+ * (CLOCK replacement; blocks of requests in flight are pinned), so requests
+ * that arrive while others run overlap with them on the device.  A request
+ * that needs a block another request in flight is decoding waits for that
+ * decode on the device instead of decoding it again. */
+mchip_server *mchip_server_create(mchip_store *st, uint64_t arena_bytes, int lanes);
+/* Submit n boxes of dz x dy x dx voxels; box i has origin (z, y, x) =
+ * origins[3i..3i+2] in voxels and is written densely to d_out + i*batch_stride
+ * (device memory that stays valid until the request completes).  Voxels
+ * outside the grid and in absent blocks are 0.  Returns a ticket >= 0, or -1. */
+int64_t      mchip_server_submit(mchip_server *sv, const int64_t *origins, int n, int64_t dz, int64_t dy,
+                                 int64_t dx, uint8_t *d_out, uint64_t batch_stride);
+/* 1 once the request has completed, 0 while it is pending, -1 on error. */
+int          mchip_server_poll(mchip_server *sv, int64_t ticket);
+/* Block until the request completes and release its ticket; latency_ms
+ * (optional) receives the time from submission to completion. */
+int          mchip_server_wait(mchip_server *sv, int64_t ticket, double *latency_ms);
+int          mchip_server_get_stats(mchip_server *sv, mchip_server_stats *out);
+/* Waits for the requests in flight, then frees the lanes and the arena. */
+void         mchip_server_destroy(mchip_server *sv);
+
 #ifdef __cplusplus
 }
 #endif

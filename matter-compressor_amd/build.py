@@ -35,8 +35,8 @@ HIP_FLAGS = [
 C_FLAGS = ["-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall",
            "-Wno-unused-function", "-I", INCLUDE, "-D_GNU_SOURCE"]
 
-HIP_SOURCES = ["mchip_kernels.hip", "mchip_api.hip"]
-HOST_SOURCES = ["mc_host.c", "mc_host_region.c"]
+HIP_SOURCES = ["mchip_kernels.hip", "mchip_api.hip", "mchip_serve.hip"]
+HOST_SOURCES = ["mc_host.c", "mc_host_region.c", "mc_host_serve.c"]
 
 
 def _run(cmd):

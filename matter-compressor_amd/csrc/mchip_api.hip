@@ -15,6 +15,7 @@
 
 #include "../../include/mchip.h"
 #include "mchip_internal.hpp"
+#include "mchip_runtime.hpp"
 
 using namespace mchip;
 
@@ -183,6 +184,21 @@ int qtab_slot(float q, uint32_t *slot) {
 }
 
 }  // namespace
+
+namespace mchip {
+int rt_fail(const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    t_err = buf;
+    return -1;
+}
+int rt_qtab_slot(float q, uint32_t *slot) { return qtab_slot(q, slot); }
+const QTab *rt_qtabs() { return G().d_qtabs; }
+int rt_decoder() { return g_tune.decoder; }
+}  // namespace mchip
 
 // ---------------------------------------------------------------------------
 // Context

@@ -51,6 +51,15 @@ def load():
         "mchip_d2h": (i, [vp, vp, vp, C.c_size_t]),
         "mchip_memset": (i, [vp, vp, i, C.c_size_t]),
         "mchip_d2d": (i, [vp, vp, vp, C.c_size_t]),
+        "mchip_store_create": (vp, [vp, C.c_uint64, vp, vp, vp, i, i, i]),
+        "mchip_store_destroy": (None, [vp]),
+        "mchip_store_device_bytes": (C.c_uint64, [vp]),
+        "mchip_server_create": (vp, [vp, C.c_uint64, i]),
+        "mchip_server_submit": (C.c_int64, [vp, vp, i, C.c_int64, C.c_int64, C.c_int64, vp, C.c_uint64]),
+        "mchip_server_poll": (i, [vp, C.c_int64]),
+        "mchip_server_wait": (i, [vp, C.c_int64, C.POINTER(C.c_double)]),
+        "mchip_server_get_stats": (i, [vp, vp]),
+        "mchip_server_destroy": (None, [vp]),
         "mchip_decode_chunks": (i, [vp, u8p, vp, vp, vp, i, u8p, C.c_uint64]),
         "mchip_decode_chunks_host": (i, [vp, vp, vp, i, u8p, C.c_uint64]),
         "mchip_decode_blocks_host": (i, [vp, vp, vp, vp, i, u8p]),
@@ -72,9 +81,13 @@ def set_tuning(key, value):
     check(load().mchip_set_tuning(key.encode(), int(value)))
 
 
+def last_error():
+    return load().mchip_last_error().decode(errors="replace")
+
+
 def check(rc):
     if rc != 0:
-        raise MchipError(load().mchip_last_error().decode(errors="replace"))
+        raise MchipError(last_error())
 
 
 def _ptr(a):
@@ -316,6 +329,12 @@ def api():
         "mc_archive_decode_chunk": (None, [vp, u64, vp, i]),
         "mc_archive_read_region": (None, [vp, i, lng, lng, lng, lng, lng, lng, vp, C.c_size_t, C.c_size_t, i]),
         "mc_archive_read_regions": (None, [vp, i, vp, i, lng, lng, lng, vp, C.c_size_t, i]),
+        "mc_serve_open": (vp, [vp, i, u64, i]),
+        "mc_serve_submit": (C.c_int64, [vp, vp, i, lng, lng, lng, vp, C.c_size_t]),
+        "mc_serve_poll": (i, [vp, C.c_int64]),
+        "mc_serve_wait": (i, [vp, C.c_int64, C.POINTER(C.c_double)]),
+        "mc_serve_close": (None, [vp]),
+        "mc_serve_device": (vp, [vp]),
         "mc_archive_block_present": (i, [vp, i, i, i, i]),
         "mc_archive_block_fraction": (f, [vp, i, i, i, i]),
         "mc_archive_sample_boxes": (i, [vp, i, u64, i, lng, lng, lng, f, vp]),
@@ -330,3 +349,51 @@ def api():
         fn.argtypes = args
     _api_declared = True
     return L
+
+
+SERVER_STATS = ["requests", "completed", "boxes", "block_refs", "hits", "misses", "evictions", "deferred",
+                "arena_slots"]
+
+
+def server_stats(server_handle):
+    """mchip_server_stats of a device serving loop as a dict."""
+    buf = (C.c_uint64 * len(SERVER_STATS))()
+    check(load().mchip_server_get_stats(server_handle, buf))
+    return dict(zip(SERVER_STATS, [int(v) for v in buf]))
+
+
+class Serve:
+    """mc_serve_*: HBM-resident box-batch serving of one archive LOD."""
+
+    def __init__(self, archive, lod=0, arena_bytes=0, lanes=8):
+        L = api()
+        self.h = L.mc_serve_open(archive, lod, arena_bytes, lanes)
+        if not self.h:
+            raise MchipError(f"mc_serve_open failed: {last_error()}")
+
+    def submit(self, origins, dz, dy, dx, d_out, batch_stride=None, offset=0):
+        o = np.ascontiguousarray(origins, np.int64).reshape(-1, 3)
+        n = o.shape[0]
+        stride = dz * dy * dx if batch_stride is None else batch_stride
+        boxes = (McBox * n)(*[McBox(int(a), int(b), int(c)) for a, b, c in o])
+        t = api().mc_serve_submit(self.h, boxes, n, dz, dy, dx, C.c_void_p(d_out.ptr + offset), stride)
+        if t < 0:
+            raise MchipError(f"mc_serve_submit failed: {last_error()}")
+        return t
+
+    def poll(self, ticket):
+        return api().mc_serve_poll(self.h, ticket)
+
+    def wait(self, ticket):
+        lat = C.c_double(0.0)
+        if api().mc_serve_wait(self.h, ticket, C.byref(lat)) != 0:
+            raise MchipError(f"serving request failed: {last_error()}")
+        return lat.value
+
+    def stats(self):
+        return server_stats(api().mc_serve_device(self.h))
+
+    def close(self):
+        if self.h:
+            api().mc_serve_close(self.h)
+            self.h = None
