@@ -22,7 +22,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 # hardware queues (HIP's default is 4); the runtime reads this variable when
 # it starts, so it is set before the library is loaded.
 QUEUES = 32
-os.environ["GPU_MAX_HW_QUEUES"] = str(QUEUES)
+# The serving leg runs in a child process of its own (--serve-only) with as
+# many hardware queues as it has streams: idle queues left by the per-chunk
+# leg would otherwise share the hardware's queue slots with the serving
+# lanes and delay their dispatches.
+SERVE_LANES = 8
+os.environ["GPU_MAX_HW_QUEUES"] = str(2 * SERVE_LANES if "--serve-only" in sys.argv else QUEUES)
 sys.path.insert(0, os.path.join(ROOT, "matter-compressor_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
@@ -91,6 +96,127 @@ def cpu_encode_and_baseline(chunks, specs, nthreads):
     return blobs, enc_gbs, dec_gbs, ns
 
 
+def serve_leg(rates, nthreads, lanes=8, arena_bytes=0, boxes_per_batch=64, edge=128, warm_batches=100):
+    """Serving of box batches from an HBM-resident 2048^3 archive
+    (volume2048_q6): batches of 64 boxes of 128^3 (mc_archive_sample_boxes,
+    seed 10000 + k, min material fraction 0.5) arrive as a Poisson process
+    (exponential gaps from seed 99) and are served by mc_serve_* into device
+    memory; latency is submission to completion."""
+    import tempfile
+    import threading
+    L = mc_amd.api()
+    specs = workloads.WORKLOADS["volume2048_q6"]()
+    path = os.path.join(tempfile.mkdtemp(prefix="mcserve_"), "vol2048.mca")
+    a = L.mc_archive_open_dims(path.encode(), 2048, 2048, 2048, C.c_float(6.0))
+    if not a:
+        raise RuntimeError("mc_archive_open_dims failed")
+    ctx = mc_amd.Ctx(0)
+    t0 = time.perf_counter()
+    for g in range(0, len(specs), 64):
+        grp = specs[g:g + 64]
+        ch = mc_amd.synth_chunks([x["seed"] for x in grp], [x["air"] for x in grp],
+                                 coords=[x["coord"] for x in grp], nthreads=nthreads)
+        blobs = ctx.encode_chunks_host(ch, [6.0] * len(grp))
+        for x, bl in zip(grp, blobs):
+            cz, cy, cx = x["coord"]
+            if L.mc_archive_append_chunk_compressed(a, 0, cz, cy, cx, bl, len(bl)) != 0:
+                raise RuntimeError("install failed")
+        del ch
+    log(f"[bench] serving volume built in {time.perf_counter() - t0:.1f}s")
+
+    def batch(k):
+        arr = (mc_amd.McBox * boxes_per_batch)()
+        got = L.mc_archive_sample_boxes(a, 0, C.c_uint64(10000 + k), boxes_per_batch, edge, edge, edge,
+                                        C.c_float(0.5), arr)
+        return [(arr[i].z0, arr[i].y0, arr[i].x0) for i in range(got)]
+
+    box_bytes = edge ** 3
+    sv = mc_amd.Serve(a, lod=0, arena_bytes=arena_bytes, lanes=lanes)
+    nbuf = 3 * lanes
+    bufs = [mc_amd.DeviceBuffer(boxes_per_batch * box_bytes) for _ in range(nbuf)]
+    # bit-exactness: one batch against mc_archive_read_regions (host output)
+    b0 = batch(0)
+    t = sv.submit(b0, edge, edge, edge, bufs[0])
+    cold_ms = sv.wait(t)
+    got = np.empty((len(b0), edge, edge, edge), np.uint8)
+    ctx.d2h(got, bufs[0])
+    want = np.empty_like(got)
+    arr = (mc_amd.McBox * len(b0))(*[mc_amd.McBox(*o) for o in b0])
+    L.mc_archive_read_regions(a, 0, arr, len(b0), edge, edge, edge, want.ctypes.data_as(C.c_void_p),
+                              box_bytes, 0)
+    exact = bool(np.array_equal(got, want))
+    del got, want
+    out = []
+    kbase = 1
+    # The arena (half of the free device memory by default) starts empty: a
+    # first run at 200 batches/s fills it from requests alone ("cold").  The
+    # arena then preloads every present block (it holds the whole volume) and
+    # the Poisson runs are served from decoded blocks ("resident").
+    runs = [("cold", 200, warm_batches)] + [("resident", lam, nb) for lam, nb in rates]
+    preload = None
+    for phase, lam, nb in runs:
+        if phase == "resident" and preload is None:
+            tp = time.perf_counter()
+            nl = sv.preload()
+            preload = {"blocks": nl, "ms": round(1e3 * (time.perf_counter() - tp), 1)}
+            log(f"[bench] serving preload: {nl} blocks in {preload['ms']} ms")
+        reqs = [batch(kbase + k) for k in range(nb)]
+        kbase += nb
+        gaps = np.random.default_rng(99).exponential(1.0 / lam, nb)
+        s0 = sv.stats()
+        lat = [None] * nb
+        tickets = [None] * nb
+        ready = [threading.Event() for _ in range(nb)]
+
+        def waiter():
+            for i in range(nb):
+                ready[i].wait()
+                lat[i] = sv.wait(tickets[i])
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        start = time.perf_counter()
+        due = start
+        for i in range(nb):
+            due += gaps[i]
+            d = due - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            if i >= nbuf:                 # output buffer reuse: its earlier request must be done
+                while lat[i - nbuf] is None:
+                    time.sleep(0.0002)
+            tickets[i] = sv.submit(reqs[i], edge, edge, edge, bufs[i % nbuf])
+            ready[i].set()
+        th.join()
+        wall = time.perf_counter() - start
+        s1 = sv.stats()
+        lv = np.array(lat)
+        vox = sum(len(r) for r in reqs) * box_bytes
+        refs = s1["block_refs"] - s0["block_refs"]
+        out.append({"phase": phase, "lambda_per_s": lam, "batches": nb, "p50_ms": round(float(np.percentile(lv, 50)), 3),
+                    "p99_ms": round(float(np.percentile(lv, 99)), 3), "mean_ms": round(float(lv.mean()), 3),
+                    "gbs": round(vox / wall / 1e9, 3),
+                    "hit_rate": round((s1["hits"] - s0["hits"]) / max(1, refs), 4),
+                    "decoded_blocks": s1["misses"] - s0["misses"], "evictions": s1["evictions"] - s0["evictions"]})
+        log(f"[bench] serving {phase} lambda={lam}/s: p50 {out[-1]['p50_ms']} ms p99 {out[-1]['p99_ms']} ms "
+            f"{out[-1]['gbs']} GB/s hit rate {out[-1]['hit_rate']}")
+    st = sv.stats()
+    sv.close()
+    for b_ in bufs:
+        b_.free()
+    L.mc_archive_close(a)
+    try:
+        os.remove(path)
+    except OSError:
+        pass
+    return {"workload": "volume2048_q6", "volume": "2048^3 u8 (8^3 chunks), q=6, air 0.30-0.95 smooth field",
+            "boxes_per_batch": boxes_per_batch, "box": f"{edge}^3", "arrivals": "Poisson, gaps from seed 99",
+            "arena_bytes": arena_bytes, "lanes": lanes, "cold_batch_ms": round(cold_ms, 3),
+            "bit_exact_vs_read_regions": exact, "rates": out, "arena_slots": st["arena_slots"],
+            "preload": preload,
+            "latency": "submission to completion, output in device memory"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,7 +226,26 @@ def main():
     ap.add_argument("--batched", action="store_true", help="also time all chunks in one launch")
     ap.add_argument("--streams", type=int, default=2 * QUEUES, help="HIP streams the per-chunk launches rotate over")
     ap.add_argument("--tune", action="append", default=[], help="key=value launch tuning (mchip_set_tuning)")
+    ap.add_argument("--serve-batches", default="200,300,400",
+                    help="batches served at 20, 50 and 100 batches/s (serving leg); 0 skips the leg")
+    ap.add_argument("--serve-only", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.serve_only:
+        nbs = [int(x) for x in args.serve_batches.split(",")]
+        print(json.dumps(serve_leg(list(zip((20, 50, 100), nbs)), cpu_threads(), lanes=SERVE_LANES)))
+        return
+    serving = None
+    nbs = [int(x) for x in args.serve_batches.split(",")]
+    if any(nbs) and args.gpus == 1:
+        # first, before this process touches the GPU: the serving child then
+        # has the device's hardware queue slots to itself
+        log("[bench] serving leg (child process)")
+        import subprocess
+        r = subprocess.run([sys.executable, "-u", os.path.abspath(__file__), "--serve-only",
+                            "--serve-batches", args.serve_batches], stdout=subprocess.PIPE, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"serving leg failed with status {r.returncode}")
+        serving = json.loads(r.stdout.strip().splitlines()[-1])
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
               "the codec shards by chunk with no cross-GPU exchange", file=sys.stderr)
@@ -282,11 +427,13 @@ def main():
                    "kernels_ms_per_chunk": {k: round(v[0] / max(1, v[1]) / 32.0, 4) for k, v in enc_k.items()}},
     }
     rec.update(extra)
-    print(json.dumps(rec))
     d_blobs.free()
     d_out.free()
     for c_ in ctxs:
         c_.close()
+    if serving is not None:
+        rec["serving"] = serving
+    print(json.dumps(rec))
 
 
 if __name__ == "__main__":

@@ -137,6 +137,9 @@ int64_t   mc_serve_submit(mc_serve *s, const mc_box *boxes, int n, long dz, long
 int       mc_serve_poll(mc_serve *s, int64_t ticket);                 /* 1 done, 0 pending, -1 error */
 int       mc_serve_wait(mc_serve *s, int64_t ticket, double *latency_ms);
 void      mc_serve_close(mc_serve *s);
+/* decode every present block of the LOD into the arena while it has room
+ * (before or between requests); loaded receives the number of blocks */
+int       mc_serve_preload(mc_serve *s, uint64_t *loaded);
 /* the device serving loop behind s (for mchip_server_get_stats, include/mchip.h) */
 struct mchip_server *mc_serve_device(mc_serve *s);
 

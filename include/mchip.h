@@ -194,6 +194,11 @@ int          mchip_server_poll(mchip_server *sv, int64_t ticket);
 /* Block until the request completes and release its ticket; latency_ms
  * (optional) receives the time from submission to completion. */
 int          mchip_server_wait(mchip_server *sv, int64_t ticket, double *latency_ms);
+/* Decode the store's present blocks into free or evictable arena slots, in
+ * grid order, until every block is resident or the arena is full (waits
+ * until no request is pending or in flight).  With an arena that holds the
+ * whole store, every later request is served from decoded blocks only. */
+int          mchip_server_preload(mchip_server *sv, uint64_t *loaded);
 int          mchip_server_get_stats(mchip_server *sv, mchip_server_stats *out);
 /* Waits for the requests in flight, then frees the lanes and the arena. */
 void         mchip_server_destroy(mchip_server *sv);

@@ -457,7 +457,13 @@ void launch_jobs(const uint8_t *base, const BlockJob *jobs, int n, const QTab *q
 // Assemble n boxes of dz x dy x dx voxels from arena blocks.  Box b's touched
 // block grid starts at block (gz0,gy0,gx0) = box origin >> 4 and has
 // gdim = (gz,gy,gx) cells in bmap[b * gz*gy*gx ...] (arena slot or -1 = 0).
-// Each thread writes 16 consecutive voxels of one output row.
+// Each thread writes 16 consecutive voxels of one output row: they come from
+// at most two 16-byte block rows (one 16-byte load each), aligned with byte
+// funnel shifts.
+__device__ __forceinline__ uint32_t pick4(uint32_t q, uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+    return q == 0 ? a : (q == 1 ? b : (q == 2 ? c : d));
+}
+
 __global__ __launch_bounds__(256) void k_crop(const uint8_t *__restrict__ arena, const int32_t *__restrict__ bmap,
                                               const int64_t *__restrict__ origin, int gz, int gy, int gx,
                                               int64_t dz, int64_t dy, int64_t dx, uint8_t *__restrict__ out,
@@ -471,30 +477,32 @@ __global__ __launch_bounds__(256) void k_crop(const uint8_t *__restrict__ arena,
     const int64_t Z = z0 + z, Y = y0 + y, X = x0 + x;
     const int32_t *bm = bmap + (size_t)b * gz * gy * gx;
     const int cz = (int)((Z >> 4) - (z0 >> 4)), cy = (int)((Y >> 4) - (y0 >> 4));
-    const int lz = (int)(Z & 15), ly = (int)(Y & 15);
-    uint8_t v[16];
-    #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        const int64_t XX = X + k;
-        const int cx = (int)((XX >> 4) - (x0 >> 4));
-        uint8_t val = 0;
-        if (x + k < dx && cx < gx) {
-            const int32_t sl = bm[(cz * gy + cy) * gx + cx];
-            if (sl >= 0) val = arena[(size_t)sl * kNVox + lz * 256 + ly * 16 + (int)(XX & 15)];
-        }
-        v[k] = val;
-    }
+    const int cxa = (int)((X >> 4) - (x0 >> 4));
+    const uint32_t row = (uint32_t)((Z & 15) * 256 + (Y & 15) * 16);
+    const uint32_t sh = (uint32_t)(X & 15);
+    const int32_t *cell = bm + (cz * gy + cy) * gx;
+    const int32_t sa = cxa < gx ? cell[cxa] : -1;
+    const int32_t sb = (sh && cxa + 1 < gx) ? cell[cxa + 1] : -1;
+    const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
+    const uint4 L = sa >= 0 ? *(const uint4 *)(arena + (size_t)sa * kNVox + row) : zero;
+    const uint4 R = sb >= 0 ? *(const uint4 *)(arena + (size_t)sb * kNVox + row) : zero;
+    // bytes sh .. sh+15 of the 32-byte concatenation L|R
+    const uint32_t q = sh >> 2, r = sh & 3;
+    const uint32_t w0 = pick4(q, L.x, L.y, L.z, L.w), w1 = pick4(q, L.y, L.z, L.w, R.x),
+                   w2 = pick4(q, L.z, L.w, R.x, R.y), w3 = pick4(q, L.w, R.x, R.y, R.z),
+                   w4 = pick4(q, R.x, R.y, R.z, R.w);
+    uint4 v;
+    v.x = __builtin_amdgcn_alignbyte(w1, w0, r);
+    v.y = __builtin_amdgcn_alignbyte(w2, w1, r);
+    v.z = __builtin_amdgcn_alignbyte(w3, w2, r);
+    v.w = __builtin_amdgcn_alignbyte(w4, w3, r);
     uint8_t *o = out + (size_t)b * batch_stride + (size_t)(z * dy + y) * dx + x;
     const int64_t n = dx - x < 16 ? dx - x : 16;
     if (n == 16 && (((uintptr_t)o) & 15) == 0) {
-        uint4 w;
-        w.x = v[0] | (v[1] << 8) | (v[2] << 16) | ((uint32_t)v[3] << 24);
-        w.y = v[4] | (v[5] << 8) | (v[6] << 16) | ((uint32_t)v[7] << 24);
-        w.z = v[8] | (v[9] << 8) | (v[10] << 16) | ((uint32_t)v[11] << 24);
-        w.w = v[12] | (v[13] << 8) | (v[14] << 16) | ((uint32_t)v[15] << 24);
-        *(uint4 *)o = w;
+        *(uint4 *)o = v;
     } else {
-        for (int k = 0; k < n; ++k) o[k] = v[k];
+        const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
+        for (int k = 0; k < n; ++k) o[k] = (uint8_t)(vv[k >> 2] >> (8 * (k & 3)));
     }
 }
 

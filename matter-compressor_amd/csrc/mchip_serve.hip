@@ -77,23 +77,31 @@ struct Req {
     std::string err;
 };
 
-int grow_device(void **p, size_t *cap, size_t need) {
+// Lane buffers grow by half again when a request needs more.  A replaced
+// buffer is not freed while the server runs (hipFree would wait for every
+// stream of the device, stalling the other lanes); it is kept on a list and
+// freed when the server is destroyed.
+struct Graveyard {
+    std::vector<void *> dev, host;
+};
+
+int grow_device(void **p, size_t *cap, size_t need, Graveyard &g) {
     if (need <= *cap) return 0;
-    if (*p) MCHIP_RT_CHECK(hipFree(*p));
+    if (*p) g.dev.push_back(*p);
     *p = nullptr;
     *cap = 0;
-    const size_t n = need + need / 4 + 4096;
+    const size_t n = need + need / 2 + 4096;
     MCHIP_RT_CHECK(hipMalloc(p, n));
     *cap = n;
     return 0;
 }
 
-int grow_pinned(void **p, size_t *cap, size_t need) {
+int grow_pinned(void **p, size_t *cap, size_t need, Graveyard &g) {
     if (need <= *cap) return 0;
-    if (*p) MCHIP_RT_CHECK(hipHostFree(*p));
+    if (*p) g.host.push_back(*p);
     *p = nullptr;
     *cap = 0;
-    const size_t n = need + need / 4 + 4096;
+    const size_t n = need + need / 2 + 4096;
     MCHIP_RT_CHECK(hipHostMalloc(p, n, hipHostMallocDefault));
     *cap = n;
     return 0;
@@ -121,6 +129,7 @@ struct mchip_server {
     bool stop = false;
     int64_t next_ticket = 0;
     mchip_server_stats stats{};
+    Graveyard grave;
 
     int evict_one() {
         for (uint32_t k = 0; k < 2 * nslots + 1; ++k) {
@@ -230,9 +239,9 @@ struct mchip_server {
             rollback(r);
             return -1;
         };
-        if (grow_pinned(&L.h_stage, &L.cap_h, total) || grow_device(&L.d_stage, &L.cap_d, total) ||
-            grow_device(&L.d_scratch, &L.cap_scratch, sizeof(uint32_t) * (sw + 64)) ||
-            grow_device(&L.d_info, &L.cap_info, sizeof(uint2) * (jobs.size() + 1)))
+        if (grow_pinned(&L.h_stage, &L.cap_h, total, grave) || grow_device(&L.d_stage, &L.cap_d, total, grave) ||
+            grow_device(&L.d_scratch, &L.cap_scratch, sizeof(uint32_t) * (sw + 64), grave) ||
+            grow_device(&L.d_info, &L.cap_info, sizeof(uint2) * (jobs.size() + 1), grave))
             return fail_req();
         uint8_t *h = (uint8_t *)L.h_stage;
         if (jb) memcpy(h, jobs.data(), jb);
@@ -316,8 +325,11 @@ struct mchip_server {
                 cv_work.wait(lk);
                 continue;
             }
+            // requests in flight: poll their completion events (the worker
+            // thread spins with yields so completions are seen within
+            // microseconds)
             lk.unlock();
-            std::this_thread::sleep_for(std::chrono::microseconds(20));
+            std::this_thread::yield();
             lk.lock();
         }
     }
@@ -404,10 +416,17 @@ mchip_server *mchip_server_create(mchip_store *st, uint64_t arena_bytes, int lan
     sv->stats.arena_slots = ns;
     sv->lanes.resize(lanes);
     for (Lane &L : sv->lanes) {
+        // initial lane buffers for a batch of 64 boxes of 128^3 (64 * 9^3
+        // touched blocks) of typical payloads; they grow when needed
+        const size_t jobs = 64 * 729;
         if (hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&L.ev_dec, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&L.ev_done, hipEventDisableTiming) != hipSuccess) {
-            rt_fail("mchip_server_create: stream or event creation failed");
+            hipEventCreateWithFlags(&L.ev_done, hipEventDisableTiming) != hipSuccess ||
+            grow_pinned(&L.h_stage, &L.cap_h, jobs * (sizeof(BlockJob) + 4) + 64 * 24, sv->grave) ||
+            grow_device(&L.d_stage, &L.cap_d, jobs * (sizeof(BlockJob) + 4) + 64 * 24, sv->grave) ||
+            grow_device(&L.d_scratch, &L.cap_scratch, jobs * (8ull * 320 + 32) * 4, sv->grave) ||
+            grow_device(&L.d_info, &L.cap_info, jobs * sizeof(uint2), sv->grave)) {
+            rt_fail("mchip_server_create: lane stream, event or buffer creation failed");
             mchip_server_destroy(sv);
             return nullptr;
         }
@@ -461,6 +480,70 @@ int mchip_server_wait(mchip_server *sv, int64_t ticket, double *latency_ms) {
     return rc;
 }
 
+int mchip_server_preload(mchip_server *sv, uint64_t *loaded) {
+    if (!sv) return rt_fail("null server");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    sv->cv_done.wait(lk, [sv] {                   // idle: nothing pending or in flight
+        if (!sv->pending.empty()) return false;
+        for (Lane &L : sv->lanes) if (L.ticket >= 0) return false;
+        return true;
+    });
+    const mchip_store &S = *sv->st;
+    Lane &L = sv->lanes[0];
+    std::vector<BlockJob> jobs;
+    std::vector<int32_t> pinned;
+    uint64_t n = 0;
+    auto unpin_all = [&]() {
+        for (int32_t s : pinned) sv->pins[s]--;
+    };
+    auto drop_jobs = [&]() {                      // jobs not decoded: their slots hold nothing
+        for (const BlockJob &j : jobs) {
+            sv->slot_of[sv->blk_of[j.slot]] = -1;
+            sv->blk_of[j.slot] = -1;
+        }
+        jobs.clear();
+    };
+    auto flush = [&]() -> int {
+        if (jobs.empty()) return 0;
+        uint64_t sw = 0;
+        for (BlockJob &j : jobs) { j.scratch_off = sw; sw += 8ull * j.len + 32ull; }
+        const size_t jb = sizeof(BlockJob) * jobs.size();
+        if (grow_pinned(&L.h_stage, &L.cap_h, jb, sv->grave) || grow_device(&L.d_stage, &L.cap_d, jb, sv->grave) ||
+            grow_device(&L.d_scratch, &L.cap_scratch, sizeof(uint32_t) * (sw + 64), sv->grave) ||
+            grow_device(&L.d_info, &L.cap_info, sizeof(uint2) * (jobs.size() + 1), sv->grave))
+            return -1;
+        memcpy(L.h_stage, jobs.data(), jb);
+        MCHIP_RT_CHECK(hipMemcpyAsync(L.d_stage, L.h_stage, jb, hipMemcpyHostToDevice, L.stream));
+        launch_jobs(S.d_pay, (const BlockJob *)L.d_stage, (int)jobs.size(), rt_qtabs(), (uint32_t *)L.d_scratch,
+                    (uint2 *)L.d_info, sv->d_arena, rt_decoder(), L.stream);
+        MCHIP_RT_CHECK(hipGetLastError());
+        MCHIP_RT_CHECK(hipStreamSynchronize(L.stream));
+        n += jobs.size();
+        jobs.clear();
+        return 0;
+    };
+    const size_t nb = (size_t)S.gz * S.gy * S.gx;
+    for (size_t id = 0; id < nb; ++id) {
+        if (!S.len[id] || sv->slot_of[id] >= 0) continue;
+        const int s = sv->evict_one();
+        if (s < 0) break;                         // the arena is full of pinned blocks
+        if (sv->blk_of[s] >= 0) { sv->slot_of[sv->blk_of[s]] = -1; }
+        sv->blk_of[s] = (int32_t)id;
+        sv->slot_of[id] = s;
+        sv->pins[s]++;                            // pinned until the preload ends
+        pinned.push_back(s);
+        jobs.push_back(BlockJob{S.off[id], 0, S.len[id], S.qslot[id], (uint32_t)s, 0});
+        if (jobs.size() == 65536 && flush()) { drop_jobs(); unpin_all(); return -1; }
+    }
+    const int rc = flush();
+    if (rc) drop_jobs();
+    unpin_all();
+    if (rc) return -1;
+    sv->stats.misses += n;
+    if (loaded) *loaded = n;
+    return 0;
+}
+
 int mchip_server_get_stats(mchip_server *sv, mchip_server_stats *out) {
     if (!sv || !out) return rt_fail("null argument");
     std::lock_guard<std::mutex> lk(sv->mu);
@@ -489,6 +572,8 @@ void mchip_server_destroy(mchip_server *sv) {
         if (L.stream) (void)hipStreamDestroy(L.stream);
     }
     for (auto &kv : sv->reqs) delete kv.second;
+    for (void *p : sv->grave.dev) (void)hipFree(p);
+    for (void *p : sv->grave.host) (void)hipHostFree(p);
     if (sv->d_arena) (void)hipFree(sv->d_arena);
     delete sv;
 }

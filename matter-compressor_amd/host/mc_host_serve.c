@@ -173,6 +173,11 @@ int mc_serve_wait(mc_serve *s, int64_t ticket, double *latency_ms) {
     return s ? mchip_server_wait(s->sv, ticket, latency_ms) : -1;
 }
 
+int mc_serve_preload(mc_serve *s, uint64_t *loaded) {
+    if (!s) return -1;
+    return mchip_server_preload(s->sv, loaded);
+}
+
 mchip_server *mc_serve_device(mc_serve *s) { return s ? s->sv : NULL; }
 
 void mc_serve_close(mc_serve *s) {

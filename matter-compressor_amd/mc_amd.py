@@ -59,6 +59,7 @@ def load():
         "mchip_server_poll": (i, [vp, C.c_int64]),
         "mchip_server_wait": (i, [vp, C.c_int64, C.POINTER(C.c_double)]),
         "mchip_server_get_stats": (i, [vp, vp]),
+        "mchip_server_preload": (i, [vp, C.POINTER(C.c_uint64)]),
         "mchip_server_destroy": (None, [vp]),
         "mchip_decode_chunks": (i, [vp, u8p, vp, vp, vp, i, u8p, C.c_uint64]),
         "mchip_decode_chunks_host": (i, [vp, vp, vp, i, u8p, C.c_uint64]),
@@ -335,6 +336,7 @@ def api():
         "mc_serve_wait": (i, [vp, C.c_int64, C.POINTER(C.c_double)]),
         "mc_serve_close": (None, [vp]),
         "mc_serve_device": (vp, [vp]),
+        "mc_serve_preload": (i, [vp, C.POINTER(C.c_uint64)]),
         "mc_archive_block_present": (i, [vp, i, i, i, i]),
         "mc_archive_block_fraction": (f, [vp, i, i, i, i]),
         "mc_archive_sample_boxes": (i, [vp, i, u64, i, lng, lng, lng, f, vp]),
@@ -389,6 +391,12 @@ class Serve:
         if api().mc_serve_wait(self.h, ticket, C.byref(lat)) != 0:
             raise MchipError(f"serving request failed: {last_error()}")
         return lat.value
+
+    def preload(self):
+        n = C.c_uint64(0)
+        if api().mc_serve_preload(self.h, C.byref(n)) != 0:
+            raise MchipError(f"mc_serve_preload failed: {last_error()}")
+        return n.value
 
     def stats(self):
         return server_stats(api().mc_serve_device(self.h))

@@ -42,6 +42,42 @@ def mixed_chunks(n=256, first_seed=1001, param_seed=7):
     return out
 
 
+def volume_chunks(nc=8, first_seed=20001, field_seed=13):
+    """An nc^3-chunk volume (2048^3 voxels for nc=8): chunk (cz, cy, cx) has
+    generator seed first_seed + index and air fraction 0.30 + 0.65 * v, v a
+    U(0,1) field on the chunk grid smoothed with [1,2,1] along each axis
+    (clamped at the edges) and min-max normalised, so air varies smoothly
+    across the volume."""
+    u = [[[unit(field_seed, (cz * nc + cy) * nc + cx, 0) for cx in range(nc)] for cy in range(nc)]
+         for cz in range(nc)]
+
+    def sm(f, axis):
+        g = [[[0.0] * nc for _ in range(nc)] for _ in range(nc)]
+        for z in range(nc):
+            for y in range(nc):
+                for x in range(nc):
+                    idx = [z, y, x]
+                    acc = 0.0
+                    for d, w in ((-1, 1.0), (0, 2.0), (1, 1.0)):
+                        j = list(idx)
+                        j[axis] = min(nc - 1, max(0, j[axis] + d))
+                        acc += w * f[j[0]][j[1]][j[2]]
+                    g[z][y][x] = acc / 4.0
+        return g
+
+    f = sm(sm(sm(u, 2), 1), 0)
+    flat = [f[z][y][x] for z in range(nc) for y in range(nc) for x in range(nc)]
+    lo, hi = min(flat), max(flat)
+    out = []
+    for z in range(nc):
+        for y in range(nc):
+            for x in range(nc):
+                v = (f[z][y][x] - lo) / (hi - lo) if hi > lo else 0.5
+                out.append(dict(seed=first_seed + (z * nc + y) * nc + x, air=0.30 + 0.65 * v, q=6.0, tau=0,
+                                preset=None, coord=(z, y, x)))
+    return out
+
+
 WORKLOADS = {
     # one chunk, CPU reference path
     "chunk_q6": lambda: plain_chunks(1),
@@ -49,4 +85,6 @@ WORKLOADS = {
     "plain64_q6": lambda: plain_chunks(64),
     # 256 chunks, mixed presets with max-error corrections
     "mixed256": lambda: mixed_chunks(256),
+    # 2048^3 volume (8^3 chunks) at q=6 with a smooth air field, for serving
+    "volume2048_q6": lambda: volume_chunks(8),
 }

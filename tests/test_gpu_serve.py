@@ -140,3 +140,31 @@ def test_serve_request_larger_than_arena_fails_cleanly(served):
         d_out.free()
     finally:
         sv.close()
+
+
+@pytest.mark.gpu
+def test_serve_preload_then_hits_only(served):
+    import mc_amd
+    a, big, ctx = served
+    sv = mc_amd.Serve(a, lod=0, arena_bytes=64 << 20, lanes=2)
+    try:
+        print("preload", flush=True)
+        n = sv.preload()
+        s0 = sv.stats()
+        out, _ = _run(sv, ctx, BOXES)
+        s1 = sv.stats()
+    finally:
+        sv.close()
+    assert n > 0 and s0["misses"] == n
+    assert s1["misses"] == s0["misses"] and s1["hits"] - s0["hits"] == s1["block_refs"] - s0["block_refs"]
+    for k, o in enumerate(BOXES):
+        assert np.array_equal(out[k], _crop(big, o)), f"box {k} {o}"
+    # an arena smaller than the store: preload fills it, requests stay exact
+    sv = mc_amd.Serve(a, lod=0, arena_bytes=700 * 4096, lanes=2)
+    try:
+        assert sv.preload() == 700
+        out, _ = _run(sv, ctx, BOXES[:4])
+    finally:
+        sv.close()
+    for k, o in enumerate(BOXES[:4]):
+        assert np.array_equal(out[k], _crop(big, o)), f"small arena box {k} {o}"
