@@ -96,6 +96,82 @@ def cpu_encode_and_baseline(chunks, specs, nthreads):
     return blobs, enc_gbs, dec_gbs, ns
 
 
+def mixed_leg(nthreads):
+    """Mixed-quality batch (mixed256: 256 chunks, 4 GiB, presets with max-error
+    corrections): batched GPU encode checked byte for byte against the
+    reference's blobs, then the batched decode autotuned on the batch, timed
+    eagerly and as HIP graph replays, checked against the reference's decoded
+    hashes."""
+    specs = workloads.WORKLOADS["mixed256"]()
+    man = json.load(open(os.path.join(ROOT, "tests", "golden", "chunk_manifest.json")))["mixed256"]
+    n = len(specs)
+    chunks = mc_amd.synth_chunks([x["seed"] for x in specs], [x["air"] for x in specs], nthreads=nthreads)
+    ctx = mc_amd.Ctx(0)
+    d_in = mc_amd.DeviceBuffer(n * mc_amd.CHUNK_BYTES)
+    ctx.h2d(d_in, chunks)
+    del chunks
+    cap = max(m["blob_len"] for m in man) + (1 << 20)
+    d_enc = mc_amd.DeviceBuffer(n * cap)
+    eoffs = np.arange(n, dtype=np.uint64) * cap
+    ecaps = np.full(n, cap, np.uint64)
+    qv = [x["q"] for x in specs]
+    tv = [x["tau"] for x in specs]
+    ctx.encode_chunks(d_in, qv, tv, d_enc, eoffs, ecaps)          # workspace allocation
+    ctx.sync()
+    t0 = time.perf_counter()
+    elens, est = ctx.encode_chunks(d_in, qv, tv, d_enc, eoffs, ecaps)
+    enc_s = time.perf_counter() - t0
+    d_in.free()
+    ok_enc = bool((est == 0).all())
+    for i in range(n):
+        b = np.empty(int(elens[i]), np.uint8)
+        ctx.d2h(b, d_enc, offset=int(eoffs[i]))
+        ok_enc &= len(b) == man[i]["blob_len"] and xxh64_hex(b) == man[i]["blob_xxh64"]
+    log(f"[bench] mixed encode {n * mc_amd.CHUNK_BYTES / enc_s / 1e9:.2f} GB/s, byte-identical={ok_enc}")
+    qs = np.array([x["q"] for x in specs], np.float32)
+    d_out = mc_amd.DeviceBuffer(n * mc_amd.CHUNK_BYTES)
+
+    def check():
+        out = np.empty(mc_amd.CHUNK_BYTES, np.uint8)
+        bad = 0
+        for i in range(n):
+            ctx.d2h(out, d_out, offset=i * mc_amd.CHUNK_BYTES)
+            bad += xxh64_hex(out) != man[i]["decoded_xxh64"]
+        return bad
+
+    tune = ctx.autotune_decode(d_enc, eoffs, elens, qs, d_out)
+    best = min(tune, key=lambda r: r["ms"])
+    log(f"[bench] mixed autotune: {tune}")
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ctx.decode_chunks(d_enc, eoffs, elens, qs, d_out)
+        ctx.sync()
+        ts.append(time.perf_counter() - t0)
+    bad_eager = check()
+    g = ctx.decode_graph(d_enc, eoffs, elens, qs, d_out)
+    tg = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ctx.graph_launch(g)
+        ctx.sync()
+        tg.append(time.perf_counter() - t0)
+    bad_graph = check()
+    ctx.graph_destroy(g)
+    d_enc.free()
+    d_out.free()
+    res = {"workload": "mixed256", "chunks": n, "decoded_bytes": n * mc_amd.CHUNK_BYTES,
+           "blob_bytes": int(np.sum(elens)),
+           "encode_gbs": round(n * mc_amd.CHUNK_BYTES / enc_s / 1e9, 3), "encode_byte_identical": ok_enc,
+           "decode_gbs": round(n * mc_amd.CHUNK_BYTES / min(ts) / 1e9, 3),
+           "decode_graph_gbs": round(n * mc_amd.CHUNK_BYTES / min(tg) / 1e9, 3),
+           "bit_exact": bad_eager == 0 and bad_graph == 0, "autotune": tune, "chosen": best,
+           "launch": "all chunks in one launch sequence (persistent entropy grid), and that sequence as a HIP graph"}
+    log(f"[bench] mixed decode {res['decode_gbs']} GB/s, graph {res['decode_graph_gbs']} GB/s, "
+        f"bit-exact={res['bit_exact']}")
+    return res
+
+
 def serve_leg(rates, nthreads, lanes=8, arena_bytes=0, boxes_per_batch=64, edge=128, warm_batches=100):
     """Serving of box batches from an HBM-resident 2048^3 archive
     (volume2048_q6): batches of 64 boxes of 128^3 (mc_archive_sample_boxes,
@@ -223,29 +299,43 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="plain64_q6")
-    ap.add_argument("--batched", action="store_true", help="also time all chunks in one launch")
+    ap.add_argument("--no-batched", action="store_true", help="skip the all-chunks-in-one-launch measurement")
+    ap.add_argument("--batched", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--streams", type=int, default=2 * QUEUES, help="HIP streams the per-chunk launches rotate over")
     ap.add_argument("--tune", action="append", default=[], help="key=value launch tuning (mchip_set_tuning)")
     ap.add_argument("--serve-batches", default="200,300,400",
                     help="batches served at 20, 50 and 100 batches/s (serving leg); 0 skips the leg")
     ap.add_argument("--serve-only", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--mixed-only", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-quality batched leg")
     args = ap.parse_args()
     if args.serve_only:
         nbs = [int(x) for x in args.serve_batches.split(",")]
         print(json.dumps(serve_leg(list(zip((20, 50, 100), nbs)), cpu_threads(), lanes=SERVE_LANES)))
         return
-    serving = None
+    if args.mixed_only:
+        print(json.dumps(mixed_leg(cpu_threads())))
+        return
+    # The legs on other workloads run first, each in a child process, before
+    # this process touches the GPU: each then has the device's hardware queue
+    # slots to itself.
+    import subprocess
+
+    def child(flag, *extra_args):
+        r = subprocess.run([sys.executable, "-u", os.path.abspath(__file__), flag, *extra_args],
+                           stdout=subprocess.PIPE, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"bench leg {flag} failed with status {r.returncode}")
+        return json.loads(r.stdout.strip().splitlines()[-1])
+
+    mixed = serving = None
+    if not args.no_mixed and args.gpus == 1:
+        log("[bench] mixed-quality batched leg (child process)")
+        mixed = child("--mixed-only")
     nbs = [int(x) for x in args.serve_batches.split(",")]
     if any(nbs) and args.gpus == 1:
-        # first, before this process touches the GPU: the serving child then
-        # has the device's hardware queue slots to itself
         log("[bench] serving leg (child process)")
-        import subprocess
-        r = subprocess.run([sys.executable, "-u", os.path.abspath(__file__), "--serve-only",
-                            "--serve-batches", args.serve_batches], stdout=subprocess.PIPE, text=True)
-        if r.returncode != 0:
-            raise RuntimeError(f"serving leg failed with status {r.returncode}")
-        serving = json.loads(r.stdout.strip().splitlines()[-1])
+        serving = child("--serve-only", "--serve-batches", args.serve_batches)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
               "the codec shards by chunk with no cross-GPU exchange", file=sys.stderr)
@@ -333,6 +423,18 @@ def main():
     calib = 256 << 20
     ctx.d2d(d_out, d_out, calib, dst_off=512 << 20, src_off=0)
     ctx.sync()
+    extra = {}
+    if not args.no_batched:
+        # all chunks in one launch sequence, measured before the per-chunk
+        # leg creates its streams (idle hardware queues slow single-stream work)
+        ctx.decode_chunks(d_blobs, offs, lens, qs, d_out)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(max(2, args.steps // 2)):
+            ctx.decode_chunks(d_blobs, offs, lens, qs, d_out)
+        ctx.sync()
+        dtb = (time.perf_counter() - t0) / max(2, args.steps // 2)
+        extra["batched_gbs"] = round(n * mc_amd.CHUNK_BYTES / dtb / 1e9, 3)
     ctxs = [ctx] + [mc_amd.Ctx(0) for _ in range(args.streams - 1)]
 
     def step():
@@ -377,16 +479,6 @@ def main():
     if bad:
         log(f"[bench] ERROR: {bad} decoded chunks differ from the reference")
 
-    extra = {}
-    if args.batched:
-        ctx.reset_timing()
-        ctx.sync()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            ctx.decode_chunks(d_blobs, offs, lens, qs, d_out)
-        ctx.sync()
-        dtb = time.perf_counter() - t0
-        extra["batched_gbs"] = n * mc_amd.CHUNK_BYTES / (dtb / args.steps) / 1e9
 
     # roofline of the dominant kernel (entropy decode, one launch per chunk):
     # algorithmic bytes per chunk = 16 MiB of voxels written + blob bytes read
@@ -431,6 +523,8 @@ def main():
     d_out.free()
     for c_ in ctxs:
         c_.close()
+    if mixed is not None:
+        rec["mixed"] = mixed
     if serving is not None:
         rec["serving"] = serving
     print(json.dumps(rec))

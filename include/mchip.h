@@ -145,6 +145,26 @@ int mchip_ctx_enable_timing(mchip_ctx *ctx, int on);
 int mchip_ctx_kernel_time(mchip_ctx *ctx, const char *kernel, double *total_ms, long *launches);
 int mchip_ctx_reset_timing(mchip_ctx *ctx);
 
+/* ---- Batched decode as a HIP graph, and the launch-shape autotuner ---------
+ * mchip_decode_graph_create captures the launches of one mchip_decode_chunks
+ * call (the index, entropy and reconstruction kernels of n resident blobs,
+ * with the tuning in force) into a HIP graph over a private workspace;
+ * mchip_graph_launch replays it on ctx's stream, decoding the blobs found at
+ * the captured device addresses into the captured output.  The graph holds
+ * no copies or memsets. */
+typedef struct mchip_graph mchip_graph;
+mchip_graph *mchip_decode_graph_create(mchip_ctx *ctx, const uint8_t *d_blobs, const uint64_t *blob_off,
+                                       const uint64_t *blob_len, const float *q, int n, uint8_t *d_out,
+                                       uint64_t out_stride);
+int          mchip_graph_launch(mchip_graph *g);
+void         mchip_graph_destroy(mchip_graph *g);
+/* Time the decode variants (entropy decoder, persistent launch and its
+ * workgroup count) on this batch, keep the fastest as the process tuning and
+ * write the timings as JSON to report (cap bytes). */
+int          mchip_autotune_decode(mchip_ctx *ctx, const uint8_t *d_blobs, const uint64_t *blob_off,
+                                   const uint64_t *blob_len, const float *q, int n, uint8_t *d_out,
+                                   uint64_t out_stride, char *report, size_t cap);
+
 /* ---- Device-resident block store and serving loop ------------------------
  * Replaces the reference's decoded-block cache (mc_cache, S3-FIFO over host
  * memory, src/mc_cache.c:287-384) for box-batch serving: compressed blocks

@@ -71,6 +71,10 @@ constexpr int kMaxQTabs = 4096;
 struct Tuning {
     int decoder = 2;          // entropy decoder: 0 per scan position, 1 branching state machine,
                               // 2 branch-free lane decoder over an LDS payload window
+    int enc_batch = 64;       // chunks per encode launch sequence (device workspace ~170 MB each)
+    int persistent = 1;       // batched decode (>= persist_min chunks): persistent entropy launch
+    int persist_min = 2;
+    int persist_wg = 0;       // workgroups of the persistent launch (0: occupancy x CUs)
 };
 Tuning g_tune;
 
@@ -220,6 +224,7 @@ struct mchip_ctx {
     void *d_stage_in = nullptr; size_t cap_stage_in = 0;
     void *d_stage_out = nullptr; size_t cap_stage_out = 0;
     void *d_aux = nullptr; size_t cap_aux = 0;
+    void *d_order = nullptr; size_t cap_order = 0;      // persistent decode: group order + counters
     // encode workspace
     void *d_ecd = nullptr; size_t cap_ecd = 0;
     void *d_einfo = nullptr; size_t cap_einfo = 0;
@@ -230,7 +235,7 @@ struct mchip_ctx {
     void *d_dst = nullptr; size_t cap_dst = 0;
     void *d_res = nullptr; size_t cap_res = 0;
     void *d_fmap = nullptr; size_t cap_fmap = 0;
-    void *h_res = nullptr;
+    void *h_res = nullptr; size_t cap_h_res = 0;
     // block jobs / crops
     void *d_jobs = nullptr; size_t cap_jobs = 0;
     void *d_jscratch = nullptr; size_t cap_jscratch = 0;
@@ -317,7 +322,102 @@ int collect_timing(mchip_ctx *c) {
     return 0;
 }
 
+// Workgroups of the persistent entropy launch: as many as the device keeps
+// resident at once (occupancy per CU x CUs).
+int persistent_grid() {
+    static int cached = 0;
+    if (cached) return cached;
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 1024;
+    if (occupancy_persist(&per) != hipSuccess || per <= 0) per = 4;
+    cached = cus * per;
+    return cached;
+}
+
 }  // namespace
+
+// Descriptors, workspace and launch arguments of a decode of n resident
+// chunk blobs (everything but the launches).
+static int prepare_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *blob_off, const uint64_t *blob_len,
+                   const float *q, int n, uint8_t *d_out, uint64_t out_stride, DecodeArgs &a) {
+    if (!c) return fail("null context");
+    if (n <= 0) return 0;
+    if (!d_blobs || !blob_off || !blob_len || !q || !d_out) return fail("null argument");
+    if (out_stride < (uint64_t)16777216) return fail("out_stride %llu < 256^3", (unsigned long long)out_stride);
+    // descriptors: inside the kernel arguments for small calls, else staged
+    // through pinned memory and copied on the stream
+    const bool inl = n <= kInlineDesc;
+    ChunkDesc local[kInlineDesc];
+    ChunkDesc *cd = local;
+    if (!inl) {
+        void *h = nullptr;
+        if (desc_buffer(c, sizeof(ChunkDesc) * (size_t)n, &h)) return -1;
+        cd = (ChunkDesc *)h;
+    }
+    uint64_t sw = 0;
+    for (int i = 0; i < n; ++i) {
+        uint32_t slot;
+        if (qtab_slot(q[i], &slot)) return -1;
+        if (blob_len[i] > 0xFFFFFFF0ull) return fail("blob %d too long", i);
+        cd[i].blob_off = blob_off[i];
+        cd[i].blob_len = blob_len[i];
+        cd[i].out_off = (uint64_t)i * out_stride;
+        cd[i].scratch_off = sw;
+        cd[i].qtab = slot;
+        cd[i].pad = 0;
+        sw += scratch_words_for(blob_len[i]);
+    }
+    size_t nb = (size_t)n * kNVox;
+    if ((!inl && ensure(c, &c->d_cd, &c->cap_cd, sizeof(ChunkDesc) * n)) ||
+        ensure(c, &c->d_rec, &c->cap_rec, sizeof(BlkRec) * nb) ||
+        ensure(c, &c->d_plist, &c->cap_plist, sizeof(uint16_t) * nb) ||
+        ensure(c, &c->d_cinfo, &c->cap_cinfo, sizeof(ChunkInfo) * n) ||
+        ensure(c, &c->d_binfo, &c->cap_binfo, sizeof(uint2) * nb) ||
+        ensure(c, &c->d_scratch, &c->cap_scratch, sizeof(uint32_t) * sw))
+        return -1;
+    DescRef dref{};
+    if (inl) {
+        dref.ptr = nullptr;
+        for (int i = 0; i < n; ++i) dref.inl[i] = cd[i];
+    } else {
+        HIPCHK(hipMemcpyAsync(c->d_cd, cd, sizeof(ChunkDesc) * n, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipEventRecord(c->ev_desc, c->stream));
+        c->desc_pending = true;
+        dref.ptr = (const ChunkDesc *)c->d_cd;
+    }
+    a.blobs = d_blobs;
+    a.cd = dref;
+    a.qtabs = G().d_qtabs;
+    a.rec = (BlkRec *)c->d_rec;
+    a.plist = (uint16_t *)c->d_plist;
+    a.cinfo = (ChunkInfo *)c->d_cinfo;
+    a.scratch = (uint32_t *)c->d_scratch;
+    a.binfo = (uint2 *)c->d_binfo;
+    a.out = d_out;
+    a.n = n;
+    a.decoder = g_tune.decoder;
+    a.persistent = g_tune.persistent && n >= g_tune.persist_min && a.decoder == 2;
+    a.nwg = 0;
+    a.order = nullptr;
+    a.pctl = nullptr;
+    if (a.persistent) {
+        if (ensure(c, &c->d_order, &c->cap_order, sizeof(uint32_t) * ((size_t)n * 64 + 64))) return -1;
+        a.order = (uint32_t *)c->d_order;
+        a.pctl = a.order + (size_t)n * 64;
+        a.nwg = g_tune.persist_wg ? g_tune.persist_wg : persistent_grid();
+        if (a.nwg > n * 64) a.nwg = n * 64;
+    }
+    return 0;
+}
+
+static void launch_decode(mchip_ctx *c, const DecodeArgs &a) {
+    timed(c, "index", [&] { launch_index(a, c->stream); });
+    timed(c, "entropy", [&] { launch_entropy(a, c->stream); });
+    timed(c, "recon", [&] { launch_recon(a, c->stream); });
+}
+
 
 extern "C" {
 
@@ -346,6 +446,18 @@ int mchip_set_priors(const uint16_t *plo, const uint16_t *phi) {
 
 int mchip_set_tuning(const char *key, int value) {
     if (!key) return fail("null key");
+    if (!strcmp(key, "persistent")) { g_tune.persistent = value != 0; return 0; }
+    if (!strcmp(key, "persist_min")) { g_tune.persist_min = value < 1 ? 1 : value; return 0; }
+    if (!strcmp(key, "persist_wg")) {
+        if (value < 0 || value > 1 << 20) return fail("persist_wg out of range");
+        g_tune.persist_wg = value;
+        return 0;
+    }
+    if (!strcmp(key, "enc_batch")) {
+        if (value < 1 || value > 1024) return fail("enc_batch must be in [1, 1024]");
+        g_tune.enc_batch = value;
+        return 0;
+    }
     if (!strcmp(key, "decoder")) {
         if (value < 0 || value > 2) return fail("decoder must be 0, 1 or 2");
         g_tune.decoder = value;
@@ -374,7 +486,7 @@ void mchip_ctx_destroy(mchip_ctx *c) {
     void *bufs[] = {c->d_cd, c->d_rec, c->d_plist, c->d_cinfo, c->d_binfo, c->d_scratch,
                     c->d_stage_in, c->d_stage_out, c->d_aux, c->d_ecd, c->d_einfo, c->d_levels,
                     c->d_corr, c->d_slots, c->d_plen, c->d_dst, c->d_res, c->d_fmap,
-                    c->d_jobs, c->d_jscratch, c->d_jinfo, c->d_bmap, c->d_arena};
+                    c->d_jobs, c->d_jscratch, c->d_jinfo, c->d_bmap, c->d_arena, c->d_order};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (c->h_desc) (void)hipHostFree(c->h_desc);
@@ -427,65 +539,116 @@ int mchip_decode_chunks(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *bl
                         uint64_t out_stride) {
     if (!c) return fail("null context");
     if (n <= 0) return 0;
-    if (!d_blobs || !blob_off || !blob_len || !q || !d_out) return fail("null argument");
-    if (out_stride < (uint64_t)16777216) return fail("out_stride %llu < 256^3", (unsigned long long)out_stride);
-    // descriptors: inside the kernel arguments for small calls, else staged
-    // through pinned memory and copied on the stream
-    const bool inl = n <= kInlineDesc;
-    ChunkDesc local[kInlineDesc];
-    ChunkDesc *cd = local;
-    if (!inl) {
-        void *h = nullptr;
-        if (desc_buffer(c, sizeof(ChunkDesc) * (size_t)n, &h)) return -1;
-        cd = (ChunkDesc *)h;
-    }
-    uint64_t sw = 0;
-    for (int i = 0; i < n; ++i) {
-        uint32_t slot;
-        if (qtab_slot(q[i], &slot)) return -1;
-        if (blob_len[i] > 0xFFFFFFF0ull) return fail("blob %d too long", i);
-        cd[i].blob_off = blob_off[i];
-        cd[i].blob_len = blob_len[i];
-        cd[i].out_off = (uint64_t)i * out_stride;
-        cd[i].scratch_off = sw;
-        cd[i].qtab = slot;
-        cd[i].pad = 0;
-        sw += scratch_words_for(blob_len[i]);
-    }
-    size_t nb = (size_t)n * kNVox;
-    if ((!inl && ensure(c, &c->d_cd, &c->cap_cd, sizeof(ChunkDesc) * n)) ||
-        ensure(c, &c->d_rec, &c->cap_rec, sizeof(BlkRec) * nb) ||
-        ensure(c, &c->d_plist, &c->cap_plist, sizeof(uint16_t) * nb) ||
-        ensure(c, &c->d_cinfo, &c->cap_cinfo, sizeof(ChunkInfo) * n) ||
-        ensure(c, &c->d_binfo, &c->cap_binfo, sizeof(uint2) * nb) ||
-        ensure(c, &c->d_scratch, &c->cap_scratch, sizeof(uint32_t) * sw))
-        return -1;
-    DescRef dref{};
-    if (inl) {
-        dref.ptr = nullptr;
-        for (int i = 0; i < n; ++i) dref.inl[i] = cd[i];
-    } else {
-        HIPCHK(hipMemcpyAsync(c->d_cd, cd, sizeof(ChunkDesc) * n, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipEventRecord(c->ev_desc, c->stream));
-        c->desc_pending = true;
-        dref.ptr = (const ChunkDesc *)c->d_cd;
-    }
     DecodeArgs a;
-    a.blobs = d_blobs;
-    a.cd = dref;
-    a.qtabs = G().d_qtabs;
-    a.rec = (BlkRec *)c->d_rec;
-    a.plist = (uint16_t *)c->d_plist;
-    a.cinfo = (ChunkInfo *)c->d_cinfo;
-    a.scratch = (uint32_t *)c->d_scratch;
-    a.binfo = (uint2 *)c->d_binfo;
-    a.out = d_out;
-    a.n = n;
-    a.decoder = g_tune.decoder;
-    timed(c, "index", [&] { launch_index(a, c->stream); });
-    timed(c, "entropy", [&] { launch_entropy(a, c->stream); });
-    timed(c, "recon", [&] { launch_recon(a, c->stream); });
+    if (prepare_decode(c, d_blobs, blob_off, blob_len, q, n, d_out, out_stride, a)) return -1;
+    launch_decode(c, a);
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+struct mchip_graph {
+    mchip_ctx *ctx = nullptr;       // private context: its workspace belongs to the graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t stream = nullptr;   // the stream replays are launched on
+};
+
+mchip_graph *mchip_decode_graph_create(mchip_ctx *caller, const uint8_t *d_blobs, const uint64_t *blob_off,
+                                       const uint64_t *blob_len, const float *q, int n, uint8_t *d_out,
+                                       uint64_t out_stride) {
+    if (!caller || n <= 0) { fail("mchip_decode_graph_create: bad argument"); return nullptr; }
+    auto *g = new mchip_graph;
+    g->stream = caller->stream;
+    g->ctx = mchip_ctx_create();
+    DecodeArgs a;
+    bool ok = g->ctx && !prepare_decode(g->ctx, d_blobs, blob_off, blob_len, q, n, d_out, out_stride, a) &&
+              hipStreamSynchronize(g->ctx->stream) == hipSuccess;
+    if (ok) {
+        // the launches only (no copies, no memsets): k_group_order resets the
+        // persistent launch's counter on every replay
+        const bool timing = g->ctx->timing;
+        g->ctx->timing = false;
+        ok = hipStreamBeginCapture(g->ctx->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
+            launch_decode(g->ctx, a);
+            ok = hipStreamEndCapture(g->ctx->stream, &g->graph) == hipSuccess && g->graph &&
+                 hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) == hipSuccess;
+        }
+        g->ctx->timing = timing;
+    }
+    if (!ok) {
+        fail("mchip_decode_graph_create: capture failed");
+        mchip_graph_destroy(g);
+        return nullptr;
+    }
+    return g;
+}
+
+int mchip_graph_launch(mchip_graph *g) {
+    if (!g || !g->exec) return fail("null graph");
+    HIPCHK(hipGraphLaunch(g->exec, g->stream));
+    return 0;
+}
+
+void mchip_graph_destroy(mchip_graph *g) {
+    if (!g) return;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->ctx) mchip_ctx_destroy(g->ctx);
+    delete g;
+}
+
+int mchip_autotune_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *blob_off, const uint64_t *blob_len,
+                          const float *q, int n, uint8_t *d_out, uint64_t out_stride, char *report, size_t cap) {
+    if (!c || n <= 0) return fail("mchip_autotune_decode: bad argument");
+    const Tuning saved = g_tune;
+    struct Cand { int decoder, persistent, wg_div; };
+    std::vector<Cand> cands = {{2, 1, 1}, {2, 1, 2}, {2, 0, 1}, {0, 0, 1}};
+    if (n < 2) cands = {{2, 0, 1}, {0, 0, 1}};
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    std::string rep = "[";
+    float best = 1e30f;
+    Cand pick = cands[0];
+    for (size_t k = 0; k < cands.size(); ++k) {
+        g_tune.decoder = cands[k].decoder;
+        g_tune.persistent = cands[k].persistent;
+        g_tune.persist_min = 1;
+        g_tune.persist_wg = cands[k].persistent ? persistent_grid() / cands[k].wg_div : 0;
+        float t = 1e30f;
+        for (int rep_i = 0; rep_i < 2; ++rep_i) {                // best of two runs
+            HIPCHK(hipEventRecord(e0, c->stream));
+            if (mchip_decode_chunks(c, d_blobs, blob_off, blob_len, q, n, d_out, out_stride)) {
+                g_tune = saved;
+                return -1;
+            }
+            HIPCHK(hipEventRecord(e1, c->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < t) t = ms;
+        }
+        char buf[160];
+        snprintf(buf, sizeof buf, "%s{\"decoder\": %d, \"persistent\": %d, \"persist_wg\": %d, \"ms\": %.3f}",
+                 k ? ", " : "", cands[k].decoder, cands[k].persistent, g_tune.persist_wg, t);
+        rep += buf;
+        if (t < best) { best = t; pick = cands[k]; }
+    }
+    rep += "]";
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g_tune = saved;
+    g_tune.decoder = pick.decoder;
+    g_tune.persistent = pick.persistent;
+    if (pick.persistent) {
+        g_tune.persist_min = n < saved.persist_min ? n : saved.persist_min;
+        g_tune.persist_wg = persistent_grid() / pick.wg_div;
+    }
+    if (report && cap) {
+        snprintf(report, cap, "%s", rep.c_str());
+    }
     return 0;
 }
 
@@ -569,8 +732,9 @@ int mchip_encode_chunks(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride, c
     if (n <= 0) return 0;
     if (!d_in || !q || !d_out || !out_off || !out_cap || !blob_len) return fail("null argument");
     if (in_stride % 16 || ((uintptr_t)d_in) % 16) return fail("input rasters must be 16-byte aligned");
-    for (int b0 = 0; b0 < n; b0 += kEncBatch) {
-        const int m = n - b0 < kEncBatch ? n - b0 : kEncBatch;
+    const int eb = g_tune.enc_batch;
+    for (int b0 = 0; b0 < n; b0 += eb) {
+        const int m = n - b0 < eb ? n - b0 : eb;
         void *h = nullptr;
         if (desc_buffer(c, sizeof(EncChunkDesc) * (size_t)m, &h)) return -1;
         EncChunkDesc *cd = (EncChunkDesc *)h;
@@ -594,10 +758,15 @@ int mchip_encode_chunks(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride, c
             ensure(c, &c->d_slots, &c->cap_slots, (size_t)kSlotBytes * nb) ||
             ensure(c, &c->d_plen, &c->cap_plen, sizeof(uint32_t) * nb) ||
             ensure(c, &c->d_dst, &c->cap_dst, sizeof(uint32_t) * nb) ||
-            ensure(c, &c->d_res, &c->cap_res, sizeof(EncChunkResult) * kEncBatch) ||
-            ensure(c, &c->d_fmap, &c->cap_fmap, (size_t)kFmapStride * kEncBatch))
+            ensure(c, &c->d_res, &c->cap_res, sizeof(EncChunkResult) * m) ||
+            ensure(c, &c->d_fmap, &c->cap_fmap, (size_t)kFmapStride * m))
             return -1;
-        if (!c->h_res) HIPCHK(hipHostMalloc(&c->h_res, sizeof(EncChunkResult) * kEncBatch, hipHostMallocDefault));
+        if ((size_t)m > c->cap_h_res) {
+            if (c->h_res) HIPCHK(hipHostFree(c->h_res));
+            c->h_res = nullptr;
+            HIPCHK(hipHostMalloc(&c->h_res, sizeof(EncChunkResult) * m, hipHostMallocDefault));
+            c->cap_h_res = m;
+        }
         HIPCHK(hipMemcpyAsync(c->d_ecd, cd, sizeof(EncChunkDesc) * m, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipEventRecord(c->ev_desc, c->stream));
         c->desc_pending = true;
@@ -646,8 +815,9 @@ int mchip_encode_chunks_host(mchip_ctx *c, const uint8_t *chunks, const float *q
         tot += (caps[i] + 255) & ~255ull;
     }
     std::vector<int> st(n, 0);
-    for (int b0 = 0; b0 < n; b0 += kEncBatch) {
-        const int m = n - b0 < kEncBatch ? n - b0 : kEncBatch;
+    const int eb = g_tune.enc_batch;
+    for (int b0 = 0; b0 < n; b0 += eb) {
+        const int m = n - b0 < eb ? n - b0 : eb;
         uint64_t base = off[b0], span = 0;
         for (int i = 0; i < m; ++i) span = off[b0 + i] + ((caps[b0 + i] + 255) & ~255ull) - base;
         if (ensure(c, &c->d_stage_in, &c->cap_stage_in, (size_t)m * CH + 256) ||

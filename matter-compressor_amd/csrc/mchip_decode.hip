@@ -182,6 +182,100 @@ __global__ __launch_bounds__(64) void k_entropy(const uint8_t *__restrict__ blob
 }
 
 // ---------------------------------------------------------------------------
+// Persistent entropy decode over a batch of chunks of uneven cost (mixed
+// qualities): the 64-block groups of all chunks (a chunk's group k holds its
+// blocks of sorted rank 64k .. 64k+63, longest payloads first) are ordered by
+// the payload length of their first block, longest first, and a machine-sized
+// grid of workgroups takes them in that order through an atomic counter, so
+// the heaviest groups start first and no workgroup idles while work remains.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_group_order(const BlkRec *__restrict__ rec,
+                                                      const uint16_t *__restrict__ plist,
+                                                      const ChunkInfo *__restrict__ cinfo, int n,
+                                                      uint32_t *__restrict__ order, uint32_t *__restrict__ pctl) {
+    constexpr int kBuckets = 256;             // 64-byte payload-length buckets, longest first
+    __shared__ uint32_t hist[kBuckets], base[kBuckets];
+    const int t = threadIdx.x;
+    for (int i = t; i < kBuckets; i += 1024) hist[i] = 0;
+    __syncthreads();
+    const int ng = n * 64;
+    auto key = [&](int g, bool &valid) -> uint32_t {
+        const int c = g >> 6, k = g & 63;
+        const uint32_t np = cinfo[c].np;
+        valid = (uint32_t)k * 64 < np;
+        if (!valid) return 0;
+        const uint32_t len = rec[(size_t)c * kNVox + plist[(size_t)c * kNVox + k * 64]].len;
+        const uint32_t b = len >> 6;
+        return kBuckets - 1 - (b < kBuckets - 1 ? b : kBuckets - 1);
+    };
+    for (int g = t; g < ng; g += 1024) {
+        bool v;
+        const uint32_t b = key(g, v);
+        if (v) atomicAdd(&hist[b], 1u);
+    }
+    __syncthreads();
+    if (t == 0) {
+        uint32_t acc = 0;
+        for (int i = 0; i < kBuckets; ++i) { base[i] = acc; acc += hist[i]; }
+        pctl[0] = 0;                         // work counter (reset here, so a graph replay needs no memset)
+        pctl[1] = acc;                       // groups
+    }
+    __syncthreads();
+    for (int g = t; g < ng; g += 1024) {
+        bool v;
+        const uint32_t b = key(g, v);
+        if (v) order[atomicAdd(&base[b], 1u)] = (uint32_t)g;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_entropy_persist(const uint8_t *__restrict__ blobs, const DescRef cd,
+                                                        const QTab *__restrict__ qtabs,
+                                                        const BlkRec *__restrict__ rec,
+                                                        const uint16_t *__restrict__ plist,
+                                                        const ChunkInfo *__restrict__ cinfo,
+                                                        uint32_t *__restrict__ scratch, uint2 *__restrict__ binfo,
+                                                        const uint32_t *__restrict__ order,
+                                                        uint32_t *__restrict__ pctl) {
+    __shared__ uint32_t sctx[17 * 64];
+    __shared__ uint32_t pwin[kWinAlloc * 64];
+    __shared__ QTab sq;
+    __shared__ uint16_t sband[9];
+    if (threadIdx.x < 9) sband[threadIdx.x] = c_band_start[threadIdx.x];
+    const uint32_t ngroups = pctl[1];
+    int cur = -1;                                // chunk whose quality table is in sq
+    for (;;) {
+        uint32_t w = 0;
+        if (threadIdx.x == 0) w = atomicAdd(&pctl[0], 1u);
+        w = __shfl(w, 0);                        // one wave per workgroup
+        if (w >= ngroups) break;
+        const uint32_t g = order[w];
+        const int c = (int)(g >> 6), k = (int)(g & 63);
+        __syncthreads();                         // the previous group is done with sq
+        if (c != cur) {
+            const uint32_t *src = (const uint32_t *)(qtabs + cd[c].qtab);
+            uint32_t *dst = (uint32_t *)&sq;
+            for (uint32_t i = threadIdx.x; i < sizeof(QTab) / 4; i += 64) dst[i] = src[i];
+            cur = c;
+        }
+        __syncthreads();
+        const uint32_t slot = (uint32_t)k * 64 + threadIdx.x;
+        if (slot < cinfo[c].np) {
+            const size_t cb = (size_t)c * kNVox;
+            const int bi = plist[cb + slot];
+            const BlkRec r = rec[cb + bi];
+            if (r.off == 0 || r.len == 0) {
+                binfo[cb + bi] = make_uint2(kStatusFallback << 16, 0);
+            } else {
+                const uint8_t *pay = blobs + cd[c].blob_off + r.off;
+                const uint64_t wofs = cd[c].scratch_off + 8ull * (r.off - cinfo[c].pay0) + 32ull * r.slot;
+                binfo[cb + bi] = entropy_to_list_lane(pay, r.len, &sq, sband, scratch + wofs, 8u * r.len + 32u,
+                                                      sctx + threadIdx.x, pwin + threadIdx.x);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // k_recon: one 256-thread workgroup per block.
 // ---------------------------------------------------------------------------
 constexpr int kPad = 17;                  // padded LDS line stride (floats)
@@ -388,7 +482,17 @@ hipError_t upload_constants(const float *cmf, const uint16_t *band_start, const 
 void launch_index(const DecodeArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_index, dim3(a.n), dim3(256), 0, s, a.blobs, a.cd, a.rec, a.plist, a.cinfo);
 }
+hipError_t occupancy_persist(int *per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_entropy_persist, 64, 0);
+}
+
 void launch_entropy(const DecodeArgs &a, hipStream_t s) {
+    if (a.persistent && a.decoder == kDecLane) {
+        hipLaunchKernelGGL(k_group_order, dim3(1), dim3(1024), 0, s, a.rec, a.plist, a.cinfo, a.n, a.order, a.pctl);
+        hipLaunchKernelGGL(k_entropy_persist, dim3(a.nwg), dim3(64), 0, s, a.blobs, a.cd, a.qtabs, a.rec, a.plist,
+                           a.cinfo, a.scratch, a.binfo, a.order, a.pctl);
+        return;
+    }
     if (a.decoder == kDecState)
         hipLaunchKernelGGL(k_entropy<kDecState>, dim3(kNVox / 64, a.n), dim3(64), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
                            a.plist, a.cinfo, a.scratch, a.binfo);

@@ -53,6 +53,13 @@ struct DecodeArgs {
     uint8_t *out;
     int n;
     int decoder;            // entropy decoder variant (tuning: 0 position, 1 state, 2 lane)
+    // persistent entropy launch (lane decoder): nwg workgroups take 64-block
+    // groups from order[] (longest payloads first, over all chunks) through
+    // the counter at pctl[0]; pctl[1] holds the number of groups
+    int persistent;
+    int nwg;
+    uint32_t *order;        // [n*64]
+    uint32_t *pctl;         // [2]
 };
 
 // ---- encode ----------------------------------------------------------------
@@ -122,6 +129,7 @@ inline uint64_t scratch_words_for(uint64_t blob_len) { return 8ull * blob_len + 
 
 hipError_t upload_constants(const float *cmf, const uint16_t *band_start, const uint16_t *scan);
 void launch_index(const DecodeArgs &a, hipStream_t s);
+hipError_t occupancy_persist(int *per_cu);   // resident persistent-entropy workgroups per CU
 void launch_entropy(const DecodeArgs &a, hipStream_t s);
 void launch_recon(const DecodeArgs &a, hipStream_t s);
 void launch_block_single(const uint8_t *base, const uint64_t *off, const uint32_t *len, const uint32_t *qi,

@@ -51,6 +51,10 @@ def load():
         "mchip_d2h": (i, [vp, vp, vp, C.c_size_t]),
         "mchip_memset": (i, [vp, vp, i, C.c_size_t]),
         "mchip_d2d": (i, [vp, vp, vp, C.c_size_t]),
+        "mchip_decode_graph_create": (vp, [vp, vp, vp, vp, vp, i, vp, C.c_uint64]),
+        "mchip_graph_launch": (i, [vp]),
+        "mchip_graph_destroy": (None, [vp]),
+        "mchip_autotune_decode": (i, [vp, vp, vp, vp, vp, i, vp, C.c_uint64, C.c_char_p, C.c_size_t]),
         "mchip_store_create": (vp, [vp, C.c_uint64, vp, vp, vp, i, i, i]),
         "mchip_store_destroy": (None, [vp]),
         "mchip_store_device_bytes": (C.c_uint64, [vp]),
@@ -176,6 +180,36 @@ class Ctx:
         qs = np.ascontiguousarray(qs, np.float32)
         check(load().mchip_decode_chunks(self.h, C.c_void_p(d_blobs.ptr), _ptr(offs), _ptr(lens), _ptr(qs), n,
                                          C.c_void_p(d_out.ptr), out_stride))
+
+    def decode_graph(self, d_blobs, offs, lens, qs, d_out, out_stride=CHUNK_BYTES):
+        """HIP graph of one batched decode (replay with graph_launch)."""
+        offs = np.ascontiguousarray(offs, np.uint64)
+        lens = np.ascontiguousarray(lens, np.uint64)
+        qs = np.ascontiguousarray(qs, np.float32)
+        g = load().mchip_decode_graph_create(self.h, C.c_void_p(d_blobs.ptr), _ptr(offs), _ptr(lens), _ptr(qs),
+                                             len(offs), C.c_void_p(d_out.ptr), out_stride)
+        if not g:
+            raise MchipError(last_error())
+        return g
+
+    @staticmethod
+    def graph_launch(g):
+        check(load().mchip_graph_launch(g))
+
+    @staticmethod
+    def graph_destroy(g):
+        load().mchip_graph_destroy(g)
+
+    def autotune_decode(self, d_blobs, offs, lens, qs, d_out, out_stride=CHUNK_BYTES):
+        """Time the decode variants on this batch; keeps the fastest.  Returns the timings."""
+        import json as _json
+        offs = np.ascontiguousarray(offs, np.uint64)
+        lens = np.ascontiguousarray(lens, np.uint64)
+        qs = np.ascontiguousarray(qs, np.float32)
+        rep = C.create_string_buffer(4096)
+        check(load().mchip_autotune_decode(self.h, C.c_void_p(d_blobs.ptr), _ptr(offs), _ptr(lens), _ptr(qs),
+                                           len(offs), C.c_void_p(d_out.ptr), out_stride, rep, 4096))
+        return _json.loads(rep.value.decode())
 
     def decode_chunks_host(self, blobs):
         n = len(blobs)

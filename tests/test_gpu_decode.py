@@ -120,3 +120,83 @@ def test_device_resident_batch_equals_per_chunk(gpu_ctx):
     assert np.array_equal(one, got[-1])
     d_in.free()
     d_out.free()
+
+
+def _resident(gpu_ctx, blobs):
+    import mc_amd
+    offs = np.zeros(len(blobs), np.uint64)
+    tot = 0
+    for k, b in enumerate(blobs):
+        offs[k] = tot
+        tot += (len(b) + 255) // 256 * 256
+    buf = np.zeros(tot, np.uint8)
+    for k, b in enumerate(blobs):
+        buf[int(offs[k]):int(offs[k]) + len(b)] = np.frombuffer(b, np.uint8)
+    d_in = mc_amd.DeviceBuffer(tot)
+    gpu_ctx.h2d(d_in, buf)
+    qs = [np.frombuffer(b[:4], np.float32)[0] for b in blobs]
+    return d_in, offs, [len(b) for b in blobs], qs
+
+
+def test_persistent_batch_and_graph_replays_equal_eager(gpu_ctx):
+    """A mixed-quality batch through the persistent launch and through HIP
+    graph replays (with eager work between replays) equals the oracle."""
+    import mc_amd
+    specs = [dict(seed=s, air=a) for s, a in [(21, 0.3), (22, 0.7), (23, 0.5), (24, 0.9)]]
+    chunks = _chunks(specs)
+    blobs = [ob.encode_chunk(c, q, t) for c, (q, t) in zip(chunks, [(0.5, 1), (6.0, 16), (2.5, 8), (64.0, 128)])]
+    want = [ob.decode_chunk(b) for b in blobs]
+    d_in, offs, lens, qs = _resident(gpu_ctx, blobs)
+    d_out = mc_amd.DeviceBuffer(len(blobs) * mc_amd.CHUNK_BYTES)
+    d_tmp = mc_amd.DeviceBuffer(mc_amd.CHUNK_BYTES)
+    got = np.empty((len(blobs), 256, 256, 256), np.uint8)
+    for persistent in (1, 0):
+        mc_amd.set_tuning("persistent", persistent)
+        print(f"batch persistent={persistent}", flush=True)
+        gpu_ctx.decode_chunks(d_in, offs, lens, qs, d_out)
+        gpu_ctx.sync()
+        gpu_ctx.d2h(got, d_out)
+        for k in range(len(blobs)):
+            assert np.array_equal(got[k], want[k]), f"persistent={persistent} chunk {k}"
+    mc_amd.set_tuning("persistent", 1)
+    print("graph", flush=True)
+    g = gpu_ctx.decode_graph(d_in, offs, lens, qs, d_out)
+    try:
+        for rep in range(3):
+            # eager work between replays, then a replay over a scrambled output
+            gpu_ctx.decode_chunks(d_in, offs[1:2], lens[1:2], qs[1:2], d_tmp)
+            gpu_ctx.h2d(d_out, np.full(len(blobs) * mc_amd.CHUNK_BYTES, 7, np.uint8))
+            gpu_ctx.graph_launch(g)
+            gpu_ctx.sync()
+            gpu_ctx.d2h(got, d_out)
+            for k in range(len(blobs)):
+                assert np.array_equal(got[k], want[k]), f"replay {rep} chunk {k}"
+    finally:
+        gpu_ctx.graph_destroy(g)
+    d_in.free()
+    d_out.free()
+    d_tmp.free()
+
+
+def test_autotune_keeps_results_exact(gpu_ctx):
+    import mc_amd
+    specs = [dict(seed=s, air=a) for s, a in [(31, 0.4), (32, 0.6)]]
+    chunks = _chunks(specs)
+    blobs = [ob.encode_chunk(c, q, 0) for c, q in zip(chunks, [1.0, 16.0])]
+    d_in, offs, lens, qs = _resident(gpu_ctx, blobs)
+    d_out = mc_amd.DeviceBuffer(len(blobs) * mc_amd.CHUNK_BYTES)
+    try:
+        print("autotune", flush=True)
+        rep = gpu_ctx.autotune_decode(d_in, offs, lens, qs, d_out)
+        assert len(rep) >= 2 and all(r["ms"] > 0 for r in rep)
+        gpu_ctx.decode_chunks(d_in, offs, lens, qs, d_out)
+        gpu_ctx.sync()
+        got = np.empty((len(blobs), 256, 256, 256), np.uint8)
+        gpu_ctx.d2h(got, d_out)
+        for k in range(len(blobs)):
+            assert np.array_equal(got[k], ob.decode_chunk(blobs[k]))
+    finally:
+        for k, v in (("decoder", 2), ("persistent", 1), ("persist_min", 2), ("persist_wg", 0)):
+            mc_amd.set_tuning(k, v)
+        d_in.free()
+        d_out.free()
