@@ -96,6 +96,22 @@ def cpu_encode_and_baseline(chunks, specs, nthreads):
     return blobs, enc_gbs, dec_gbs, ns
 
 
+def cpu_verify_baseline(blobs):
+    """The oracle's blob hash (the reference's mc_chunk_compute_hash restated)
+    over the blobs on one core, repeated to about half a second: GB/s."""
+    import oracle_bind as ob
+    O = ob.oracle()
+    tot, reps = sum(b.size for b in blobs), 0
+    t0 = time.perf_counter()
+    while True:
+        for b in blobs:
+            O.orc_blob_hash(ob.p(b), b.size)
+        reps += 1
+        dt = time.perf_counter() - t0
+        if dt > 0.5:
+            return reps * tot / dt / 1e9
+
+
 def mixed_leg(nthreads):
     """Mixed-quality batch (mixed256: 256 chunks, 4 GiB, presets with max-error
     corrections): batched GPU encode checked byte for byte against the
@@ -341,9 +357,12 @@ def main():
               "the codec shards by chunk with no cross-GPU exchange", file=sys.stderr)
         sys.exit(2)
 
+    enc_batch = 64                              # chunks per encode launch sequence (library default)
     for kv in args.tune:
         k, v = kv.split("=")
         mc_amd.set_tuning(k, int(v))
+        if k == "enc_batch":
+            enc_batch = int(v)
     specs = workloads.WORKLOADS[args.workload]()
     man = json.load(open(os.path.join(ROOT, "tests", "golden", "chunk_manifest.json")))[args.workload]
     n = len(specs)
@@ -435,6 +454,33 @@ def main():
         ctx.sync()
         dtb = (time.perf_counter() - t0) / max(2, args.steps // 2)
         extra["batched_gbs"] = round(n * mc_amd.CHUNK_BYTES / dtb / 1e9, 3)
+    # ---- verify: blob hashes recomputed and checked against the stored ones,
+    # for the resident blobs (one wave per blob) and for host blobs (gathered
+    # into pinned slabs and uploaded), against the oracle's hash on one core;
+    # measured, like the batched decode, before the per-chunk streams exist
+    stored = np.array([int.from_bytes(b[4:12], "little") for b in blobs], np.uint64)
+    ctx.verify_blobs(d_blobs, offs, lens)
+    vreps = 5
+    t0 = time.perf_counter()
+    for _ in range(vreps):
+        nbad_v, hv, _ = ctx.verify_blobs(d_blobs, offs, lens)
+    dtv = (time.perf_counter() - t0) / vreps
+    host_blobs = [np.frombuffer(b, np.uint8) for b in blobs]
+    ctx.verify_blobs_host(host_blobs)
+    t0 = time.perf_counter()
+    for _ in range(2):
+        nbad_h, hh, _ = ctx.verify_blobs_host(host_blobs)
+    dth = (time.perf_counter() - t0) / 2
+    verify_ok = nbad_v == 0 and nbad_h == 0 and (hv == stored).all() and (hh == stored).all()
+    if not verify_ok:
+        log("[bench] ERROR: verify disagrees with the stored blob hashes")
+    cpu_v = cpu_verify_baseline(host_blobs)
+    verify = {"unit": "GB/s of blob bytes", "blobs": n, "blob_bytes": blob_bytes,
+              "device_resident": round(blob_bytes / dtv / 1e9, 3),
+              "host_blobs": round(blob_bytes / dth / 1e9, 3), "hashes_equal_stored": bool(verify_ok),
+              "cpu_oracle_1_core": round(cpu_v, 3)}
+    log(f"[bench] verify resident {verify['device_resident']} GB/s, host {verify['host_blobs']} GB/s, "
+        f"oracle 1 core {verify['cpu_oracle_1_core']} GB/s")
     ctxs = [ctx] + [mc_amd.Ctx(0) for _ in range(args.streams - 1)]
 
     def step():
@@ -515,8 +561,9 @@ def main():
                          "sample": f"oracle decode of {ns} of the {n} chunks, {nthreads} threads over blocks",
                          "encode_gbs": round(cpu_enc, 3)},
         "encode": {"value": round(enc_gbs, 3), "unit": "GB/s", "byte_identical": bad_enc == 0,
-                   "launch": "32 chunks per launch",
-                   "kernels_ms_per_chunk": {k: round(v[0] / max(1, v[1]) / 32.0, 4) for k, v in enc_k.items()}},
+                   "launch": f"{min(n, enc_batch)} chunks per launch",
+                   "kernels_ms_per_chunk": {k: round(v[0] / (esteps * n), 4) for k, v in enc_k.items()}},
+        "verify": verify,
     }
     rec.update(extra)
     d_blobs.free()

@@ -138,6 +138,28 @@ int mchip_read_regions_host(mchip_ctx *ctx, int nblk, const uint8_t *const *payl
                             int gy, int gx, int64_t dz, int64_t dy, int64_t dx, uint8_t *out,
                             uint64_t batch_stride);
 
+/* XXH64 of n byte ranges of device memory: range i is d_base + off[i],
+ * len[i] bytes, hashed with seed[i] (the published XXH64 algorithm, the
+ * mc_xxh64 of src/mc_archive.c); out[i] (host) receives the hash.  Any
+ * alignment.  Synchronous. */
+int mchip_xxh64_ranges(mchip_ctx *ctx, const uint8_t *d_base, const uint64_t *off, const uint64_t *len,
+                       const uint64_t *seed, int n, uint64_t *out);
+
+/* Chunk-blob verification of n device-resident blobs: blob i is d_base +
+ * off[i], len[i] bytes.  hash_out[i] receives the recomputed blob hash
+ * (mc_chunk_compute_hash, src/mc_archive.c:76-88) and bad[i] 1 when the
+ * blob's structure does not fit in len[i] or the recomputed hash differs from
+ * the one in its header (src/mc_archive.c:1093-1100), else 0.  Either output
+ * may be NULL.  Synchronous; returns the number of bad blobs, or -1. */
+long mchip_verify_blobs(mchip_ctx *ctx, const uint8_t *d_base, const uint64_t *off, const uint64_t *len, int n,
+                        uint64_t *hash_out, uint8_t *bad);
+
+/* The same for blobs in host memory (blobs[i], len[i] bytes): gathered by
+ * host threads into pinned slabs, uploaded and hashed while the next slab
+ * is gathered.  Synchronous; returns the number of bad blobs, or -1. */
+long mchip_verify_blobs_host(mchip_ctx *ctx, const uint8_t *const *blobs, const uint64_t *len, int n,
+                             uint64_t *hash_out, uint8_t *bad);
+
 /* Kernel timing: when enabled, every kernel launched on the context is
  * bracketed by HIP events on the context's stream; stats accumulate per
  * kernel name ("index", "entropy", "recon", ...). */

@@ -8,9 +8,12 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mchip.h"
@@ -212,6 +215,17 @@ struct TimedLaunch {
     hipEvent_t a, b;
 };
 
+// One half of the double-buffered host-blob verify: a pinned slab the host
+// threads gather blobs into, its device copy, and the per-blob offsets,
+// lengths and results.
+struct VerifySlab {
+    uint8_t *h = nullptr, *d = nullptr; size_t cap = 0;
+    uint8_t *hm = nullptr, *dm = nullptr; size_t capm = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+    int i0 = 0, i1 = 0;
+};
+
 struct mchip_ctx {
     hipStream_t stream = nullptr;
     // growable device buffers
@@ -249,6 +263,8 @@ struct mchip_ctx {
     void *h_desc = nullptr; size_t cap_h_desc = 0;
     hipEvent_t ev_desc = nullptr;
     bool desc_pending = false;
+    // host-blob verify
+    VerifySlab vslab[2];
     // timing
     bool timing = false;
     std::vector<TimedLaunch> pending;
@@ -493,6 +509,13 @@ void mchip_ctx_destroy(mchip_ctx *c) {
     if (c->h_res) (void)hipHostFree(c->h_res);
     if (c->h_crop) (void)hipHostFree(c->h_crop);
     if (c->ev_crop) (void)hipEventDestroy(c->ev_crop);
+    for (VerifySlab &v : c->vslab) {
+        if (v.h) (void)hipHostFree(v.h);
+        if (v.hm) (void)hipHostFree(v.hm);
+        if (v.d) (void)hipFree(v.d);
+        if (v.dm) (void)hipFree(v.dm);
+        if (v.ev) (void)hipEventDestroy(v.ev);
+    }
     for (hipEvent_t e : c->free_events) (void)hipEventDestroy(e);
     (void)hipEventDestroy(c->ev_desc);
     (void)hipStreamDestroy(c->stream);
@@ -996,6 +1019,152 @@ int mchip_read_regions_host(mchip_ctx *c, int nblk, const uint8_t *const *payloa
                                   box, hipMemcpyDeviceToHost, c->stream));
     }
     return mchip_ctx_sync(c);
+}
+
+int mchip_xxh64_ranges(mchip_ctx *c, const uint8_t *d_base, const uint64_t *off, const uint64_t *len,
+                       const uint64_t *seed, int n, uint64_t *out) {
+    if (!c) return fail("null context");
+    if (n <= 0) return 0;
+    if (!d_base || !off || !len || !seed || !out) return fail("null argument");
+    const size_t m = 8 * (size_t)n;
+    void *h = nullptr;
+    if (desc_buffer(c, 4 * m, &h) || ensure(c, &c->d_aux, &c->cap_aux, 4 * m)) return -1;
+    uint8_t *hb = (uint8_t *)h, *db = (uint8_t *)c->d_aux;
+    memcpy(hb, off, m);
+    memcpy(hb + m, len, m);
+    memcpy(hb + 2 * m, seed, m);
+    HIPCHK(hipMemcpyAsync(db, hb, 3 * m, hipMemcpyHostToDevice, c->stream));
+    timed(c, "xxh64", [&] {
+        launch_xxh64_ranges(d_base, (const uint64_t *)db, (const uint64_t *)(db + m), (const uint64_t *)(db + 2 * m),
+                            n, (uint64_t *)(db + 3 * m), c->stream);
+    });
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(hb + 3 * m, db + 3 * m, m, hipMemcpyDeviceToHost, c->stream));
+    if (mchip_ctx_sync(c)) return -1;
+    memcpy(out, hb + 3 * m, m);
+    return 0;
+}
+
+long mchip_verify_blobs(mchip_ctx *c, const uint8_t *d_base, const uint64_t *off, const uint64_t *len, int n,
+                        uint64_t *hash_out, uint8_t *bad) {
+    if (!c) return fail("null context");
+    if (n <= 0) return 0;
+    if (!d_base || !off || !len) return fail("null argument");
+    const size_t m = 8 * (size_t)n;
+    const size_t bytes = 3 * m + (size_t)n;
+    void *h = nullptr;
+    if (desc_buffer(c, bytes, &h) || ensure(c, &c->d_aux, &c->cap_aux, bytes)) return -1;
+    uint8_t *hb = (uint8_t *)h, *db = (uint8_t *)c->d_aux;
+    memcpy(hb, off, m);
+    memcpy(hb + m, len, m);
+    HIPCHK(hipMemcpyAsync(db, hb, 2 * m, hipMemcpyHostToDevice, c->stream));
+    timed(c, "blob_hash", [&] {
+        launch_blob_hash(d_base, (const uint64_t *)db, (const uint64_t *)(db + m), n, (uint64_t *)(db + 2 * m),
+                         db + 3 * m, c->stream);
+    });
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(hb + 2 * m, db + 2 * m, m + (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    if (mchip_ctx_sync(c)) return -1;
+    long nbad = 0;
+    for (int i = 0; i < n; ++i) nbad += hb[3 * m + i];
+    if (hash_out) memcpy(hash_out, hb + 2 * m, m);
+    if (bad) memcpy(bad, hb + 3 * m, (size_t)n);
+    return nbad;
+}
+
+long mchip_verify_blobs_host(mchip_ctx *c, const uint8_t *const *blobs, const uint64_t *len, int n,
+                             uint64_t *hash_out, uint8_t *bad) {
+    if (!c) return fail("null context");
+    if (n <= 0) return 0;
+    if (!blobs || !len) return fail("null argument");
+    auto r16 = [](uint64_t x) -> uint64_t { return (x + 15) & ~(uint64_t)15; };
+    uint64_t biggest = 0;
+    for (int i = 0; i < n; ++i) biggest = std::max(biggest, r16(len[i]));
+    const size_t cap = std::max<size_t>(256ull << 20, biggest);
+    const size_t capm = 25 * (size_t)n + 256;
+    for (VerifySlab &v : c->vslab) {
+        if (v.pending) { HIPCHK(hipEventSynchronize(v.ev)); v.pending = false; }
+        if (!v.ev) HIPCHK(hipEventCreateWithFlags(&v.ev, hipEventDisableTiming));
+        if (v.cap < cap) {
+            if (v.h) HIPCHK(hipHostFree(v.h));
+            if (v.d) HIPCHK(hipFree(v.d));
+            v.h = v.d = nullptr;
+            v.cap = 0;
+            HIPCHK(hipHostMalloc((void **)&v.h, cap, hipHostMallocDefault));
+            HIPCHK(hipMalloc((void **)&v.d, cap));
+            v.cap = cap;
+        }
+        if (v.capm < capm) {
+            if (v.hm) HIPCHK(hipHostFree(v.hm));
+            if (v.dm) HIPCHK(hipFree(v.dm));
+            v.hm = v.dm = nullptr;
+            v.capm = 0;
+            HIPCHK(hipHostMalloc((void **)&v.hm, capm, hipHostMallocDefault));
+            HIPCHK(hipMalloc((void **)&v.dm, capm));
+            v.capm = capm;
+        }
+    }
+    const int nthr = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    long nbad = 0;
+    auto harvest = [&](VerifySlab &v) -> int {
+        HIPCHK(hipEventSynchronize(v.ev));
+        v.pending = false;
+        const int k = v.i1 - v.i0;
+        const uint64_t *hh = (const uint64_t *)(v.hm + 16 * (size_t)k);
+        const uint8_t *hbad = v.hm + 24 * (size_t)k;
+        for (int j = 0; j < k; ++j) {
+            nbad += hbad[j];
+            if (hash_out) hash_out[v.i0 + j] = hh[j];
+            if (bad) bad[v.i0 + j] = hbad[j];
+        }
+        return 0;
+    };
+    int i = 0, turn = 0;
+    while (i < n) {
+        VerifySlab &v = c->vslab[turn++ & 1];
+        if (v.pending && harvest(v)) return -1;
+        // pack the next blobs into this slab
+        v.i0 = i;
+        std::vector<uint64_t> at;
+        size_t pos = 0;
+        while (i < n && pos + r16(len[i]) <= v.cap) { at.push_back(pos); pos += r16(len[i]); ++i; }
+        v.i1 = i;
+        const int k = v.i1 - v.i0;
+        // gather: pieces of at most 4 MiB, taken by host threads in turn
+        struct Piece { const uint8_t *src; uint8_t *dst; size_t n; };
+        std::vector<Piece> pieces;
+        for (int j = 0; j < k; ++j)
+            for (uint64_t o = 0; o < len[v.i0 + j]; o += 4ull << 20)
+                pieces.push_back({blobs[v.i0 + j] + o, v.h + at[j] + o,
+                                  (size_t)std::min<uint64_t>(4ull << 20, len[v.i0 + j] - o)});
+        std::atomic<size_t> next{0};
+        auto work = [&] {
+            for (size_t p; (p = next.fetch_add(1)) < pieces.size();) memcpy(pieces[p].dst, pieces[p].src, pieces[p].n);
+        };
+        std::vector<std::thread> pool;
+        const int t = (int)std::min<size_t>((size_t)nthr, pieces.size());
+        for (int w = 1; w < t; ++w) pool.emplace_back(work);
+        work();
+        for (std::thread &th : pool) th.join();
+        uint64_t *mo = (uint64_t *)v.hm, *ml = mo + k;
+        for (int j = 0; j < k; ++j) { mo[j] = at[j]; ml[j] = len[v.i0 + j]; }
+        HIPCHK(hipMemcpyAsync(v.d, v.h, pos, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(v.dm, v.hm, 16 * (size_t)k, hipMemcpyHostToDevice, c->stream));
+        timed(c, "blob_hash", [&] {
+            launch_blob_hash(v.d, (const uint64_t *)v.dm, (const uint64_t *)(v.dm + 8 * (size_t)k), k,
+                             (uint64_t *)(v.dm + 16 * (size_t)k), v.dm + 24 * (size_t)k, c->stream);
+        });
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(v.hm + 16 * (size_t)k, v.dm + 16 * (size_t)k, 9 * (size_t)k, hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipEventRecord(v.ev, c->stream));
+        v.pending = true;
+    }
+    for (int w = 0; w < 2; ++w) {
+        VerifySlab &v = c->vslab[turn++ & 1];
+        if (v.pending && harvest(v)) return -1;
+    }
+    return nbad;
 }
 
 int mchip_ctx_enable_timing(mchip_ctx *c, int on) {

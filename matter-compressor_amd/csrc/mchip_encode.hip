@@ -17,8 +17,8 @@
 //                    and payload offsets by prefix scans, the fraction map
 //                    (one lane, src/mc_codec.c:600-611), blob header;
 //   k_enc_copy       one workgroup per block: payload into its final place;
-//   k_enc_hash       4 lanes per chunk: the XXH64 chain over fraction map,
-//                    bitmap, lengths and payloads (src/mc_archive.c:76-88).
+//   k_enc_hash       one wave per chunk: the blob's XXH64 (mchip_hash.hip,
+//                    src/mc_archive.c:76-88).
 #include "mchip_codec.hpp"
 #include "mchip_device.hpp"
 #include "mchip_internal.hpp"
@@ -395,102 +395,6 @@ __global__ __launch_bounds__(64) void k_enc_copy(const EncChunkDesc *__restrict_
 }
 
 // ---------------------------------------------------------------------------
-// k_enc_hash: XXH64 chain, 4 lanes per chunk (one wave per chunk)
-// ---------------------------------------------------------------------------
-#define XP1 0x9E3779B185EBCA87ULL
-#define XP2 0xC2B2AE3D27D4EB4FULL
-#define XP3 0x165667B19E3779F9ULL
-#define XP4 0x85EBCA77C2B2AE63ULL
-#define XP5 0x27D4EB2F165667C5ULL
-__device__ __forceinline__ uint64_t rotl64(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
-__device__ __forceinline__ uint64_t xround(uint64_t acc, uint64_t in) { return rotl64(acc + in * XP2, 31) * XP1; }
-__device__ __forceinline__ uint64_t ld64u(const uint8_t *p) {
-    uint64_t v = 0;
-    #pragma unroll
-    for (int i = 7; i >= 0; --i) v = (v << 8) | p[i];
-    return v;
-}
-__device__ __forceinline__ uint32_t ld32u(const uint8_t *p) {
-    return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
-}
-// Unaligned 64-bit little-endian load from three aligned dwords.
-__device__ __forceinline__ uint64_t ld64a(const uint8_t *p) {
-    const uintptr_t a = (uintptr_t)p;
-    const uint32_t *w = (const uint32_t *)(a & ~(uintptr_t)3);
-    const uint32_t sh = (uint32_t)(a & 3) * 8;
-    const uint32_t w0 = w[0], w1 = w[1];
-    if (!sh) return (uint64_t)w0 | ((uint64_t)w1 << 32);
-    const uint32_t w2 = w[2];
-    const uint32_t lo = (w0 >> sh) | (w1 << (32 - sh));
-    const uint32_t hi = (w1 >> sh) | (w2 << (32 - sh));
-    return (uint64_t)lo | ((uint64_t)hi << 32);
-}
-
-// Finish an XXH64 after the stripe phase: p points at the unconsumed tail.
-__device__ uint64_t xxh_tail(uint64_t h, const uint8_t *p, const uint8_t *end) {
-    for (; p + 8 <= end; p += 8) { h ^= xround(0, ld64u(p)); h = rotl64(h, 27) * XP1 + XP4; }
-    if (p + 4 <= end) { h ^= (uint64_t)ld32u(p) * XP1; h = rotl64(h, 23) * XP2 + XP3; p += 4; }
-    for (; p < end; ++p) { h ^= (uint64_t)(*p) * XP5; h = rotl64(h, 11) * XP1; }
-    h ^= h >> 33; h *= XP2; h ^= h >> 29; h *= XP3; h ^= h >> 32;
-    return h;
-}
-
-// XXH64 of [p, p+len) computed by lanes 0..3 of the calling wave; every lane
-// of the wave must call it; the result is valid in lane 0.
-__device__ uint64_t xxh64_wave4(const uint8_t *p, uint64_t len, uint64_t seed) {
-    const int lane = threadIdx.x & 63;
-    uint64_t h = 0;
-    const uint64_t nstripe = len >= 32 ? len / 32 : 0;
-    uint64_t v = 0;
-    if (lane < 4) {
-        v = lane == 0 ? seed + XP1 + XP2 : lane == 1 ? seed + XP2 : lane == 2 ? seed : seed - XP1;
-        const uint8_t *q = p + 8 * lane;
-        uint64_t s = 0;
-        // eight stripes per step: the loads are independent of the chain
-        for (; s + 8 <= nstripe; s += 8) {
-            uint64_t in[8];
-            #pragma unroll
-            for (int u = 0; u < 8; ++u) in[u] = ld64a(q + 32 * (s + u));
-            #pragma unroll
-            for (int u = 0; u < 8; ++u) v = xround(v, in[u]);
-        }
-        for (; s < nstripe; ++s) v = xround(v, ld64a(q + 32 * s));
-    }
-    const uint64_t v1 = __shfl(v, 0, 64), v2 = __shfl(v, 1, 64), v3 = __shfl(v, 2, 64), v4 = __shfl(v, 3, 64);
-    if (lane == 0) {
-        if (len >= 32) {
-            h = rotl64(v1, 1) + rotl64(v2, 7) + rotl64(v3, 12) + rotl64(v4, 18);
-            const uint64_t vv[4] = {v1, v2, v3, v4};
-            for (int i = 0; i < 4; ++i) { h ^= xround(0, vv[i]); h = h * XP1 + XP4; }
-        } else {
-            h = seed + XP5;
-        }
-        h += len;
-        h = xxh_tail(h, p + 32 * nstripe, p + len);
-    }
-    return h;
-}
-
-__global__ __launch_bounds__(64) void k_enc_hash(const EncChunkDesc *__restrict__ cd,
-                                                 const EncChunkResult *__restrict__ res,
-                                                 uint8_t *__restrict__ out) {
-    const int c = blockIdx.x;
-    if (res[c].status != 0 || res[c].blob_len == 0) return;
-    uint8_t *o = out + cd[c].out_off;
-    const uint64_t total = res[c].blob_len, pay0 = res[c].pay0, np = res[c].npresent;
-    const uint32_t fml = (uint32_t)o[12] | ((uint32_t)o[13] << 8);
-    uint64_t h = xxh64_wave4(o + 14, fml, 0x6D636368756E6BULL);
-    h = __shfl(h, 0, 64);
-    h ^= xxh64_wave4(o + 14 + fml, 512, h);
-    h = __shfl(h, 0, 64);
-    h ^= xxh64_wave4(o + 14 + fml + 512, 2 * np, h);
-    h = __shfl(h, 0, 64);
-    h ^= xxh64_wave4(o + pay0, total - pay0, h);
-    if (threadIdx.x == 0)
-        for (int i = 0; i < 8; ++i) o[4 + i] = (uint8_t)(h >> (8 * i));
-}
-
-// ---------------------------------------------------------------------------
 void launch_enc_transform(const EncodeArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_enc_transform<false>, dim3((unsigned)a.n * kNVox), dim3(256), 0, s, a.in, a.cd, a.qtabs,
                        a.einfo, a.levels, a.corr, a.n);
@@ -513,9 +417,6 @@ void launch_enc_assemble(const EncodeArgs &a, hipStream_t s) {
 void launch_enc_copy(const EncodeArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_enc_copy, dim3((unsigned)a.n * kNVox), dim3(64), 0, s, a.cd, a.einfo, a.plen, a.dst,
                        a.slots, a.res, a.out);
-}
-void launch_enc_hash(const EncodeArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_enc_hash, dim3(a.n), dim3(64), 0, s, a.cd, a.res, a.out);
 }
 
 }  // namespace mchip

@@ -108,6 +108,12 @@ void launch_enc_entropy(const EncodeArgs &a, hipStream_t s);
 void launch_enc_assemble(const EncodeArgs &a, hipStream_t s);
 void launch_enc_copy(const EncodeArgs &a, hipStream_t s);
 void launch_enc_hash(const EncodeArgs &a, hipStream_t s);
+// XXH64 (mchip_hash.hip): blob hash + stored-hash check of n blobs at
+// base + off[i] (len[i] bytes), and plain XXH64 of n byte ranges.
+void launch_blob_hash(const uint8_t *base, const uint64_t *off, const uint64_t *len, int n, uint64_t *out,
+                      uint8_t *bad, hipStream_t s);
+void launch_xxh64_ranges(const uint8_t *base, const uint64_t *off, const uint64_t *len, const uint64_t *seed, int n,
+                         uint64_t *out, hipStream_t s);
 void launch_enc_blocks(const EncodeArgs &a, hipStream_t s);
 
 // ---- block jobs (region reads, decoded-block arena) --------------------------

@@ -789,12 +789,26 @@ int mc_archive_sample_boxes(mc_archive *a, int lod, uint64_t seed, int count, lo
 
 /* ------------------------------------------------------------------------
  * Verify (src/mc_archive.c:1072-1111)
+ *
+ * The index walk, its bounds checks and the per-blob structure check run
+ * here as in the reference; the blobs whose structure fits are hashed on the
+ * device (mchip_verify_blobs_host), and the CORRUPT lines are printed
+ * afterwards in walk order.
  * ------------------------------------------------------------------------ */
+typedef struct {
+    int lod, n2, n1, n0;
+    uint64_t co;
+} verify_ent;
+
 long mc_verify_archive(const uint8_t *arc, size_t len, int verbose) {
     if (!arc || len < MCH_HDR) return -1;
     const uint64_t NB = MCH_NODE_BYTES;
 #define NODE_OK(o) ((o) >= MCH_HDR && (o) <= len && len - (o) >= NB)
     long bad = 0, total = 0;
+    size_t n = 0, cap = 0;
+    verify_ent *ent = NULL;
+    const uint8_t **ptr = NULL;
+    uint64_t *blen = NULL;
     for (int lod = 0; lod < 8; ++lod) {
         uint64_t root; memcpy(&root, arc + MCH_ROOTOFF + (uint64_t)lod * 8, 8);
         if (!root) continue;
@@ -811,20 +825,49 @@ long mc_verify_archive(const uint8_t *arc, size_t len, int verbose) {
                     uint64_t co; memcpy(&co, arc + shard + (size_t)n0 * 8, 8);
                     if (co <= MCH_SLOT_ZERO) continue;
                     total++;
-                    uint64_t blen = mch_blob_len_bounded(arc, co, len);
-                    if (!blen) { bad++; continue; }
-                    uint64_t want; memcpy(&want, arc + co + 4, 8);
-                    if (want != mc_chunk_compute_hash(arc + co, blen)) {
-                        bad++;
-                        if (verbose)
-                            fprintf(stderr, "mc_verify: lod %d node %d/%d/%d chunk@%llu CORRUPT\n", lod, n2, n1, n0,
-                                    (unsigned long long)co);
+                    const uint64_t bl = mch_blob_len_bounded(arc, co, len);
+                    if (!bl) { bad++; continue; }
+                    if (n == cap) {
+                        cap = cap ? 2 * cap : 1024;
+                        verify_ent *e2 = realloc(ent, sizeof *ent * cap);
+                        const uint8_t **p2 = realloc(ptr, sizeof *ptr * cap);
+                        uint64_t *l2 = realloc(blen, sizeof *blen * cap);
+                        if (e2) ent = e2;
+                        if (p2) ptr = p2;
+                        if (l2) blen = l2;
+                        if (!e2 || !p2 || !l2) { bad = -1; goto out; }
                     }
+                    ent[n] = (verify_ent){lod, n2, n1, n0, co};
+                    ptr[n] = arc + co;
+                    blen[n] = bl;
+                    n++;
                 }
             }
         }
     }
 #undef NODE_OK
+    if (n) {
+        if (n > 0x7fffffff) { bad = -1; goto out; }
+        mchip_ctx *d = mch_dev();
+        uint8_t *flag = malloc(n);
+        if (!d || !flag) { free(flag); bad = -1; goto out; }
+        if (mchip_verify_blobs_host(d, ptr, blen, (int)n, NULL, flag) < 0) {
+            mch_dev_error("mc_verify_archive");
+            free(flag);
+            bad = -1;
+            goto out;
+        }
+        for (size_t i = 0; i < n; ++i) {
+            if (!flag[i]) continue;
+            bad++;
+            if (verbose)
+                fprintf(stderr, "mc_verify: lod %d node %d/%d/%d chunk@%llu CORRUPT\n", ent[i].lod, ent[i].n2,
+                        ent[i].n1, ent[i].n0, (unsigned long long)ent[i].co);
+        }
+        free(flag);
+    }
     if (verbose) fprintf(stderr, "mc_verify: %ld chunks checked, %ld corrupt\n", total, bad);
+out:
+    free(ent); free(ptr); free(blen);
     return bad;
 }

@@ -70,6 +70,9 @@ def load():
         "mchip_decode_blocks_host": (i, [vp, vp, vp, vp, i, u8p]),
         "mchip_encode_chunks": (i, [vp, u8p, C.c_uint64, vp, vp, vp, i, u8p, vp, vp, vp, vp]),
         "mchip_encode_chunks_host": (i, [vp, vp, vp, vp, vp, i, vp, vp, vp, vp]),
+        "mchip_xxh64_ranges": (i, [vp, vp, vp, vp, vp, i, vp]),
+        "mchip_verify_blobs": (C.c_long, [vp, vp, vp, vp, i, vp, vp]),
+        "mchip_verify_blobs_host": (C.c_long, [vp, vp, vp, i, vp, vp]),
         "mchip_ctx_enable_timing": (i, [vp, i]),
         "mchip_ctx_kernel_time": (i, [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)]),
         "mchip_ctx_reset_timing": (i, [vp]),
@@ -157,6 +160,36 @@ class Ctx:
 
     def d2h(self, arr, dbuf, offset=0):
         check(load().mchip_d2h(self.h, _ptr(arr), C.c_void_p(dbuf.ptr + offset), arr.nbytes))
+
+    def xxh64_ranges(self, buf, offs, lens, seeds):
+        """XXH64 of byte ranges buf[offs[i] : offs[i]+lens[i]] of a device buffer."""
+        offs, lens, seeds = (np.ascontiguousarray(x, np.uint64) for x in (offs, lens, seeds))
+        out = np.zeros(len(offs), np.uint64)
+        check(load().mchip_xxh64_ranges(self.h, C.c_void_p(buf.ptr), _ptr(offs), _ptr(lens), _ptr(seeds),
+                                        len(offs), _ptr(out)))
+        return out
+
+    def verify_blobs(self, buf, offs, lens):
+        """Blob hashes and bad flags of device-resident chunk blobs."""
+        offs, lens = (np.ascontiguousarray(x, np.uint64) for x in (offs, lens))
+        h = np.zeros(len(offs), np.uint64)
+        bad = np.zeros(len(offs), np.uint8)
+        n = load().mchip_verify_blobs(self.h, C.c_void_p(buf.ptr), _ptr(offs), _ptr(lens), len(offs), _ptr(h),
+                                      _ptr(bad))
+        if n < 0:
+            raise MchipError(last_error())
+        return n, h, bad
+
+    def verify_blobs_host(self, blobs):
+        """The same for host blobs (numpy uint8 arrays)."""
+        arrs = [np.ascontiguousarray(b, np.uint8) for b in blobs]
+        lens = np.array([a.size for a in arrs], np.uint64)
+        h = np.zeros(len(arrs), np.uint64)
+        bad = np.zeros(len(arrs), np.uint8)
+        n = load().mchip_verify_blobs_host(self.h, _ptr_array(arrs), _ptr(lens), len(arrs), _ptr(h), _ptr(bad))
+        if n < 0:
+            raise MchipError(last_error())
+        return n, h, bad
 
     def d2d(self, dst, src, nbytes, dst_off=0, src_off=0):
         check(load().mchip_d2d(self.h, C.c_void_p(dst.ptr + dst_off), C.c_void_p(src.ptr + src_off), nbytes))

@@ -1,5 +1,5 @@
 """The drop-in mc_archive API: file format, index, install, metadata, priors,
-verify, fraction map and sampler against the reference (CPU parts), and the
+blob hash, fraction map and sampler against the reference (CPU parts), and the
 GPU-backed decode / append / region entry points against the oracle."""
 import ctypes as C
 import json
@@ -71,7 +71,7 @@ def test_archive_file_identical_to_reference():
     assert open(p_ref, "rb").read() == open(p_ours, "rb").read()
 
 
-def test_archive_reopen_index_metadata_verify():
+def test_archive_reopen_index_metadata_hash():
     L = _api()
     blobs, _ = _blobs()
     path = _tmp("a.mca")
@@ -90,10 +90,7 @@ def test_archive_reopen_index_metadata_verify():
     assert a and [L.mc_archive_chunk_offset(a, *pl) for pl in PLACES] == offs
     assert not L.mc_archive_open_dims(path.encode(), 256, 512, 256, C.c_float(6.0))
     L.mc_archive_close(a)
-    data = np.fromfile(path, np.uint8)
-    assert L.mc_verify_archive(ob.p(data), data.size, 0) == 0
-    data[offs[0] + 2000] ^= 0x55
-    assert L.mc_verify_archive(ob.p(data), data.size, 0) == 1
+    # mc_verify_archive hashes on the device: tests/test_gpu_hash.py
     for b in blobs:
         arr = np.frombuffer(b, np.uint8)
         assert L.mc_chunk_compute_hash(ob.p(arr), arr.size) == int.from_bytes(b[4:12], "little")
