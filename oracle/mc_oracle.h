@@ -18,7 +18,10 @@
  *   - XXH64 against the published xxHash known answers;
  *   - the DCT against orthonormality / round trip closed forms and a
  *     float64 matrix product;
- *   - the range coder against encode->decode identity on random streams.
+ *   - the range coder against encode->decode identity on random streams;
+ *   - the LOD decimation and the quad render against the reference's
+ *     mc_build and mc_render_quad_lod compiled from its sources
+ *     (oracle/_ref/libmcref_render.so) and against closed-form cases.
  */
 #ifndef MC_ORACLE_H
 #define MC_ORACLE_H
@@ -101,6 +104,38 @@ void     orc_decode_chunks_mt(int n, const uint8_t *const *blobs, const uint64_t
 void     orc_encode_chunks_mt(int n, const uint8_t *chunks, const float *q, const int *tau,
                               uint8_t *const *out_blobs, const size_t *caps, size_t *out_lens,
                               int nthreads);
+
+/* ---- LOD decimation and quad render (mc_oracle_render.c) ----------------- */
+/* 2x decimation: mean of the nonzero voxels of each 2x2x2 cell, rounded half
+ * up; all-zero cells stay 0; odd dims round up (src/mc_archive.c:61-72).
+ * out holds ((dz+1)/2) x ((dy+1)/2) x ((dx+1)/2) voxels. */
+void     orc_decimate(const uint8_t *src, int dz, int dy, int dx, uint8_t *out);
+
+enum { ORC_FILTER_NEAREST = 0, ORC_FILTER_TRILINEAR = 1 };
+enum { ORC_COMP_NONE = 0, ORC_COMP_MIN = 1, ORC_COMP_MEAN = 2, ORC_COMP_MAX = 3 };
+
+/* One level of a pyramid: a dense z,y,x raster, or (dense == NULL) 16^3
+ * blocks at blk_data + blk_off[(bz*gby + by)*gbx + bx] (-1 = absent, reads 0)
+ * with gby = ceil(ny/16), gbx = ceil(nx/16). */
+typedef struct {
+    const uint8_t *dense;
+    const int64_t *blk_off;
+    const uint8_t *blk_data;
+    int nz, ny, nx;
+} orc_level;
+
+/* Point and unit normal of pixel (row, j) of a quad render. */
+void     orc_quad_point(const float *grid, int gw, int gh, float x0, float y0, float step, int row, int j,
+                        float *P, float *N);
+int      orc_pick_lod(int nlods, float vox_per_pixel);
+float    orc_quad_spacing(const float *grid, int gw, int gh);
+/* Ray steps of a [t0, t1] walk at dt (make_cfg, src/mc_render.c:827-835). */
+int      orc_render_steps(float t0, float t1, float dt);
+/* mc_render_quad_lod for filters NEAREST/TRILINEAR and composites
+ * NONE/MIN/MEAN/MAX; returns the level rendered, or -1. */
+int      orc_render_quad_lod(int nlods, const orc_level *levels, const float *grid, int gw, int gh, float x0,
+                             float y0, float step, int w, int h, int filter, int comp, float t0, float t1,
+                             float dt, uint8_t *out);
 
 #ifdef __cplusplus
 }

@@ -54,8 +54,59 @@ def oracle():
     O.orc_rc_decode_ops.argtypes = [C.c_int, vp, vp, vp, vp, C.c_size_t, vp]
     O.orc_decode_chunks_mt.argtypes = [C.c_int, vp, vp, vp, C.c_int]
     O.orc_encode_chunks_mt.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, C.c_int]
+    O.orc_decimate.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp]
+    O.orc_pick_lod.argtypes = [C.c_int, C.c_float]
+    O.orc_pick_lod.restype = C.c_int
+    O.orc_quad_spacing.argtypes = [vp, C.c_int, C.c_int]
+    O.orc_quad_spacing.restype = C.c_float
+    O.orc_render_steps.argtypes = [C.c_float, C.c_float, C.c_float]
+    O.orc_render_steps.restype = C.c_int
+    O.orc_quad_point.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, vp, vp]
+    O.orc_render_quad_lod.argtypes = [C.c_int, vp, vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                      C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, vp]
+    O.orc_render_quad_lod.restype = C.c_int
     _o = O
     return O
+
+
+class OrcLevel(C.Structure):
+    _fields_ = [("dense", C.c_void_p), ("blk_off", C.c_void_p), ("blk_data", C.c_void_p),
+                ("nz", C.c_int), ("ny", C.c_int), ("nx", C.c_int)]
+
+
+def decimate(vol):
+    dz, dy, dx = vol.shape
+    out = np.zeros(((dz + 1) // 2, (dy + 1) // 2, (dx + 1) // 2), np.uint8)
+    v = np.ascontiguousarray(vol)
+    oracle().orc_decimate(p(v), dz, dy, dx, p(out))
+    return out
+
+
+def pyramid(vol, nlods):
+    levels = [np.ascontiguousarray(vol)]
+    for _ in range(nlods - 1):
+        levels.append(decimate(levels[-1]))
+    return levels
+
+
+def render_quad_lod(levels, grid, x0, y0, step, w, h, filt=1, comp=2, t0=-4.0, t1=4.0, dt=1.0, blocks=None):
+    """Oracle quad render over dense levels (or, with blocks=[(off, data)] per
+    level, through 16^3 block tables).  Returns (image, level rendered)."""
+    arr = (OrcLevel * len(levels))()
+    keep = []
+    for i, v in enumerate(levels):
+        nz, ny, nx = v.shape
+        if blocks is None:
+            arr[i] = OrcLevel(v.ctypes.data, None, None, nz, ny, nx)
+        else:
+            off, data = blocks[i]
+            keep += [off, data]
+            arr[i] = OrcLevel(None, off.ctypes.data, data.ctypes.data, nz, ny, nx)
+    g = np.ascontiguousarray(grid, np.float32)
+    out = np.zeros((h, w), np.uint8)
+    L = oracle().orc_render_quad_lod(len(levels), arr, p(g), g.shape[1], g.shape[0], x0, y0, step, w, h, filt,
+                                     comp, t0, t1, dt, p(out))
+    return out, L
 
 
 def p(a):
@@ -101,6 +152,60 @@ def xxh64(data, seed=0):
 # ---- the reference build (only in the container that holds its sources) ----
 class _McBuf(C.Structure):
     _fields_ = [("p", C.c_void_p), ("len", C.c_size_t), ("cap", C.c_size_t)]
+
+
+REF_RENDER_SO = os.path.join(ROOT, "oracle", "_ref", "libmcref_render.so")
+_rr = None
+
+
+def ref_render_available():
+    return os.path.exists(REF_RENDER_SO)
+
+
+def ref_render():
+    """The reference's renderer and one-shot builder (oracle/ref_render_shim.c)."""
+    global _rr
+    if _rr is not None:
+        return _rr
+    R = C.CDLL(REF_RENDER_SO)
+    vp = C.c_void_p
+    R.refshim_render_quad_lod.argtypes = [C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                          C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                          C.c_int, vp, C.c_int]
+    R.refshim_render_quad_lod.restype = C.c_int
+    R.refshim_pick_lod.argtypes = [C.c_int, C.c_float]
+    R.refshim_pick_lod.restype = C.c_int
+    R.refshim_quad_spacing.argtypes = [vp, C.c_int, C.c_int]
+    R.refshim_quad_spacing.restype = C.c_float
+    R.refshim_build.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_size_t)]
+    R.refshim_build.restype = vp
+    R.refshim_free.argtypes = [vp]
+    _rr = R
+    return R
+
+
+def ref_render_quad_lod(levels, grid, x0, y0, step, w, h, filt=1, comp=2, t0=-4.0, t1=4.0, dt=1.0, blocked=0):
+    R = ref_render()
+    vols = (C.c_void_p * len(levels))(*[v.ctypes.data for v in levels])
+    dims = np.array([d for v in levels for d in v.shape], np.int32)
+    g = np.ascontiguousarray(grid, np.float32)
+    out = np.zeros((h, w), np.uint8)
+    rc = R.refshim_render_quad_lod(len(levels), vols, p(dims), p(g), g.shape[1], g.shape[0], x0, y0, step, w, h,
+                                   filt, comp, t0, t1, dt, blocked, p(out), 1)
+    assert rc == 0
+    return out
+
+
+def ref_build(vol, q):
+    """The reference's mc_build of a dense volume: archive bytes."""
+    R = ref_render()
+    n = C.c_size_t()
+    v = np.ascontiguousarray(vol)
+    ptr = R.refshim_build(p(v), v.shape[0], v.shape[1], v.shape[2], q, C.byref(n))
+    assert ptr
+    data = C.string_at(ptr, n.value)
+    R.refshim_free(ptr)
+    return data
 
 
 def ref_available():
