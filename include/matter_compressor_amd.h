@@ -132,6 +132,18 @@ void priors_load(const uint8_t *base);
  * after mc_serve_open are not seen. */
 typedef struct mc_serve mc_serve;
 mc_serve *mc_serve_open(mc_archive *a, int lod, uint64_t arena_bytes, int lanes);
+/* LODs lod0 .. lod0+nlods-1 in one store sharing one arena: box requests read
+ * lod0, render requests choose among the levels.  dims (optional) gives each
+ * level's voxel dims nz, ny, nx (the bounds its samplers use); NULL takes
+ * the archive's dims halved per level, rounded up. */
+mc_serve *mc_serve_open_levels(mc_archive *a, int lod0, int nlods, const int32_t *dims, uint64_t arena_bytes,
+                               int lanes);
+/* Quad render over the levels (mc_render_quad_lod, src/mc_render.c:2302-2314,
+ * filters trilinear/nearest, composites none/min/mean/max; see
+ * mchip_server_submit_render) into the w x h image at d_out (device).
+ * Pixels equal the reference renderer's fed the same decoded levels. */
+int64_t   mc_serve_render_quad(mc_serve *s, const float *grid, int gw, int gh, float x0, float y0, float step,
+                               int w, int h, int filter, int comp, float t0, float t1, float dt, void *d_out);
 int64_t   mc_serve_submit(mc_serve *s, const mc_box *boxes, int n, long dz, long dy, long dx,
                           void *d_out, size_t batch_stride);          /* ticket >= 0, or -1 */
 int       mc_serve_poll(mc_serve *s, int64_t ticket);                 /* 1 done, 0 pending, -1 error */

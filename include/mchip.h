@@ -160,6 +160,22 @@ long mchip_verify_blobs(mchip_ctx *ctx, const uint8_t *d_base, const uint64_t *o
 long mchip_verify_blobs_host(mchip_ctx *ctx, const uint8_t *const *blobs, const uint64_t *len, int n,
                              uint64_t *hash_out, uint8_t *bad);
 
+/* Quad render over an LOD pyramid of dense rasters in device memory
+ * (d_levels[l], dims[3l..3l+2] = nz, ny, nx): as mchip_server_submit_render,
+ * into the w x h image at d_out, on the context's stream (asynchronous).
+ * *lod_out (optional) receives the level rendered. */
+int mchip_render_quad(mchip_ctx *ctx, int nlods, const uint8_t *const *d_levels, const int32_t *dims,
+                      const float *grid, int gw, int gh, float x0, float y0, float step, int w, int h,
+                      int filter, int comp, float t0, float t1, float dt, uint8_t *d_out, int *lod_out);
+
+/* 2x LOD decimation (src/mc_archive.c:61-72): the mean of the nonzero voxels
+ * of each 2x2x2 cell, rounded half up, all-air cells 0.  mchip_decimate maps
+ * a dense dz*dy*dx raster to ((dz+1)/2) x ((dy+1)/2) x ((dx+1)/2);
+ * mchip_decimate_chunks maps a 2x2x2 group of 256^3 chunks (d_src[(z*2+y)*2+x],
+ * NULL for an absent all-air chunk) to one 256^3 chunk.  Asynchronous. */
+int mchip_decimate(mchip_ctx *ctx, const uint8_t *d_src, int dz, int dy, int dx, uint8_t *d_out);
+int mchip_decimate_chunks(mchip_ctx *ctx, const uint8_t *const *d_src, uint8_t *d_out);
+
 /* Kernel timing: when enabled, every kernel launched on the context is
  * bracketed by HIP events on the context's stream; stats accumulate per
  * kernel name ("index", "entropy", "recon", ...). */
@@ -201,6 +217,16 @@ typedef struct mchip_server mchip_server;
  * quality q[i]; len[i] == 0 marks an absent block (reads as zeros). */
 mchip_store *mchip_store_create(const uint8_t *payload, uint64_t nbytes, const uint64_t *off,
                                 const uint32_t *len, const float *q, int gz, int gy, int gx);
+/* A store over the levels of an LOD pyramid.  Level l is described by
+ * geom[9l .. 9l+8] = gz, gy, gx (its block grid), obz, oby, obx (the level
+ * block coordinates of grid block (0,0,0)) and nz, ny, nx (the level's voxel
+ * dims, the bounds its samplers use).  Blocks are indexed level after level,
+ * each level's grid in (z*gy + y)*gx + x order; off/len/q cover all levels.
+ * Box requests read level 0 (grid coordinates); render requests choose a
+ * level. */
+mchip_store *mchip_store_create_levels(const uint8_t *payload, uint64_t nbytes, int nlevels,
+                                       const int32_t *geom, const uint64_t *off, const uint32_t *len,
+                                       const float *q);
 void         mchip_store_destroy(mchip_store *st);
 uint64_t     mchip_store_device_bytes(const mchip_store *st);
 
@@ -214,6 +240,7 @@ typedef struct {
     uint64_t evictions;
     uint64_t deferred;        /* admissions postponed for lanes or arena slots */
     uint64_t arena_slots;
+    uint64_t renders;         /* render requests submitted */
 } mchip_server_stats;
 
 /* A serving loop over st with an arena of arena_bytes / 4096 decoded-block
@@ -231,6 +258,19 @@ mchip_server *mchip_server_create(mchip_store *st, uint64_t arena_bytes, int lan
  * outside the grid and in absent blocks are 0.  Returns a ticket >= 0, or -1. */
 int64_t      mchip_server_submit(mchip_server *sv, const int64_t *origins, int n, int64_t dz, int64_t dy,
                                  int64_t dx, uint8_t *d_out, uint64_t batch_stride);
+/* Submit a quad render (mc_render_quad_lod, src/mc_render.c:2302-2314): the
+ * gw x gh control grid (host, (z,y,x) per point in level-0 voxels, copied) is
+ * rendered over the store's levels into the w x h image at d_out (device).
+ * filter: 0 nearest, 1 trilinear; comp: 0 none, 1 min, 2 mean, 3 max (the
+ * reference's mc_filter / mc_comp values); the slab [t0, t1] along the
+ * normal is walked at dt.  The level is chosen and the points remapped as the
+ * reference does; the render reads only decoded blocks of that level, which
+ * are admitted and decoded like a box request's.  Returns a ticket, or -1. */
+int64_t      mchip_server_submit_render(mchip_server *sv, const float *grid, int gw, int gh, float x0,
+                                        float y0, float step, int w, int h, int filter, int comp, float t0,
+                                        float t1, float dt, uint8_t *d_out);
+/* Level a pending render request renders (valid until it is waited for). */
+int          mchip_server_render_level(mchip_server *sv, int64_t ticket);
 /* 1 once the request has completed, 0 while it is pending, -1 on error. */
 int          mchip_server_poll(mchip_server *sv, int64_t ticket);
 /* Block until the request completes and release its ticket; latency_ms

@@ -25,17 +25,19 @@ INCLUDE = os.path.join(ROOT, "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # -ffp-contract=off: every f32 multiply and add stays a separate IEEE
-# operation (bit-exact with the reference's deterministic build).
+# operation (bit-exact with the reference's deterministic build); f32
+# division and square root are correctly rounded, as on the host.
 # -packed-fp32-ops is disabled: no v_pk_*_f32 instructions are generated.
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
-    "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-fno-fast-math", "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", INCLUDE, "-I", CSRC, "-Wall", "-Wno-unused-function",
 ]
 C_FLAGS = ["-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall",
            "-Wno-unused-function", "-I", INCLUDE, "-D_GNU_SOURCE"]
 
-HIP_SOURCES = ["mchip_kernels.hip", "mchip_api.hip", "mchip_serve.hip"]
+HIP_SOURCES = ["mchip_kernels.hip", "mchip_api.hip", "mchip_serve.hip", "mchip_render.hip"]
 HOST_SOURCES = ["mc_host.c", "mc_host_region.c", "mc_host_serve.c"]
 
 

@@ -1167,6 +1167,51 @@ long mchip_verify_blobs_host(mchip_ctx *c, const uint8_t *const *blobs, const ui
     return nbad;
 }
 
+int mchip_render_quad(mchip_ctx *c, int nlods, const uint8_t *const *d_levels, const int32_t *dims,
+                      const float *grid, int gw, int gh, float x0, float y0, float step, int w, int h, int filter,
+                      int comp, float t0, float t1, float dt, uint8_t *d_out, int *lod_out) {
+    if (!c) return fail("null context");
+    if (nlods < 1 || nlods > 16 || !d_levels || !dims || !grid || gw < 1 || gh < 1 || w <= 0 || h <= 0 || !d_out ||
+        (int64_t)w * h > (1ll << 30))
+        return fail("mchip_render_quad: bad argument");
+    for (int l = 0; l < nlods; ++l)
+        if (!d_levels[l] || dims[3 * l] <= 0 || dims[3 * l + 1] <= 0 || dims[3 * l + 2] <= 0)
+            return fail("mchip_render_quad: level %d is empty", l);
+    const size_t gb = 12 * (size_t)gw * gh;
+    void *hg = nullptr;
+    if (desc_buffer(c, gb, &hg) || ensure(c, &c->d_aux, &c->cap_aux, gb)) return -1;
+    RenderDesc d;
+    const int L = render_describe(d, nlods, dims, grid, (const float *)c->d_aux, gw, gh, x0, y0, step, w, h, filter,
+                                  comp, t0, t1, dt);
+    if (L < 0) return fail("mchip_render_quad: filter %d / composite %d not supported", filter, comp);
+    memcpy(hg, grid, gb);
+    HIPCHK(hipMemcpyAsync(c->d_aux, hg, gb, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->ev_desc, c->stream));
+    c->desc_pending = true;
+    timed(c, "render", [&] { launch_render_dense(d, d_levels[L], d_out, c->stream); });
+    HIPCHK(hipGetLastError());
+    if (lod_out) *lod_out = L;
+    return 0;
+}
+
+int mchip_decimate(mchip_ctx *c, const uint8_t *d_src, int dz, int dy, int dx, uint8_t *d_out) {
+    if (!c) return fail("null context");
+    if (!d_src || !d_out || dz <= 0 || dy <= 0 || dx <= 0) return fail("mchip_decimate: bad argument");
+    timed(c, "decimate", [&] { launch_decimate(d_src, dz, dy, dx, d_out, c->stream); });
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mchip_decimate_chunks(mchip_ctx *c, const uint8_t *const *d_src, uint8_t *d_out) {
+    if (!c) return fail("null context");
+    if (!d_src || !d_out) return fail("mchip_decimate_chunks: bad argument");
+    ChunkGroup g;
+    for (int k = 0; k < 8; ++k) g.c[k] = d_src[k];
+    timed(c, "decimate", [&] { launch_decimate_group(g, d_out, c->stream); });
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 int mchip_ctx_enable_timing(mchip_ctx *c, int on) {
     if (!c) return fail("null context");
     c->timing = on != 0;

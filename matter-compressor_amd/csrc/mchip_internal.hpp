@@ -130,6 +130,51 @@ void launch_jobs(const uint8_t *base, const BlockJob *jobs, int n, const QTab *q
 void launch_crop(const uint8_t *arena, const int32_t *bmap, const int64_t *origin, int n, int gz, int gy, int gx,
                  int64_t dz, int64_t dy, int64_t dx, uint8_t *out, uint64_t batch_stride, hipStream_t s);
 
+// ---- quad render and LOD decimation (mchip_render.hip) -----------------------
+// One quad render at one level (src/mc_render.c:1981-2052, :2255-2314).
+struct RenderDesc {
+    const float *grid;      // gw*gh*3 control points (z,y,x), device
+    int gw, gh;
+    float x0, y0, step;     // pixel (i,j) maps to grid (x0 + j*step, y0 + i*step)
+    int w, h;
+    int filter, comp;       // 0 nearest / 1 trilinear; 0 none, 1 min, 2 mean, 3 max
+    float t0, dt;           // first ray offset and step along the normal (level units)
+    int nsteps;
+    int lod;                // rendered level; > 0: points remapped by P*a + b, clamped at 0
+    float a, b;
+    int nz, ny, nx;         // voxel dims of the rendered level
+};
+// A level read from decoded blocks: table[(bz-obz)*gy*gx + ...] = arena slot
+// of block (bz,by,bx), -1 absent; blocks outside the gz*gy*gx grid read 0.
+struct TableLevel {
+    const int32_t *table;
+    const uint8_t *arena;
+    int obz, oby, obx;
+    int gz, gy, gx;
+};
+// Host side: level choice, step count and descriptor of a render
+// (src/mc_render.c:827-835, :2222-2250, :2276-2288).  dims: nz,ny,nx per
+// level.  Returns the level, or -1 for an unsupported filter / composite.
+int render_describe(RenderDesc &d, int nlods, const int32_t *dims, const float *grid_host, const float *grid_dev,
+                    int gw, int gh, float x0, float y0, float step, int w, int h, int filter, int comp, float t0,
+                    float t1, float dt);
+void launch_render_dense(const RenderDesc &d, const uint8_t *vol, uint8_t *out, hipStream_t s);
+void launch_render_table(const RenderDesc &d, const TableLevel &t, uint8_t *out, hipStream_t s);
+// Marks (bits over the gz*gy*gx grid of t) every block the render reads.
+void launch_render_touch(const RenderDesc &d, const TableLevel &t, uint32_t *bits, hipStream_t s);
+// Compacts a bitmap of nbits into block indices; count must start at 0.
+void launch_bits_list(const uint32_t *bits, uint32_t nbits, uint32_t *list, uint32_t *count, hipStream_t s);
+void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
+// table[upd[i].x] = upd[i].y
+void launch_table_set(int32_t *table, const int2 *upd, int n, hipStream_t s);
+// 2x decimation of a dense dz*dy*dx raster (src/mc_archive.c:61-72), and of
+// a 2x2x2 group of 256^3 chunks (null: all air) into one 256^3 chunk.
+void launch_decimate(const uint8_t *src, int dz, int dy, int dx, uint8_t *out, hipStream_t s);
+struct ChunkGroup {
+    const uint8_t *c[8];    // (z,y,x) order: index (gz*2 + gy)*2 + gx
+};
+void launch_decimate_group(const ChunkGroup &g, uint8_t *out, hipStream_t s);
+
 // Scratch words a chunk needs: 8 per payload byte + 32 per block.
 inline uint64_t scratch_words_for(uint64_t blob_len) { return 8ull * blob_len + 32ull * 4096ull; }
 

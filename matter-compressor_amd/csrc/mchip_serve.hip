@@ -16,10 +16,20 @@
 //   decode   k_entropy_jobs + k_recon_jobs write the misses into their slots.
 //   assemble k_crop writes the boxes from the arena (absent / outside -> 0).
 //
+// Render requests (quad surfaces, src/mc_render.c:2302-2314) run a touch
+// pass first: the render's own pixel walk, recording every block it reads
+// at the chosen level; those blocks are then admitted like a box request's,
+// and the render kernel reads them from the arena through a per-block slot
+// table (each decode writes its block's slot there before the request's
+// decode event).
+//
 // Requests are admitted in arrival order as lanes (streams) and arena slots
 // free up, so requests that arrive while others run overlap on the device.
+// The store may hold several levels (an LOD pyramid); box requests read
+// level 0.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -36,11 +46,25 @@
 
 using namespace mchip;
 
+namespace {
+// One level of a store: a gz*gy*gx block grid whose block (0,0,0) is level
+// block (obz, oby, obx); the level has nz*ny*nx voxels; its blocks have the
+// store-wide indices base + (z*gy + y)*gx + x.
+struct StoreLevel {
+    int gz, gy, gx;
+    int obz, oby, obx;
+    int nz, ny, nx;
+    uint64_t base;
+};
+}  // namespace
+
 struct mchip_store {
     int device = 0;
     uint8_t *d_pay = nullptr;
     uint64_t nbytes = 0;
-    int gz = 0, gy = 0, gx = 0;
+    int gz = 0, gy = 0, gx = 0;           // level 0 (box requests)
+    std::vector<StoreLevel> lv;
+    uint64_t nblocks = 0;
     std::vector<uint64_t> off;
     std::vector<uint32_t> len;
     std::vector<uint32_t> qslot;
@@ -59,14 +83,22 @@ struct Lane {
     void *d_stage = nullptr; size_t cap_d = 0;          // device copy of the same
     void *d_scratch = nullptr; size_t cap_scratch = 0;  // coefficient lists of the jobs
     void *d_info = nullptr; size_t cap_info = 0;        // per-job block records
+    void *d_bits = nullptr; size_t cap_bits = 0;        // render touch bitmap + count
+    void *d_list = nullptr; size_t cap_list = 0;        // touched block list
     int64_t ticket = -1;                                // request in flight, -1: free
 };
 
 struct Req {
     int64_t ticket = 0;
+    int kind = 0;         // 0 boxes, 1 render
     std::vector<int64_t> origin;
     int n = 0;
     int64_t dz = 0, dy = 0, dx = 0;
+    // render: control grid (host copy), descriptor, blocks it reads
+    std::vector<float> grid;
+    RenderDesc rd{};
+    bool touched_ok = false;
+    std::vector<uint32_t> touched;
     uint8_t *d_out = nullptr;
     uint64_t stride = 0;
     double t_submit = 0.0, t_done = 0.0;
@@ -112,6 +144,7 @@ int grow_pinned(void **p, size_t *cap, size_t need, Graveyard &g) {
 struct mchip_server {
     mchip_store *st = nullptr;
     uint8_t *d_arena = nullptr;
+    int32_t *d_table = nullptr;      // per block: arena slot, written by the decoding request
     uint32_t nslots = 0;
     std::vector<int32_t> slot_of;    // per block: arena slot or -1
     std::vector<int32_t> blk_of;     // per slot: block or -1
@@ -168,71 +201,141 @@ struct mchip_server {
         r->pinned.clear();
     }
 
+    // The blocks of one admission attempt.
+    struct Admit {
+        std::vector<BlockJob> jobs;
+        std::vector<int2> upd;       // (block, slot) table entries of the jobs
+        std::vector<int> deps;       // lanes whose decode events the request waits for
+        uint64_t sw = 0, refs = 0, hits = 0;
+    };
+
+    // Arena slot of block id for request r: a resident block is pinned (and,
+    // while another request in flight still decodes it, that request's lane
+    // becomes a dependency); a missing block takes a slot by CLOCK and
+    // becomes a decode job.  -1: no slot can be freed now.
+    int32_t take(Req *r, uint64_t id, Admit &ad) {
+        const mchip_store &S = *st;
+        if (mark[id] == gen) return slot_of[id];
+        mark[id] = gen;
+        ad.refs++;
+        int32_t s = slot_of[id];
+        if (s >= 0) {
+            ad.hits++;
+            if (producer[s] >= 0 && producer[s] != r->ticket) {
+                auto it = reqs.find(producer[s]);
+                if (it != reqs.end() && it->second->lane >= 0) {
+                    const int dl = it->second->lane;
+                    bool have = false;
+                    for (int x : ad.deps) have |= x == dl;
+                    if (!have) ad.deps.push_back(dl);
+                }
+            }
+        } else {
+            s = evict_one();
+            if (s < 0) return -1;
+            blk_of[s] = (int32_t)id;
+            slot_of[id] = s;
+            producer[s] = r->ticket;
+            ad.jobs.push_back(BlockJob{S.off[id], ad.sw, S.len[id], S.qslot[id], (uint32_t)s, 0});
+            ad.upd.push_back(make_int2((int)id, s));
+            ad.sw += 8ull * S.len[id] + 32ull;
+        }
+        pins[s]++;
+        refbit[s] = 1;
+        r->pinned.push_back(s);
+        return s;
+    }
+
+    // Render touch pass on lane L: the blocks (level-local indices) the
+    // render reads, into r->touched.  Synchronous on the lane's stream.
+    int touch(Req *r, Lane &L) {
+        const StoreLevel &lv = st->lv[r->rd.lod];
+        const uint64_t nb = (uint64_t)lv.gz * lv.gy * lv.gx, words = (nb + 31) / 32;
+        const size_t gb = r->grid.size() * sizeof(float);
+        if (grow_pinned(&L.h_stage, &L.cap_h, std::max<size_t>(gb, 4 * nb + 16), grave) ||
+            grow_device(&L.d_stage, &L.cap_d, gb, grave) ||
+            grow_device(&L.d_bits, &L.cap_bits, 4 * words + 16, grave) ||
+            grow_device(&L.d_list, &L.cap_list, 4 * nb + 16, grave))
+            return -1;
+        memcpy(L.h_stage, r->grid.data(), gb);
+        MCHIP_RT_CHECK(hipMemcpyAsync(L.d_stage, L.h_stage, gb, hipMemcpyHostToDevice, L.stream));
+        uint32_t *bits = (uint32_t *)L.d_bits;
+        launch_fill_u32(bits, 0, words + 1, L.stream);
+        RenderDesc d = r->rd;
+        d.grid = (const float *)L.d_stage;
+        const TableLevel t{nullptr, nullptr, lv.obz, lv.oby, lv.obx, lv.gz, lv.gy, lv.gx};
+        launch_render_touch(d, t, bits, L.stream);
+        launch_bits_list(bits, (uint32_t)nb, (uint32_t *)L.d_list, bits + words, L.stream);
+        MCHIP_RT_CHECK(hipGetLastError());
+        MCHIP_RT_CHECK(hipMemcpyAsync(L.h_stage, bits + words, 4, hipMemcpyDeviceToHost, L.stream));
+        MCHIP_RT_CHECK(hipStreamSynchronize(L.stream));
+        const uint32_t n = *(const uint32_t *)L.h_stage;
+        if (n > nb) return rt_fail("render touch pass: %u blocks of %llu", n, (unsigned long long)nb);
+        if (n) {
+            MCHIP_RT_CHECK(hipMemcpyAsync(L.h_stage, L.d_list, 4ull * n, hipMemcpyDeviceToHost, L.stream));
+            MCHIP_RT_CHECK(hipStreamSynchronize(L.stream));
+        }
+        r->touched.assign((const uint32_t *)L.h_stage, (const uint32_t *)L.h_stage + n);
+        std::sort(r->touched.begin(), r->touched.end());
+        r->touched_ok = true;
+        return 0;
+    }
+
     // 1 admitted and issued, 0 not enough free slots now, -1 error (r->err).
     int try_admit(Req *r, int li) {
         Lane &L = lanes[li];
         const mchip_store &S = *st;
-        const int gzb = (int)((r->dz + 15) >> 4) + 1, gyb = (int)((r->dy + 15) >> 4) + 1,
-                  gxb = (int)((r->dx + 15) >> 4) + 1;
-        const size_t cells_per = (size_t)gzb * gyb * gxb, cells = cells_per * r->n;
+        if (r->kind == 1 && !r->touched_ok && touch(r, L)) {
+            r->err = std::string("render touch pass failed: ") + mchip_last_error();
+            return -1;
+        }
         if (++gen == 0) { std::fill(mark.begin(), mark.end(), 0u); gen = 1; }
-        std::vector<BlockJob> jobs;
-        std::vector<int32_t> bmap(cells, -1);
-        std::vector<int> deps;
-        uint64_t sw = 0, refs = 0, hits = 0;
+        Admit ad;
         bool ok = true;
-        for (int b = 0; b < r->n && ok; ++b) {
-            const int64_t z0 = r->origin[3 * b], y0 = r->origin[3 * b + 1], x0 = r->origin[3 * b + 2];
-            const int64_t bz0 = z0 >> 4, by0 = y0 >> 4, bx0 = x0 >> 4;
-            const int64_t nbz = ((z0 + r->dz - 1) >> 4) - bz0 + 1, nby = ((y0 + r->dy - 1) >> 4) - by0 + 1,
-                          nbx = ((x0 + r->dx - 1) >> 4) - bx0 + 1;
-            for (int64_t iz = 0; iz < nbz && ok; ++iz)
-                for (int64_t iy = 0; iy < nby && ok; ++iy)
-                    for (int64_t ix = 0; ix < nbx; ++ix) {
-                        const int64_t bz = bz0 + iz, by = by0 + iy, bx = bx0 + ix;
-                        if (bz < 0 || by < 0 || bx < 0 || bz >= S.gz || by >= S.gy || bx >= S.gx) continue;
-                        const int64_t id = (bz * S.gy + by) * S.gx + bx;
-                        if (!S.len[id]) continue;
-                        int32_t s;
-                        if (mark[id] == gen) {
-                            s = slot_of[id];
-                        } else {
-                            mark[id] = gen;
-                            refs++;
-                            s = slot_of[id];
-                            if (s >= 0) {
-                                hits++;
-                                if (producer[s] >= 0 && producer[s] != r->ticket) {
-                                    auto it = reqs.find(producer[s]);
-                                    if (it != reqs.end() && it->second->lane >= 0) {
-                                        const int dl = it->second->lane;
-                                        bool have = false;
-                                        for (int x : deps) have |= x == dl;
-                                        if (!have) deps.push_back(dl);
-                                    }
-                                }
-                            } else {
-                                s = evict_one();
-                                if (s < 0) { ok = false; break; }
-                                blk_of[s] = (int32_t)id;
-                                slot_of[id] = s;
-                                producer[s] = r->ticket;
-                                jobs.push_back(BlockJob{S.off[id], sw, S.len[id], S.qslot[id], (uint32_t)s, 0});
-                                sw += 8ull * S.len[id] + 32ull;
-                            }
-                            pins[s]++;
-                            refbit[s] = 1;
-                            r->pinned.push_back(s);
+        // boxes: the touched block grid of each box, as arena slots (-1 zeros)
+        int gzb = 0, gyb = 0, gxb = 0;
+        size_t cells_per = 0, cells = 0;
+        std::vector<int32_t> bmap;
+        if (r->kind == 0) {
+            gzb = (int)((r->dz + 15) >> 4) + 1; gyb = (int)((r->dy + 15) >> 4) + 1; gxb = (int)((r->dx + 15) >> 4) + 1;
+            cells_per = (size_t)gzb * gyb * gxb;
+            cells = cells_per * r->n;
+            bmap.assign(cells, -1);
+            for (int b = 0; b < r->n && ok; ++b) {
+                const int64_t z0 = r->origin[3 * b], y0 = r->origin[3 * b + 1], x0 = r->origin[3 * b + 2];
+                const int64_t bz0 = z0 >> 4, by0 = y0 >> 4, bx0 = x0 >> 4;
+                const int64_t nbz = ((z0 + r->dz - 1) >> 4) - bz0 + 1, nby = ((y0 + r->dy - 1) >> 4) - by0 + 1,
+                              nbx = ((x0 + r->dx - 1) >> 4) - bx0 + 1;
+                for (int64_t iz = 0; iz < nbz && ok; ++iz)
+                    for (int64_t iy = 0; iy < nby && ok; ++iy)
+                        for (int64_t ix = 0; ix < nbx; ++ix) {
+                            const int64_t bz = bz0 + iz, by = by0 + iy, bx = bx0 + ix;
+                            if (bz < 0 || by < 0 || bx < 0 || bz >= S.gz || by >= S.gy || bx >= S.gx) continue;
+                            const uint64_t id = (uint64_t)((bz * S.gy + by) * S.gx + bx);
+                            if (!S.len[id]) continue;
+                            const int32_t s = take(r, id, ad);
+                            if (s < 0) { ok = false; break; }
+                            bmap[(size_t)b * cells_per + (size_t)((iz * gyb + iy) * gxb + ix)] = s;
                         }
-                        bmap[(size_t)b * cells_per + (size_t)((iz * gyb + iy) * gxb + ix)] = s;
-                    }
+            }
+        } else {
+            const uint64_t base = S.lv[r->rd.lod].base;
+            for (uint32_t loc : r->touched) {
+                const uint64_t id = base + loc;
+                if (!S.len[id]) continue;
+                if (take(r, id, ad) < 0) { ok = false; break; }
+            }
         }
         if (!ok) {                         // not enough unpinned slots now
             rollback(r);
             return 0;
         }
-        const size_t jb = sizeof(BlockJob) * jobs.size(), ob = 24 * (size_t)r->n, mb = 4 * cells;
-        const size_t o_org = (jb + 255) & ~size_t(255), o_map = (o_org + ob + 255) & ~size_t(255);
+        // staging: [grid (render)] [jobs] [table entries] [origins | block map (boxes)]
+        auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+        const size_t gb = r->kind == 1 ? r->grid.size() * sizeof(float) : 0;
+        const size_t jb = sizeof(BlockJob) * ad.jobs.size(), ub = sizeof(int2) * ad.upd.size();
+        const size_t ob = r->kind == 0 ? 24 * (size_t)r->n : 0, mb = 4 * cells;
+        const size_t o_job = al(gb), o_upd = al(o_job + jb), o_org = al(o_upd + ub), o_map = al(o_org + ob);
         const size_t total = o_map + mb;
         auto fail_req = [&](void) {
             r->err = "serving request: device allocation or launch failed";
@@ -240,29 +343,41 @@ struct mchip_server {
             return -1;
         };
         if (grow_pinned(&L.h_stage, &L.cap_h, total, grave) || grow_device(&L.d_stage, &L.cap_d, total, grave) ||
-            grow_device(&L.d_scratch, &L.cap_scratch, sizeof(uint32_t) * (sw + 64), grave) ||
-            grow_device(&L.d_info, &L.cap_info, sizeof(uint2) * (jobs.size() + 1), grave))
+            grow_device(&L.d_scratch, &L.cap_scratch, sizeof(uint32_t) * (ad.sw + 64), grave) ||
+            grow_device(&L.d_info, &L.cap_info, sizeof(uint2) * (ad.jobs.size() + 1), grave))
             return fail_req();
         uint8_t *h = (uint8_t *)L.h_stage;
-        if (jb) memcpy(h, jobs.data(), jb);
-        memcpy(h + o_org, r->origin.data(), ob);
-        memcpy(h + o_map, bmap.data(), mb);
+        if (gb) memcpy(h, r->grid.data(), gb);
+        if (jb) memcpy(h + o_job, ad.jobs.data(), jb);
+        if (ub) memcpy(h + o_upd, ad.upd.data(), ub);
+        if (ob) memcpy(h + o_org, r->origin.data(), ob);
+        if (mb) memcpy(h + o_map, bmap.data(), mb);
         uint8_t *d = (uint8_t *)L.d_stage;
         if (hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, L.stream) != hipSuccess) return fail_req();
-        for (int dl : deps)
+        for (int dl : ad.deps)
             if (hipStreamWaitEvent(L.stream, lanes[dl].ev_dec, 0) != hipSuccess) return fail_req();
-        if (!jobs.empty())
-            launch_jobs(S.d_pay, (const BlockJob *)d, (int)jobs.size(), rt_qtabs(), (uint32_t *)L.d_scratch,
-                        (uint2 *)L.d_info, d_arena, rt_decoder(), L.stream);
+        if (!ad.jobs.empty()) {
+            launch_jobs(S.d_pay, (const BlockJob *)(d + o_job), (int)ad.jobs.size(), rt_qtabs(),
+                        (uint32_t *)L.d_scratch, (uint2 *)L.d_info, d_arena, rt_decoder(), L.stream);
+            launch_table_set(d_table, (const int2 *)(d + o_upd), (int)ad.upd.size(), L.stream);
+        }
         if (hipEventRecord(L.ev_dec, L.stream) != hipSuccess) return fail_req();
-        launch_crop(d_arena, (const int32_t *)(d + o_map), (const int64_t *)(d + o_org), r->n, gzb, gyb, gxb,
-                    r->dz, r->dy, r->dx, r->d_out, r->stride, L.stream);
+        if (r->kind == 0) {
+            launch_crop(d_arena, (const int32_t *)(d + o_map), (const int64_t *)(d + o_org), r->n, gzb, gyb, gxb,
+                        r->dz, r->dy, r->dx, r->d_out, r->stride, L.stream);
+        } else {
+            const StoreLevel &lv = S.lv[r->rd.lod];
+            RenderDesc rd = r->rd;
+            rd.grid = (const float *)d;
+            const TableLevel t{d_table + lv.base, d_arena, lv.obz, lv.oby, lv.obx, lv.gz, lv.gy, lv.gx};
+            launch_render_table(rd, t, r->d_out, L.stream);
+        }
         if (hipGetLastError() != hipSuccess || hipEventRecord(L.ev_done, L.stream) != hipSuccess) return fail_req();
         L.ticket = r->ticket;
         r->lane = li;
-        stats.block_refs += refs;
-        stats.hits += hits;
-        stats.misses += jobs.size();
+        stats.block_refs += ad.refs;
+        stats.hits += ad.hits;
+        stats.misses += ad.jobs.size();
         return 1;
     }
 
@@ -337,25 +452,42 @@ struct mchip_server {
 
 extern "C" {
 
-mchip_store *mchip_store_create(const uint8_t *payload, uint64_t nbytes, const uint64_t *off, const uint32_t *len,
-                                const float *q, int gz, int gy, int gx) {
-    if (gz <= 0 || gy <= 0 || gx <= 0 || !off || !len || !q || (nbytes && !payload)) {
+mchip_store *mchip_store_create_levels(const uint8_t *payload, uint64_t nbytes, int nlevels, const int32_t *geom,
+                                       const uint64_t *off, const uint32_t *len, const float *q) {
+    if (nlevels < 1 || nlevels > 16 || !geom || !off || !len || !q || (nbytes && !payload)) {
         rt_fail("mchip_store_create: bad argument");
         return nullptr;
     }
-    const size_t nb = (size_t)gz * gy * gx;
-    for (size_t i = 0; i < nb; ++i)
-        if (len[i] && (off[i] > nbytes || len[i] > nbytes - off[i])) {
-            rt_fail("mchip_store_create: block %zu lies outside the payload bytes", i);
+    auto *st = new mchip_store;
+    uint64_t nb = 0;
+    for (int l = 0; l < nlevels; ++l) {
+        const int32_t *g = geom + 9 * l;
+        if (g[0] <= 0 || g[1] <= 0 || g[2] <= 0 || g[6] <= 0 || g[7] <= 0 || g[8] <= 0) {
+            delete st;
+            rt_fail("mchip_store_create: level %d has an empty grid or volume", l);
             return nullptr;
         }
-    auto *st = new mchip_store;
-    st->gz = gz; st->gy = gy; st->gx = gx;
+        st->lv.push_back(StoreLevel{g[0], g[1], g[2], g[3], g[4], g[5], g[6], g[7], g[8], nb});
+        nb += (uint64_t)g[0] * g[1] * g[2];
+    }
+    if (nb > (1ull << 30)) {
+        delete st;
+        rt_fail("mchip_store_create: %llu blocks", (unsigned long long)nb);
+        return nullptr;
+    }
+    for (uint64_t i = 0; i < nb; ++i)
+        if (len[i] && (off[i] > nbytes || len[i] > nbytes - off[i])) {
+            delete st;
+            rt_fail("mchip_store_create: block %llu lies outside the payload bytes", (unsigned long long)i);
+            return nullptr;
+        }
+    st->gz = st->lv[0].gz; st->gy = st->lv[0].gy; st->gx = st->lv[0].gx;
+    st->nblocks = nb;
     st->off.assign(off, off + nb);
     st->len.assign(len, len + nb);
     st->qslot.assign(nb, 0);
     std::map<uint32_t, uint32_t> qs;              // q bit pattern -> table slot
-    for (size_t i = 0; i < nb; ++i) {
+    for (uint64_t i = 0; i < nb; ++i) {
         if (!len[i]) continue;
         uint32_t key;
         memcpy(&key, &q[i], 4);
@@ -378,6 +510,16 @@ mchip_store *mchip_store_create(const uint8_t *payload, uint64_t nbytes, const u
     }
     st->nbytes = nbytes;
     return st;
+}
+
+mchip_store *mchip_store_create(const uint8_t *payload, uint64_t nbytes, const uint64_t *off, const uint32_t *len,
+                                const float *q, int gz, int gy, int gx) {
+    if (gz <= 0 || gy <= 0 || gx <= 0 || gz > (1 << 26) || gy > (1 << 26) || gx > (1 << 26)) {
+        rt_fail("mchip_store_create: bad argument");
+        return nullptr;
+    }
+    const int32_t geom[9] = {gz, gy, gx, 0, 0, 0, 16 * gz, 16 * gy, 16 * gx};
+    return mchip_store_create_levels(payload, nbytes, 1, geom, off, len, q);
 }
 
 void mchip_store_destroy(mchip_store *st) {
@@ -406,7 +548,15 @@ mchip_server *mchip_server_create(mchip_store *st, uint64_t arena_bytes, int lan
         delete sv;
         return nullptr;
     }
-    const size_t nb = (size_t)st->gz * st->gy * st->gx;
+    const size_t nb = (size_t)st->nblocks;
+    if (hipMalloc((void **)&sv->d_table, 4 * nb + 4) != hipSuccess ||
+        hipMemset(sv->d_table, 0xFF, 4 * nb + 4) != hipSuccess) {
+        rt_fail("mchip_server_create: slot table allocation failed");
+        (void)hipFree(sv->d_arena);
+        if (sv->d_table) (void)hipFree(sv->d_table);
+        delete sv;
+        return nullptr;
+    }
     sv->slot_of.assign(nb, -1);
     sv->mark.assign(nb, 0u);
     sv->blk_of.assign(ns, -1);
@@ -455,6 +605,41 @@ int64_t mchip_server_submit(mchip_server *sv, const int64_t *origins, int n, int
     sv->stats.boxes += (uint64_t)n;
     sv->cv_work.notify_one();
     return r->ticket;
+}
+
+int64_t mchip_server_submit_render(mchip_server *sv, const float *grid, int gw, int gh, float x0, float y0,
+                                   float step, int w, int h, int filter, int comp, float t0, float t1, float dt,
+                                   uint8_t *d_out) {
+    if (!sv || !grid || gw < 1 || gh < 1 || w <= 0 || h <= 0 || !d_out || (int64_t)w * h > (1ll << 30))
+        return rt_fail("mchip_server_submit_render: bad argument");
+    auto *r = new Req;
+    r->kind = 1;
+    r->grid.assign(grid, grid + 3 * (size_t)gw * gh);
+    std::vector<int32_t> dims;
+    for (const StoreLevel &l : sv->st->lv) { dims.push_back(l.nz); dims.push_back(l.ny); dims.push_back(l.nx); }
+    if (render_describe(r->rd, (int)sv->st->lv.size(), dims.data(), r->grid.data(), nullptr, gw, gh, x0, y0, step,
+                        w, h, filter, comp, t0, t1, dt) < 0) {
+        delete r;
+        return rt_fail("mchip_server_submit_render: filter %d / composite %d not supported", filter, comp);
+    }
+    r->d_out = d_out;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    r->ticket = sv->next_ticket++;
+    r->t_submit = now_ms();
+    sv->reqs[r->ticket] = r;
+    sv->pending.push_back(r);
+    sv->stats.requests++;
+    sv->stats.renders++;
+    sv->cv_work.notify_one();
+    return r->ticket;
+}
+
+int mchip_server_render_level(mchip_server *sv, int64_t ticket) {
+    if (!sv) return rt_fail("null server");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(ticket);
+    if (it == sv->reqs.end() || it->second->kind != 1) return rt_fail("not a pending render ticket");
+    return it->second->rd.lod;
 }
 
 int mchip_server_poll(mchip_server *sv, int64_t ticket) {
@@ -506,23 +691,32 @@ int mchip_server_preload(mchip_server *sv, uint64_t *loaded) {
     auto flush = [&]() -> int {
         if (jobs.empty()) return 0;
         uint64_t sw = 0;
-        for (BlockJob &j : jobs) { j.scratch_off = sw; sw += 8ull * j.len + 32ull; }
-        const size_t jb = sizeof(BlockJob) * jobs.size();
-        if (grow_pinned(&L.h_stage, &L.cap_h, jb, sv->grave) || grow_device(&L.d_stage, &L.cap_d, jb, sv->grave) ||
+        std::vector<int2> upd;
+        for (BlockJob &j : jobs) {
+            j.scratch_off = sw;
+            sw += 8ull * j.len + 32ull;
+            upd.push_back(make_int2(sv->blk_of[j.slot], (int)j.slot));
+        }
+        const size_t jb = sizeof(BlockJob) * jobs.size(), o_upd = (jb + 255) & ~size_t(255);
+        const size_t total = o_upd + sizeof(int2) * upd.size();
+        if (grow_pinned(&L.h_stage, &L.cap_h, total, sv->grave) ||
+            grow_device(&L.d_stage, &L.cap_d, total, sv->grave) ||
             grow_device(&L.d_scratch, &L.cap_scratch, sizeof(uint32_t) * (sw + 64), sv->grave) ||
             grow_device(&L.d_info, &L.cap_info, sizeof(uint2) * (jobs.size() + 1), sv->grave))
             return -1;
         memcpy(L.h_stage, jobs.data(), jb);
-        MCHIP_RT_CHECK(hipMemcpyAsync(L.d_stage, L.h_stage, jb, hipMemcpyHostToDevice, L.stream));
+        memcpy((uint8_t *)L.h_stage + o_upd, upd.data(), sizeof(int2) * upd.size());
+        MCHIP_RT_CHECK(hipMemcpyAsync(L.d_stage, L.h_stage, total, hipMemcpyHostToDevice, L.stream));
         launch_jobs(S.d_pay, (const BlockJob *)L.d_stage, (int)jobs.size(), rt_qtabs(), (uint32_t *)L.d_scratch,
                     (uint2 *)L.d_info, sv->d_arena, rt_decoder(), L.stream);
+        launch_table_set(sv->d_table, (const int2 *)((uint8_t *)L.d_stage + o_upd), (int)upd.size(), L.stream);
         MCHIP_RT_CHECK(hipGetLastError());
         MCHIP_RT_CHECK(hipStreamSynchronize(L.stream));
         n += jobs.size();
         jobs.clear();
         return 0;
     };
-    const size_t nb = (size_t)S.gz * S.gy * S.gx;
+    const size_t nb = (size_t)S.nblocks;
     for (size_t id = 0; id < nb; ++id) {
         if (!S.len[id] || sv->slot_of[id] >= 0) continue;
         const int s = sv->evict_one();
@@ -567,6 +761,8 @@ void mchip_server_destroy(mchip_server *sv) {
         if (L.d_stage) (void)hipFree(L.d_stage);
         if (L.d_scratch) (void)hipFree(L.d_scratch);
         if (L.d_info) (void)hipFree(L.d_info);
+        if (L.d_bits) (void)hipFree(L.d_bits);
+        if (L.d_list) (void)hipFree(L.d_list);
         if (L.ev_dec) (void)hipEventDestroy(L.ev_dec);
         if (L.ev_done) (void)hipEventDestroy(L.ev_done);
         if (L.stream) (void)hipStreamDestroy(L.stream);
@@ -575,6 +771,7 @@ void mchip_server_destroy(mchip_server *sv) {
     for (void *p : sv->grave.dev) (void)hipFree(p);
     for (void *p : sv->grave.host) (void)hipHostFree(p);
     if (sv->d_arena) (void)hipFree(sv->d_arena);
+    if (sv->d_table) (void)hipFree(sv->d_table);
     delete sv;
 }
 

@@ -1,14 +1,16 @@
 /*
  * mc_host_serve.c -- HBM-resident serving of one LOD of an archive.
  *
- * mc_serve_open walks the LOD's chunk index (src/mc_archive.h:133-145),
- * uploads every present chunk blob once and gives the device the payload
- * range and quality of every block of the chunks' bounding box; requests
- * of boxes then run through the device serving loop (mchip_server_*), which
- * decodes each block at most once while it stays in the HBM arena.  The
- * voxels of every box equal mc_archive_read_regions' for the same box
- * (absent chunks and blocks, and everything outside the LOD's chunks, read
- * as 0).  Chunks appended after mc_serve_open are not seen.
+ * mc_serve_open walks the chunk index of one LOD, mc_serve_open_levels of
+ * several consecutive LODs (src/mc_archive.h:133-145); every present chunk
+ * blob is uploaded once and the device is given the payload range and
+ * quality of every block of each level's chunk bounding box.  Requests
+ * (boxes of the first level, or quad renders over the levels) then run
+ * through the device serving loop (mchip_server_*), which decodes each block
+ * at most once while it stays in the HBM arena.  The voxels of every box
+ * equal mc_archive_read_regions' for the same box (absent chunks and blocks,
+ * and everything outside the LOD's chunks, read as 0).  Chunks appended
+ * after the open are not seen.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -51,87 +53,111 @@ static void walk(const uint8_t *arc, uint64_t end, uint64_t node, int lv, int cz
     }
 }
 
-mc_serve *mc_serve_open(mc_archive *a, int lod, uint64_t arena_bytes, int lanes) {
-    if (!a || lod < 0 || lod > 7 || lanes <= 0) return NULL;
+/* Per-level chunk list, bounding box and block grid. */
+typedef struct {
+    chunk_list cl;
+    int lo[3], hi[3];
+    int gz, gy, gx;
+    uint64_t base;                   /* store index of the level's first block */
+} serve_level;
+
+mc_serve *mc_serve_open_levels(mc_archive *a, int lod0, int nlods, const int32_t *dims, uint64_t arena_bytes,
+                               int lanes) {
+    if (!a || lod0 < 0 || nlods < 1 || lod0 + nlods > 8 || lanes <= 0) return NULL;
     const uint8_t *arc = a->base;
     const uint64_t end = atomic_load_explicit(&a->cursor, memory_order_acquire);
-    uint64_t root;
-    memcpy(&root, arc + MCH_ROOTOFF + (uint64_t)lod * 8, 8);
-    chunk_list cl = {0};
-    walk(arc, end, root, 2, 0, 0, 0, &cl);
-    int lo[3] = {4096, 4096, 4096}, hi[3] = {-1, -1, -1};
-    for (int i = 0; i < cl.n; ++i) {
-        const int c[3] = {cl.v[i].cz, cl.v[i].cy, cl.v[i].cx};
-        for (int k = 0; k < 3; ++k) {
-            if (c[k] < lo[k]) lo[k] = c[k];
-            if (c[k] > hi[k]) hi[k] = c[k];
+    serve_level lv[8];
+    memset(lv, 0, sizeof lv);
+    uint64_t nb = 0;
+    int ok = 1;
+    for (int l = 0; l < nlods; ++l) {
+        serve_level *L = &lv[l];
+        uint64_t root;
+        memcpy(&root, arc + MCH_ROOTOFF + (uint64_t)(lod0 + l) * 8, 8);
+        walk(arc, end, root, 2, 0, 0, 0, &L->cl);
+        for (int k = 0; k < 3; ++k) { L->lo[k] = 4096; L->hi[k] = -1; }
+        for (int i = 0; i < L->cl.n; ++i) {
+            const int c[3] = {L->cl.v[i].cz, L->cl.v[i].cy, L->cl.v[i].cx};
+            for (int k = 0; k < 3; ++k) {
+                if (c[k] < L->lo[k]) L->lo[k] = c[k];
+                if (c[k] > L->hi[k]) L->hi[k] = c[k];
+            }
         }
+        if (L->cl.n == 0) for (int k = 0; k < 3; ++k) { L->lo[k] = 0; L->hi[k] = 0; }
+        L->gz = 16 * (L->hi[0] - L->lo[0] + 1);
+        L->gy = 16 * (L->hi[1] - L->lo[1] + 1);
+        L->gx = 16 * (L->hi[2] - L->lo[2] + 1);
+        L->base = nb;
+        nb += (uint64_t)L->gz * L->gy * L->gx;
     }
-    if (cl.n == 0) { lo[0] = lo[1] = lo[2] = 0; hi[0] = hi[1] = hi[2] = 0; }
-    const int gz = 16 * (hi[0] - lo[0] + 1), gy = 16 * (hi[1] - lo[1] + 1), gx = 16 * (hi[2] - lo[2] + 1);
-    const uint64_t nb = (uint64_t)gz * gy * gx;
-    if (nb > (1ull << 28)) {                          /* the block grid is the chunks' bounding box */
-        fprintf(stderr, "mc_serve_open: chunk bounding box of %llu blocks is too large\n", (unsigned long long)nb);
-        free(cl.v);
-        return NULL;
+    if (nb > (1ull << 28)) {                          /* each level's grid is its chunks' bounding box */
+        fprintf(stderr, "mc_serve_open: chunk bounding boxes of %llu blocks are too large\n", (unsigned long long)nb);
+        ok = 0;
     }
-    uint64_t *off = calloc(nb, sizeof *off);
-    uint32_t *len = calloc(nb, sizeof *len);
-    float *q = calloc(nb, sizeof *q);
+    int nchunks = 0;
+    for (int l = 0; l < nlods; ++l) nchunks += lv[l].cl.n;
+    uint64_t *off = ok ? calloc(nb, sizeof *off) : NULL;
+    uint32_t *len = ok ? calloc(nb, sizeof *len) : NULL;
+    float *q = ok ? calloc(nb, sizeof *q) : NULL;
     uint64_t total = 0;
-    uint64_t *blob_at = malloc(sizeof *blob_at * (cl.n ? cl.n : 1));
-    uint64_t *blob_len = malloc(sizeof *blob_len * (cl.n ? cl.n : 1));
+    uint64_t *blob_at = malloc(sizeof *blob_at * (nchunks ? nchunks : 1));
+    uint64_t *blob_len = malloc(sizeof *blob_len * (nchunks ? nchunks : 1));
     uint8_t *stage = NULL;
     mc_serve *s = NULL;
-    if (!off || !len || !q || !blob_at || !blob_len) goto done;
+    if (!ok || !off || !len || !q || !blob_at || !blob_len) goto done;
     /* Pass 1: per-block payload ranges, each bounded by the archive's live
      * end like mc_archive_read_regions (src/mc_archive.c:728-740); a chunk's
-     * upload extent is the end of its last valid payload. */
+     * upload extent is the end of its last valid payload.  Pass 2 fills the
+     * per-block records and the upload. */
     for (int pass = 0; pass < 2; ++pass) {
-        for (int i = 0; i < cl.n; ++i) {
-            const uint64_t co = cl.v[i].co;
-            uint64_t ext = 0;
-            if (co + MCH_BLOB_HDR <= end) {
-                float cq;
-                memcpy(&cq, arc + co, 4);
-                uint16_t fml;
-                memcpy(&fml, arc + co + 12, 2);
-                const uint64_t bm = co + MCH_BLOB_HDR + (uint64_t)fml;
-                if (bm + 512 <= end) {
-                    int np = 0;
-                    for (int k = 0; k < 512; ++k) np += __builtin_popcount(arc[bm + k]);
-                    const uint64_t lens = bm + 512;
-                    uint64_t pay = lens + 2ull * np;
-                    if (pay <= end) {
-                        int rank = 0;
-                        for (int bi = 0; bi < 4096; ++bi) {
-                            if (!((arc[bm + (bi >> 3)] >> (bi & 7)) & 1)) continue;
-                            uint16_t l;
-                            memcpy(&l, arc + lens + 2ull * rank, 2);
-                            rank++;
-                            if (pay + l <= end && l) {
-                                if (pay + l - co > ext) ext = pay + l - co;
-                                if (pass) {
-                                    const int bz = 16 * (cl.v[i].cz - lo[0]) + (bi >> 8);
-                                    const int by = 16 * (cl.v[i].cy - lo[1]) + ((bi >> 4) & 15);
-                                    const int bx = 16 * (cl.v[i].cx - lo[2]) + (bi & 15);
-                                    const uint64_t id = ((uint64_t)bz * gy + by) * gx + bx;
-                                    off[id] = blob_at[i] + (pay - co);
-                                    len[id] = l;
-                                    q[id] = cq;
+        int ci = 0;
+        for (int l = 0; l < nlods; ++l) {
+            const serve_level *L = &lv[l];
+            for (int i = 0; i < L->cl.n; ++i, ++ci) {
+                const uint64_t co = L->cl.v[i].co;
+                uint64_t ext = 0;
+                if (co + MCH_BLOB_HDR <= end) {
+                    float cq;
+                    memcpy(&cq, arc + co, 4);
+                    uint16_t fml;
+                    memcpy(&fml, arc + co + 12, 2);
+                    const uint64_t bm = co + MCH_BLOB_HDR + (uint64_t)fml;
+                    if (bm + 512 <= end) {
+                        int np = 0;
+                        for (int k = 0; k < 512; ++k) np += __builtin_popcount(arc[bm + k]);
+                        const uint64_t lens = bm + 512;
+                        uint64_t pay = lens + 2ull * np;
+                        if (pay <= end) {
+                            int rank = 0;
+                            for (int bi = 0; bi < 4096; ++bi) {
+                                if (!((arc[bm + (bi >> 3)] >> (bi & 7)) & 1)) continue;
+                                uint16_t bl;
+                                memcpy(&bl, arc + lens + 2ull * rank, 2);
+                                rank++;
+                                if (pay + bl <= end && bl) {
+                                    if (pay + bl - co > ext) ext = pay + bl - co;
+                                    if (pass) {
+                                        const int bz = 16 * (L->cl.v[i].cz - L->lo[0]) + (bi >> 8);
+                                        const int by = 16 * (L->cl.v[i].cy - L->lo[1]) + ((bi >> 4) & 15);
+                                        const int bx = 16 * (L->cl.v[i].cx - L->lo[2]) + (bi & 15);
+                                        const uint64_t id = L->base + ((uint64_t)bz * L->gy + by) * L->gx + bx;
+                                        off[id] = blob_at[ci] + (pay - co);
+                                        len[id] = bl;
+                                        q[id] = cq;
+                                    }
                                 }
+                                pay += bl;
                             }
-                            pay += l;
                         }
                     }
                 }
-            }
-            if (!pass) {
-                blob_len[i] = ext;
-                blob_at[i] = total;
-                total += (ext + 15) & ~15ull;
-            } else if (blob_len[i]) {
-                memcpy(stage + blob_at[i], arc + co, blob_len[i]);
+                if (!pass) {
+                    blob_len[ci] = ext;
+                    blob_at[ci] = total;
+                    total += (ext + 15) & ~15ull;
+                } else if (blob_len[ci]) {
+                    memcpy(stage + blob_at[ci], arc + co, blob_len[ci]);
+                }
             }
         }
         if (!pass) {
@@ -139,17 +165,41 @@ mc_serve *mc_serve_open(mc_archive *a, int lod, uint64_t arena_bytes, int lanes)
             if (!stage) goto done;
         }
     }
+    int32_t geom[9 * 8];
+    for (int l = 0; l < nlods; ++l) {
+        const serve_level *L = &lv[l];
+        int32_t *g = geom + 9 * l;
+        g[0] = L->gz; g[1] = L->gy; g[2] = L->gx;
+        g[3] = 16 * L->lo[0]; g[4] = 16 * L->lo[1]; g[5] = 16 * L->lo[2];
+        const int sh = lod0 + l;
+        g[6] = dims ? dims[3 * l] : (int32_t)(((int64_t)a->nz + (1 << sh) - 1) >> sh);
+        g[7] = dims ? dims[3 * l + 1] : (int32_t)(((int64_t)a->ny + (1 << sh) - 1) >> sh);
+        g[8] = dims ? dims[3 * l + 2] : (int32_t)(((int64_t)a->nx + (1 << sh) - 1) >> sh);
+    }
     s = calloc(1, sizeof *s);
     if (!s) goto done;
-    s->oz = 256L * lo[0]; s->oy = 256L * lo[1]; s->ox = 256L * lo[2];
+    s->oz = 256L * lv[0].lo[0]; s->oy = 256L * lv[0].lo[1]; s->ox = 256L * lv[0].lo[2];
     if (!mch_dev()) { free(s); s = NULL; goto done; }
-    s->st = mchip_store_create(stage, total, off, len, q, gz, gy, gx);
+    s->st = mchip_store_create_levels(stage, total, nlods, geom, off, len, q);
     if (!s->st) { mch_dev_error("mc_serve_open"); free(s); s = NULL; goto done; }
     s->sv = mchip_server_create(s->st, arena_bytes, lanes);
     if (!s->sv) { mch_dev_error("mc_serve_open"); mchip_store_destroy(s->st); free(s); s = NULL; goto done; }
 done:
-    free(stage); free(off); free(len); free(q); free(blob_at); free(blob_len); free(cl.v);
+    free(stage); free(off); free(len); free(q); free(blob_at); free(blob_len);
+    for (int l = 0; l < nlods; ++l) free(lv[l].cl.v);
     return s;
+}
+
+mc_serve *mc_serve_open(mc_archive *a, int lod, uint64_t arena_bytes, int lanes) {
+    if (!a || lod < 0 || lod > 7) return NULL;
+    return mc_serve_open_levels(a, lod, 1, NULL, arena_bytes, lanes);
+}
+
+int64_t mc_serve_render_quad(mc_serve *s, const float *grid, int gw, int gh, float x0, float y0, float step, int w,
+                             int h, int filter, int comp, float t0, float t1, float dt, void *d_out) {
+    if (!s) return -1;
+    return mchip_server_submit_render(s->sv, grid, gw, gh, x0, y0, step, w, h, filter, comp, t0, t1, dt,
+                                      (uint8_t *)d_out);
 }
 
 int64_t mc_serve_submit(mc_serve *s, const mc_box *boxes, int n, long dz, long dy, long dx, void *d_out,

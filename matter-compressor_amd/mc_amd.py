@@ -71,6 +71,14 @@ def load():
         "mchip_encode_chunks": (i, [vp, u8p, C.c_uint64, vp, vp, vp, i, u8p, vp, vp, vp, vp]),
         "mchip_encode_chunks_host": (i, [vp, vp, vp, vp, vp, i, vp, vp, vp, vp]),
         "mchip_xxh64_ranges": (i, [vp, vp, vp, vp, vp, i, vp]),
+        "mchip_render_quad": (i, [vp, i, vp, vp, vp, i, i, C.c_float, C.c_float, C.c_float, i, i, i, i,
+                                  C.c_float, C.c_float, C.c_float, vp, C.POINTER(C.c_int)]),
+        "mchip_decimate": (i, [vp, vp, i, i, i, vp]),
+        "mchip_decimate_chunks": (i, [vp, vp, vp]),
+        "mchip_store_create_levels": (vp, [vp, C.c_uint64, i, vp, vp, vp, vp]),
+        "mchip_server_submit_render": (C.c_int64, [vp, vp, i, i, C.c_float, C.c_float, C.c_float, i, i, i, i,
+                                                   C.c_float, C.c_float, C.c_float, vp]),
+        "mchip_server_render_level": (i, [vp, C.c_int64]),
         "mchip_verify_blobs": (C.c_long, [vp, vp, vp, vp, i, vp, vp]),
         "mchip_verify_blobs_host": (C.c_long, [vp, vp, vp, i, vp, vp]),
         "mchip_ctx_enable_timing": (i, [vp, i]),
@@ -190,6 +198,25 @@ class Ctx:
         if n < 0:
             raise MchipError(last_error())
         return n, h, bad
+
+    def render_quad(self, levels, dims, grid, x0, y0, step, w, h, d_out, filt=1, comp=2, t0=-4.0, t1=4.0,
+                    dt=1.0):
+        """Quad render over dense device levels (DeviceBuffers); returns the level rendered."""
+        g = np.ascontiguousarray(grid, np.float32)
+        lv = (C.c_void_p * len(levels))(*[b.ptr for b in levels])
+        dd = np.ascontiguousarray(dims, np.int32).reshape(-1)
+        lod = C.c_int(-1)
+        check(load().mchip_render_quad(self.h, len(levels), lv, _ptr(dd), _ptr(g), g.shape[1], g.shape[0], x0, y0,
+                                       step, w, h, filt, comp, t0, t1, dt, C.c_void_p(d_out.ptr), C.byref(lod)))
+        return lod.value
+
+    def decimate(self, src, dz, dy, dx, dst):
+        check(load().mchip_decimate(self.h, C.c_void_p(src.ptr), dz, dy, dx, C.c_void_p(dst.ptr)))
+
+    def decimate_chunks(self, srcs, dst):
+        """srcs: 8 (buffer, byte offset) pairs or None, (z, y, x) order."""
+        arr = (C.c_void_p * 8)(*[None if x is None else x[0].ptr + x[1] for x in srcs])
+        check(load().mchip_decimate_chunks(self.h, arr, C.c_void_p(dst.ptr)))
 
     def d2d(self, dst, src, nbytes, dst_off=0, src_off=0):
         check(load().mchip_d2d(self.h, C.c_void_p(dst.ptr + dst_off), C.c_void_p(src.ptr + src_off), nbytes))
@@ -398,6 +425,8 @@ def api():
         "mc_archive_read_region": (None, [vp, i, lng, lng, lng, lng, lng, lng, vp, C.c_size_t, C.c_size_t, i]),
         "mc_archive_read_regions": (None, [vp, i, vp, i, lng, lng, lng, vp, C.c_size_t, i]),
         "mc_serve_open": (vp, [vp, i, u64, i]),
+        "mc_serve_open_levels": (vp, [vp, i, i, vp, u64, i]),
+        "mc_serve_render_quad": (C.c_int64, [vp, vp, i, i, f, f, f, i, i, i, i, f, f, f, vp]),
         "mc_serve_submit": (C.c_int64, [vp, vp, i, lng, lng, lng, vp, C.c_size_t]),
         "mc_serve_poll": (i, [vp, C.c_int64]),
         "mc_serve_wait": (i, [vp, C.c_int64, C.POINTER(C.c_double)]),
@@ -421,7 +450,7 @@ def api():
 
 
 SERVER_STATS = ["requests", "completed", "boxes", "block_refs", "hits", "misses", "evictions", "deferred",
-                "arena_slots"]
+                "arena_slots", "renders"]
 
 
 def server_stats(server_handle):
@@ -432,13 +461,29 @@ def server_stats(server_handle):
 
 
 class Serve:
-    """mc_serve_*: HBM-resident box-batch serving of one archive LOD."""
+    """mc_serve_*: HBM-resident serving of an archive LOD (box batches), or of
+    consecutive LODs (boxes of the first, quad renders over all)."""
 
-    def __init__(self, archive, lod=0, arena_bytes=0, lanes=8):
+    def __init__(self, archive, lod=0, arena_bytes=0, lanes=8, nlods=None, dims=None):
         L = api()
-        self.h = L.mc_serve_open(archive, lod, arena_bytes, lanes)
+        if nlods is None:
+            self.h = L.mc_serve_open(archive, lod, arena_bytes, lanes)
+        else:
+            d = None if dims is None else np.ascontiguousarray(dims, np.int32).reshape(-1)
+            self.h = L.mc_serve_open_levels(archive, lod, nlods, None if d is None else _ptr(d), arena_bytes, lanes)
         if not self.h:
             raise MchipError(f"mc_serve_open failed: {last_error()}")
+
+    def render(self, grid, x0, y0, step, w, h, d_out, filt=1, comp=2, t0=-4.0, t1=4.0, dt=1.0, offset=0):
+        g = np.ascontiguousarray(grid, np.float32)
+        t = api().mc_serve_render_quad(self.h, _ptr(g), g.shape[1], g.shape[0], x0, y0, step, w, h, filt, comp,
+                                       t0, t1, dt, C.c_void_p(d_out.ptr + offset))
+        if t < 0:
+            raise MchipError(f"mc_serve_render_quad failed: {last_error()}")
+        return t
+
+    def render_level(self, ticket):
+        return load().mchip_server_render_level(api().mc_serve_device(self.h), ticket)
 
     def submit(self, origins, dz, dy, dx, d_out, batch_stride=None, offset=0):
         o = np.ascontiguousarray(origins, np.int64).reshape(-1, 3)
