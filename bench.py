@@ -188,7 +188,239 @@ def mixed_leg(nthreads):
     return res
 
 
-def serve_leg(rates, nthreads, lanes=8, arena_bytes=0, boxes_per_batch=64, edge=128, warm_batches=100):
+def build_pyramid(L, ctx, path, nthreads):
+    """The 2048^3 volume (volume2048_q6, LOD 0) and its decimations LOD 1-3
+    (2x2x2 chunk groups decimated on the device, src/mc_archive.c:61-72), each
+    chunk GPU-encoded at q=6 and installed in one archive.  Returns the
+    archive and, per level, {(cz, cy, cx): blob}."""
+    CH = mc_amd.CHUNK_BYTES
+    specs = workloads.WORKLOADS["volume2048_q6"]()
+    by = {x["coord"]: x for x in specs}
+    a = L.mc_archive_open_dims(path.encode(), 2048, 2048, 2048, C.c_float(6.0))
+    if not a:
+        raise RuntimeError("mc_archive_open_dims failed")
+    blobs = [dict() for _ in range(4)]
+
+    def install(lod, coord, bl):
+        if bl is None or len(bl) == 0:
+            return
+        if L.mc_archive_append_chunk_compressed(a, lod, *coord, bl, len(bl)) != 0:
+            raise RuntimeError("install failed")
+        blobs[lod][coord] = bl
+
+    cube = [(z, y, x) for z in range(2) for y in range(2) for x in range(2)]
+    d_grp = mc_amd.DeviceBuffer(8 * CH)
+    levels = [None, mc_amd.DeviceBuffer(64 * CH), mc_amd.DeviceBuffer(8 * CH), mc_amd.DeviceBuffer(CH)]
+    # LOD 0: 64 groups of 2x2x2 chunks, synthesized on the host, encoded, and
+    # decimated into LOD 1 on the device
+    for gz in range(4):
+        for gy in range(4):
+            for gx in range(4):
+                grp = [by[(2 * gz + z, 2 * gy + y, 2 * gx + x)] for z, y, x in cube]
+                ch = mc_amd.synth_chunks([x["seed"] for x in grp], [x["air"] for x in grp],
+                                         coords=[x["coord"] for x in grp], nthreads=nthreads)
+                for x, bl in zip(grp, ctx.encode_chunks_host(ch, [6.0] * 8)):
+                    install(0, x["coord"], bl)
+                ctx.h2d(d_grp, ch)
+                ctx.decimate_chunks([(d_grp, k * CH) for k in range(8)], levels[1],
+                                    dst_off=((gz * 4 + gy) * 4 + gx) * CH)
+                del ch
+    # LOD 1..3: encode the level on the device, decimate it into the next
+    d_enc = mc_amd.DeviceBuffer(64 * (20 << 20))
+    for lod, n in ((1, 4), (2, 2), (3, 1)):
+        cnt = n ** 3
+        offs = np.arange(cnt, dtype=np.uint64) * (20 << 20)
+        lens, st = ctx.encode_chunks(levels[lod], [6.0] * cnt, [0] * cnt, d_enc, offs, np.full(cnt, 20 << 20))
+        assert (st == 0).all(), f"LOD {lod} encode status {st}"
+        for i in range(cnt):
+            coord = (i // (n * n), (i // n) % n, i % n)
+            if lens[i]:
+                b = np.empty(int(lens[i]), np.uint8)
+                ctx.d2h(b, d_enc, offset=int(offs[i]))
+                install(lod, coord, b.tobytes())
+        if lod < 3:
+            m = n // 2
+            for hz in range(m):
+                for hy in range(m):
+                    for hx in range(m):
+                        src = [(levels[lod], (((2 * hz + z) * n + 2 * hy + y) * n + 2 * hx + x) * CH)
+                               for z, y, x in cube]
+                        ctx.decimate_chunks(src, levels[lod + 1], dst_off=((hz * m + hy) * m + hx) * CH)
+    ctx.sync()
+    for b in [d_grp, d_enc] + levels[1:]:
+        b.free()
+    return a, blobs
+
+
+def dense_levels(ctx, blobs):
+    """Decoded levels (GPU decode of every installed chunk) as dense rasters."""
+    out = []
+    for lod, lb in enumerate(blobs):
+        n = 2048 >> lod
+        v = np.zeros((n, n, n), np.uint8)
+        items = sorted(lb.items())
+        for i in range(0, len(items), 32):
+            part = items[i:i + 32]
+            dec = ctx.decode_chunks_host([b for _, b in part])
+            for (cz, cy, cx), d in zip([c for c, _ in part], dec):
+                v[256 * cz:256 * cz + 256, 256 * cy:256 * cy + 256, 256 * cx:256 * cx + 256] = d
+        out.append(v)
+    return out
+
+
+def render_frame(k, w=1024, h=1024):
+    """Render request k: a plane through a point of the volume interior with
+    a normal uniform on the sphere (seed 5000 + k), as a 129 x 129 control
+    grid of 64-voxel spacing; zoom 1, 2, 4, 8 voxels per pixel in turn (the
+    step makes mc_render_pick_lod choose LOD 0, 1, 2, 3); the 1024^2 window
+    is centred on the point."""
+    rng = np.random.default_rng(5000 + k)
+    n = rng.normal(size=3)
+    n /= np.linalg.norm(n)
+    u = np.cross(n, rng.normal(size=3))
+    u /= np.linalg.norm(u)
+    v = np.cross(n, u)
+    o = rng.uniform(512, 1536, 3)
+    g = 129
+    sp = 64.0
+    j = np.arange(g, dtype=np.float64) - (g - 1) / 2
+    grid = (o[None, None, :] + sp * (j[:, None, None] * v[None, None, :] + j[None, :, None] * u[None, None, :]))
+    grid = grid.astype(np.float32)
+    zoom = [1, 2, 4, 8][k % 4]
+    step = zoom * 1.001 / sp
+    x0 = (g - 1) / 2 - w * step / 2
+    return grid, float(x0), float(x0), float(step), w, h, zoom
+
+
+def render_leg(a, blobs, ctx, nthreads, arena_bytes=4 << 30, lanes=8, boxes_per_batch=64, edge=128,
+               n_render=40, n_batch=80, lam_render=10.0, lam_batch=20.0):
+    """Mixed render + box-batch serving over LODs 0-3 of the 2048^3 volume with
+    a 4 GiB decoded-block arena: 1024^2 quad renders (trilinear, 9-step mean
+    along the normal, t0 = -4, t1 = 4) and batches of 64 boxes of 128^3
+    arrive as independent Poisson processes; renders are checked bit for bit
+    against the oracle's renderer fed the same decoded levels."""
+    import threading
+    import oracle_bind as ob
+    L = mc_amd.api()
+    sv = mc_amd.Serve(a, 0, arena_bytes=arena_bytes, lanes=lanes, nlods=4)
+    fb = [mc_amd.DeviceBuffer(1024 * 1024) for _ in range(2 * lanes)]
+    bb = [mc_amd.DeviceBuffer(boxes_per_batch * edge ** 3) for _ in range(2 * lanes)]
+
+    def one(k, buf):
+        grid, x0, y0, step, w, h, _ = render_frame(k)
+        t = sv.render(grid, x0, y0, step, w, h, buf)
+        lod = sv.render_level(t)
+        return lod, sv.wait(t)
+
+    # cold (blocks decoded by the request) and resident frames, one at a time
+    cold, warm, lods, frames = [], [], [], []
+    for k in range(4):
+        lod, ms = one(k, fb[k])
+        cold.append(round(ms, 3))
+        lods.append(lod)
+        img = np.empty((1024, 1024), np.uint8)
+        ctx.d2h(img, fb[k])
+        frames.append(img)
+    for k in range(4):
+        warm.append(round(one(k, fb[k])[1], 3))
+    log(f"[bench] render frames LOD {lods}: cold {cold} ms, resident {warm} ms")
+    # parity and the CPU baseline: the oracle renders the same frames from
+    # the GPU-decoded levels (one core)
+    t0 = time.perf_counter()
+    dense = dense_levels(ctx, blobs)
+    log(f"[bench] render parity: decoded levels in {time.perf_counter() - t0:.1f}s")
+    exact, cpu_s = [], []
+    for k in range(4):
+        grid, x0, y0, step, w, h, _ = render_frame(k)
+        t0 = time.perf_counter()
+        want, wl = ob.render_quad_lod(dense, grid, x0, y0, step, w, h, 1, 2, -4.0, 4.0, 1.0)
+        cpu_s.append(time.perf_counter() - t0)
+        exact.append(bool(wl == lods[k] and np.array_equal(want, frames[k])))
+        lit = float((want > 0).mean())
+        log(f"[bench] render frame {k}: bit-exact {exact[-1]}, lit {lit:.2f}, oracle {cpu_s[-1]:.2f}s")
+    del dense
+
+    # mixed Poisson phases
+    def batch(k):
+        arr = (mc_amd.McBox * boxes_per_batch)()
+        got = L.mc_archive_sample_boxes(a, 0, C.c_uint64(30000 + k), boxes_per_batch, edge, edge, edge,
+                                        C.c_float(0.5), arr)
+        return [(arr[i].z0, arr[i].y0, arr[i].x0) for i in range(got)]
+
+    def mixed(n_render, n_batch, lam_render, lam_batch, seed, kbase):
+        breqs = [batch(kbase + k) for k in range(n_batch)]
+        rng = np.random.default_rng(seed)
+        ev = [(t, "r", i) for i, t in enumerate(np.cumsum(rng.exponential(1.0 / lam_render, n_render)))]
+        ev += [(t, "b", i) for i, t in enumerate(np.cumsum(rng.exponential(1.0 / lam_batch, n_batch)))]
+        ev.sort()
+        lat = {"r": [None] * n_render, "b": [None] * n_batch}
+        tick = []
+        ready = threading.Semaphore(0)
+
+        def waiter():
+            for j in range(len(ev)):
+                ready.acquire()
+                cls, i, t = tick[j]
+                lat[cls][i] = sv.wait(t)
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        s0 = sv.stats()
+        start = time.perf_counter()
+        used = {"r": 0, "b": 0}
+        for tdue, cls, i in ev:
+            d = start + tdue - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            ring = fb if cls == "r" else bb
+            k = used[cls]
+            if k >= len(ring):            # output buffer reuse: its earlier request must be done
+                while lat[cls][k - len(ring)] is None:
+                    time.sleep(0.0002)
+            if cls == "r":
+                grid, x0, y0, step, w, h, _ = render_frame(kbase + i)
+                t = sv.render(grid, x0, y0, step, w, h, ring[k % len(ring)])
+            else:
+                t = sv.submit(breqs[i], edge, edge, edge, ring[k % len(ring)])
+            used[cls] += 1
+            tick.append((cls, i, t))
+            ready.release()
+        th.join()
+        wall = time.perf_counter() - start
+        s1 = sv.stats()
+        lr, lb = np.array(lat["r"]), np.array(lat["b"])
+        refs = s1["block_refs"] - s0["block_refs"]
+        m = {"renders": n_render, "render_lambda_per_s": lam_render, "batches": n_batch,
+             "batch_lambda_per_s": lam_batch, "wall_s": round(wall, 3),
+             "render_p50_ms": round(float(np.percentile(lr, 50)), 3),
+             "render_p99_ms": round(float(np.percentile(lr, 99)), 3),
+             "batch_p50_ms": round(float(np.percentile(lb, 50)), 3),
+             "batch_p99_ms": round(float(np.percentile(lb, 99)), 3),
+             "frames_per_s": round(n_render / wall, 3),
+             "batch_gbs": round(sum(len(r) for r in breqs) * edge ** 3 / wall / 1e9, 3),
+             "hit_rate": round((s1["hits"] - s0["hits"]) / max(1, refs), 4),
+             "decoded_blocks": s1["misses"] - s0["misses"],
+             "evictions": s1["evictions"] - s0["evictions"]}
+        log(f"[bench] render+serve lambda {lam_render}/{lam_batch} per s: render p50 {m['render_p50_ms']} "
+            f"p99 {m['render_p99_ms']} ms, batch p50 {m['batch_p50_ms']} p99 {m['batch_p99_ms']} ms, "
+            f"{m['frames_per_s']} frames/s, {m['batch_gbs']} GB/s, hit rate {m['hit_rate']}")
+        return m
+
+    phases = [mixed(n_render, n_batch, lam_render, lam_batch, 77, 100),
+              mixed(4 * n_render, 2 * n_batch, 20 * lam_render, 5 * lam_batch, 78, 1000)]
+    sv.close()
+    for b in fb + bb:
+        b.free()
+    return {"volume": "2048^3 u8 + LOD 1-3 (device decimation), q=6", "arena_bytes": arena_bytes,
+            "lanes": lanes, "frame": "1024^2 quad, trilinear, mean of 9 steps (t0=-4, t1=4, dt=1)",
+            "frame_lods": lods, "frame_cold_ms": cold, "frame_resident_ms": warm,
+            "bit_exact_vs_oracle_renderer": all(exact),
+            "cpu_oracle_frame_s": [round(x, 3) for x in cpu_s],
+            "mixed": phases}
+
+
+def serve_leg(rates, nthreads, lanes=8, arena_bytes=0, boxes_per_batch=64, edge=128, warm_batches=100,
+              with_render=True):
     """Serving of box batches from an HBM-resident 2048^3 archive
     (volume2048_q6): batches of 64 boxes of 128^3 (mc_archive_sample_boxes,
     seed 10000 + k, min material fraction 0.5) arrive as a Poisson process
@@ -197,24 +429,11 @@ def serve_leg(rates, nthreads, lanes=8, arena_bytes=0, boxes_per_batch=64, edge=
     import tempfile
     import threading
     L = mc_amd.api()
-    specs = workloads.WORKLOADS["volume2048_q6"]()
     path = os.path.join(tempfile.mkdtemp(prefix="mcserve_"), "vol2048.mca")
-    a = L.mc_archive_open_dims(path.encode(), 2048, 2048, 2048, C.c_float(6.0))
-    if not a:
-        raise RuntimeError("mc_archive_open_dims failed")
     ctx = mc_amd.Ctx(0)
     t0 = time.perf_counter()
-    for g in range(0, len(specs), 64):
-        grp = specs[g:g + 64]
-        ch = mc_amd.synth_chunks([x["seed"] for x in grp], [x["air"] for x in grp],
-                                 coords=[x["coord"] for x in grp], nthreads=nthreads)
-        blobs = ctx.encode_chunks_host(ch, [6.0] * len(grp))
-        for x, bl in zip(grp, blobs):
-            cz, cy, cx = x["coord"]
-            if L.mc_archive_append_chunk_compressed(a, 0, cz, cy, cx, bl, len(bl)) != 0:
-                raise RuntimeError("install failed")
-        del ch
-    log(f"[bench] serving volume built in {time.perf_counter() - t0:.1f}s")
+    a, lv_blobs = build_pyramid(L, ctx, path, nthreads)
+    log(f"[bench] serving volume (LOD 0-3) built in {time.perf_counter() - t0:.1f}s")
 
     def batch(k):
         arr = (mc_amd.McBox * boxes_per_batch)()
@@ -296,6 +515,7 @@ def serve_leg(rates, nthreads, lanes=8, arena_bytes=0, boxes_per_batch=64, edge=
     sv.close()
     for b_ in bufs:
         b_.free()
+    render = render_leg(a, lv_blobs, ctx, nthreads, lanes=lanes) if with_render else None
     L.mc_archive_close(a)
     try:
         os.remove(path)
@@ -306,7 +526,7 @@ def serve_leg(rates, nthreads, lanes=8, arena_bytes=0, boxes_per_batch=64, edge=
             "arena_bytes": arena_bytes, "lanes": lanes, "cold_batch_ms": round(cold_ms, 3),
             "bit_exact_vs_read_regions": exact, "rates": out, "arena_slots": st["arena_slots"],
             "preload": preload,
-            "latency": "submission to completion, output in device memory"}
+            "latency": "submission to completion, output in device memory", "render_serving": render}
 
 
 def main():
@@ -573,6 +793,7 @@ def main():
     if mixed is not None:
         rec["mixed"] = mixed
     if serving is not None:
+        rec["render_serving"] = serving.pop("render_serving", None)
         rec["serving"] = serving
     print(json.dumps(rec))
 

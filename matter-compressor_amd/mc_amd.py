@@ -213,10 +213,10 @@ class Ctx:
     def decimate(self, src, dz, dy, dx, dst):
         check(load().mchip_decimate(self.h, C.c_void_p(src.ptr), dz, dy, dx, C.c_void_p(dst.ptr)))
 
-    def decimate_chunks(self, srcs, dst):
+    def decimate_chunks(self, srcs, dst, dst_off=0):
         """srcs: 8 (buffer, byte offset) pairs or None, (z, y, x) order."""
         arr = (C.c_void_p * 8)(*[None if x is None else x[0].ptr + x[1] for x in srcs])
-        check(load().mchip_decimate_chunks(self.h, arr, C.c_void_p(dst.ptr)))
+        check(load().mchip_decimate_chunks(self.h, arr, C.c_void_p(dst.ptr + dst_off)))
 
     def d2d(self, dst, src, nbytes, dst_off=0, src_off=0):
         check(load().mchip_d2d(self.h, C.c_void_p(dst.ptr + dst_off), C.c_void_p(src.ptr + src_off), nbytes))
