@@ -82,6 +82,10 @@ int mc_archive_append_chunk_raw_q(mc_archive *a, int lod, int cz, int cy, int cx
                                   const mc_u8 vox[256 * 256 * 256], float q);            /* :206 */
 int mc_archive_append_chunk_ctx(mc_archive *a, mc_codec_ctx *C, int lod, int cz, int cy, int cx,
                                 const mc_u8 vox[256 * 256 * 256]);                       /* :212 */
+/* rate control: sample-encode 256 blocks at the archive quality, scale q by
+ * bytes ~ q^-0.75 toward 256^3 / target_ratio bytes, encode at that q (:221) */
+int mc_archive_append_chunk_target(mc_archive *a, int lod, int cz, int cy, int cx,
+                                   const mc_u8 vox[256 * 256 * 256], float target_ratio, float *q_out);
 int mc_archive_append_chunk_compressed(mc_archive *a, int lod, int cz, int cy, int cx,
                                        const uint8_t *blob, size_t len);                 /* :227 */
 int mc_archive_reserve_index(mc_archive *a, int lod, int cz, int cy, int cx);           /* :235 */
@@ -118,6 +122,35 @@ int mc_archive_set_priors(struct mc_archive *a, const uint16_t plo[8][32],
 /* install an archive's prior override as the process priors (used by the
  * reference's reader, src/mc_stream.c; declared in src/mc_archive.h:27) */
 void priors_load(const uint8_t *base);
+
+/* ---- read-only readers over archive bytes (src/matter_compressor.h:313-361)
+ * A flat reader reads an archive held in memory; a streaming reader pulls
+ * byte ranges through a callback, fetching whole chunk blobs or (partial
+ * fetch) only a chunk's bitmap and lengths plus each block's own payload.
+ * Block decodes run on the GPU; voxels equal the reference reader's. */
+typedef struct mc_reader mc_reader;
+typedef int (*mc_read_fn)(void *ud, uint64_t off, uint32_t len, uint8_t *dst);
+mc_reader *mc_open(const uint8_t *arc, size_t len);                                      /* :316 */
+void       mc_close(mc_reader *r);                                                       /* :317 */
+void       mc_reader_set_quality(mc_reader *r, float q);                                 /* :318 */
+uint64_t   mc_chunk_offset(mc_reader *r, int lod, int cz, int cy, int cx);               /* :319 */
+uint64_t   mc_chunk_offset_chk(mc_reader *r, int lod, int cz, int cy, int cx, int *err); /* :323 */
+void       mc_decode_block(mc_reader *r, uint64_t chunk_off, int bz, int by, int bx,
+                           mc_u8 *dst);                                                  /* :324 */
+mc_reader *mc_open_streaming(mc_read_fn read, void *ud, uint64_t total_len);             /* :334 */
+void       mc_reader_set_partial_fetch(mc_reader *r, int on);                            /* :341 */
+uint64_t   mc_reader_chunk_blob_len(mc_reader *r, uint64_t chunk_off);                   /* :346 */
+int        mc_reader_read_blob(mc_reader *r, uint64_t chunk_off, size_t len, uint8_t *dst); /* :352 */
+int        mc_reader_priors(mc_reader *r, const uint16_t **plo, const uint16_t **phi);   /* :356 */
+void       mc_reader_dims(mc_reader *r, int *nx, int *ny, int *nz);                      /* :359 */
+float      mc_reader_quality(mc_reader *r);                                              /* :360 */
+int        mc_reader_nlods(mc_reader *r);                                                /* :361 */
+/* (additive) decode n blocks through the reader in one GPU launch: block i is
+ * (bzyx[3i], bzyx[3i+1], bzyx[3i+2]) of the chunk blob at chunk_off[i], written
+ * to dst + 4096*i exactly as mc_decode_block would.  0, or -1 on a device
+ * failure (dst zeroed). */
+int        mc_reader_decode_blocks(mc_reader *r, int n, const uint64_t *chunk_off, const int32_t *bzyx,
+                                   mc_u8 *dst);
 
 /* ---- HBM-resident serving (additive; replaces the reference's decoded-block
  * cache for box batches, mc_cache_new_archive + mc_cache_get_copy,

@@ -38,7 +38,7 @@ C_FLAGS = ["-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-
            "-Wno-unused-function", "-I", INCLUDE, "-D_GNU_SOURCE"]
 
 HIP_SOURCES = ["mchip_kernels.hip", "mchip_api.hip", "mchip_serve.hip", "mchip_render.hip"]
-HOST_SOURCES = ["mc_host.c", "mc_host_region.c", "mc_host_serve.c"]
+HOST_SOURCES = ["mc_host.c", "mc_host_region.c", "mc_host_serve.c", "mc_host_stream.c"]
 
 
 def _run(cmd):

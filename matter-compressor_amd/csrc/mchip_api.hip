@@ -709,8 +709,9 @@ int mchip_decode_blocks_host(mchip_ctx *c, const uint8_t *const *payloads, const
     uint64_t tot = 0;
     for (int i = 0; i < n; ++i) if (payloads[i]) tot += ((uint64_t)lens[i] + 15) & ~15ull;
     size_t meta = (size_t)n * (8 + 4 + 4);
+    const size_t pay_at = (meta + 255) & ~size_t(255);     // payloads gathered after the records
     void *h = nullptr;
-    if (desc_buffer(c, meta, &h)) return -1;
+    if (desc_buffer(c, pay_at + tot, &h)) return -1;
     uint64_t *hoff = (uint64_t *)h;
     uint32_t *hlen = (uint32_t *)(hoff + n);
     uint32_t *hq = hlen + n;
@@ -727,9 +728,10 @@ int mchip_decode_blocks_host(mchip_ctx *c, const uint8_t *const *payloads, const
         ensure(c, &c->d_stage_out, &c->cap_stage_out, (size_t)n * kNVox))
         return -1;
     uint8_t *din = (uint8_t *)c->d_stage_in;
+    uint8_t *hpay = (uint8_t *)h + pay_at;
     for (int i = 0; i < n; ++i)
-        if (payloads[i] && lens[i])
-            HIPCHK(hipMemcpyAsync(din + hoff[i], payloads[i], lens[i], hipMemcpyHostToDevice, c->stream));
+        if (payloads[i] && lens[i]) memcpy(hpay + hoff[i], payloads[i], lens[i]);
+    if (tot) HIPCHK(hipMemcpyAsync(din, hpay, tot, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_aux, h, meta, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipEventRecord(c->ev_desc, c->stream));
     c->desc_pending = true;

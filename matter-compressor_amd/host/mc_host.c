@@ -695,6 +695,63 @@ int mc_archive_append_chunk_raw(mc_archive *a, int lod, int cz, int cy, int cx, 
     return mc_archive_append_chunk_raw_q(a, lod, cz, cy, cx, vox, a ? a->quality : 8.0f);
 }
 
+/* Rate control (src/mc_archive.c:618-648): the 256 blocks (bz, by, (bz+by)&15)
+ * are sample-encoded at the archive's quality on the GPU (one launch); their
+ * payload bytes, scaled to 4096 blocks, set q once by bytes ~ q^-0.75; the
+ * chunk is then encoded at that q. */
+int mc_archive_append_chunk_target(mc_archive *a, int lod, int cz, int cy, int cx,
+                                   const mc_u8 vox[256 * 256 * 256], float target_ratio, float *q_out) {
+    if (!a || lod < 0 || lod > 7 || !vox || !(target_ratio > 1.0f)) return -1;
+    mc_codec_ctx C = {8.0f, 0};
+    mc_codec_ctx_set_quality(&C, a->quality);
+    uint8_t *blk = malloc(256 * 4096);
+    uint8_t *pay = malloc((size_t)256 * MCH_SLOT_BYTES);
+    uint8_t *pp[256];
+    uint32_t lens[256];
+    float qv[256];
+    int tv[256], st[256], nb = 0;
+    if (!blk || !pay) { free(blk); free(pay); return -1; }
+    for (int bz = 0; bz < 16; ++bz)
+        for (int by = 0; by < 16; ++by) {
+            const int bx = (bz + by) & 15;
+            uint8_t *d = blk + (size_t)nb * 4096;
+            int any = 0;
+            for (int z = 0; z < 16; ++z)
+                for (int y = 0; y < 16; ++y) {
+                    const mc_u8 *src = vox + ((size_t)(16 * bz + z) * 256 + (16 * by + y)) * 256 + 16 * bx;
+                    memcpy(d + (z * 16 + y) * 16, src, 16);
+                    for (int x = 0; x < 16; ++x) any |= src[x];
+                }
+            if (!any) continue;                       /* air: counted as sampled, no bytes */
+            pp[nb] = pay + (size_t)nb * MCH_SLOT_BYTES;
+            qv[nb] = C.quality;
+            tv[nb] = 0;
+            nb++;
+        }
+    size_t sample_bytes = 0;
+    if (nb) {
+        mchip_ctx *d = mch_dev();
+        if (!d || mchip_encode_blocks_host(d, blk, qv, tv, nb, pp, lens, st) != 0) {
+            mch_dev_error("mc_archive_append_chunk_target");
+            free(blk); free(pay);
+            return -1;
+        }
+        for (int i = 0; i < nb; ++i) sample_bytes += lens[i];
+    }
+    free(blk); free(pay);
+    const float q0 = a->quality;                      /* as given; the sample encode used it sanitised */
+    float q = q0;
+    if (sample_bytes) {
+        const double est_total = (double)sample_bytes * (4096.0 / 256);
+        const double want_total = (double)(256.0 * 256.0 * 256.0) / target_ratio;
+        q = (float)(q0 * pow(est_total / want_total, 1.0 / 0.75f));
+        if (q < 0.5f) q = 0.5f;
+        if (q > 24.0f) q = 24.0f;
+    }
+    if (q_out) *q_out = q;
+    return mc_archive_append_chunk_raw_q(a, lod, cz, cy, cx, vox, q);
+}
+
 int mc_archive_append_chunk_compressed(mc_archive *a, int lod, int cz, int cy, int cx, const uint8_t *blob,
                                        size_t len) {
     if (!a || lod < 0 || lod > 7 || !blob || !len) return -1;
