@@ -103,20 +103,30 @@ struct TableSrc {
 };
 
 // Records the blocks a sample reads (bit per block of the table's grid);
-// every sample returns 0.
+// every sample returns 0.  A workgroup's threads first deduplicate their
+// blocks in an LDS hash set (kTouchSet entries, linear probing), so the
+// bitmap in global memory takes one atomic per distinct block per workgroup.
+constexpr int kTouchSet = 1024;
 struct TouchSrc {
     uint32_t *bits;
     TableLevel t;
     int nz, ny, nx;
     mutable int last;
+    int *set;                  // LDS, kTouchSet entries, -1 = empty
     __device__ void mark(int bz, int by, int bx) const {
         bz -= t.obz; by -= t.oby; bx -= t.obx;
         if ((unsigned)bz >= (unsigned)t.gz || (unsigned)by >= (unsigned)t.gy || (unsigned)bx >= (unsigned)t.gx) return;
         const int id = (bz * t.gy + by) * t.gx + bx;
         if (id == last) return;
         last = id;
-        const uint32_t m = 1u << (id & 31);
-        if (!(*(volatile uint32_t *)(bits + (id >> 5)) & m)) atomicOr(bits + (id >> 5), m);
+        uint32_t h = ((uint32_t)id * 2654435761u) >> 22;                  // 10 bits
+        for (int k = 0; k < 16; ++k, h = (h + 1) & (kTouchSet - 1)) {
+            const int v = atomicCAS(set + h, -1, id);
+            if (v == id) return;                      // another thread of the workgroup marked it
+            if (v == -1) break;                       // first in this workgroup: mark globally
+            if (k == 15) break;                       // set crowded: mark globally
+        }
+        atomicOr(bits + (id >> 5), 1u << (id & 31));
     }
     __device__ float voxel(int z, int y, int x) const {
         if ((unsigned)z >= (unsigned)nz || (unsigned)y >= (unsigned)ny || (unsigned)x >= (unsigned)nx) return 0.0f;
@@ -266,9 +276,13 @@ __global__ __launch_bounds__(256) void k_render(RenderDesc d, Src s, uint8_t *__
 }
 
 __global__ __launch_bounds__(256) void k_render_touch(RenderDesc d, TouchSrc s) {
+    __shared__ int set[kTouchSet];
+    for (int i = threadIdx.x; i < kTouchSet; i += 256) set[i] = -1;
+    __syncthreads();
     const int64_t pix = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (pix >= (int64_t)d.w * d.h) return;
     s.last = -1;
+    s.set = set;
     (void)render_pixel(d, s, (int)(pix / d.w), (int)(pix % d.w));
 }
 
@@ -424,7 +438,7 @@ void launch_render_table(const RenderDesc &d, const TableLevel &t, uint8_t *out,
     hipLaunchKernelGGL(k_render<TableSrc>, dim3(grid_of((int64_t)d.w * d.h)), dim3(256), 0, s, d, src, out);
 }
 void launch_render_touch(const RenderDesc &d, const TableLevel &t, uint32_t *bits, hipStream_t s) {
-    TouchSrc src{bits, t, d.nz, d.ny, d.nx, -1};
+    TouchSrc src{bits, t, d.nz, d.ny, d.nx, -1, nullptr};
     hipLaunchKernelGGL(k_render_touch, dim3(grid_of((int64_t)d.w * d.h)), dim3(256), 0, s, d, src);
 }
 void launch_bits_list(const uint32_t *bits, uint32_t nbits, uint32_t *list, uint32_t *count, hipStream_t s) {
