@@ -261,8 +261,9 @@ int64_t      mchip_server_submit(mchip_server *sv, const int64_t *origins, int n
 /* Submit a quad render (mc_render_quad_lod, src/mc_render.c:2302-2314): the
  * gw x gh control grid (host, (z,y,x) per point in level-0 voxels, copied) is
  * rendered over the store's levels into the w x h image at d_out (device).
- * filter: 0 nearest, 1 trilinear; comp: 0 none, 1 min, 2 mean, 3 max (the
- * reference's mc_filter / mc_comp values); the slab [t0, t1] along the
+ * filter: 0 nearest, 1 trilinear; comp: 0 none, 1 min, 2 mean, 3 max,
+ * 4 alpha, 5 stddev (the reference's mc_filter / mc_comp values; alpha with
+ * the default alpha_min / alpha_opacity); the slab [t0, t1] along the
  * normal is walked at dt.  The level is chosen and the points remapped as the
  * reference does; the render reads only decoded blocks of that level, which
  * are admitted and decoded like a box request's.  Returns a ticket, or -1. */

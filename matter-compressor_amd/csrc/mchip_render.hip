@@ -5,8 +5,9 @@
 // quad's control grid (mc_quad_gen, :1981-2052), remaps it to the chosen
 // level (lod_rowgen, :2262-2274), walks the ray along the normal and reduces
 // the samples (render_pixel, :1150-1281: trilinear or nearest; none, min,
-// mean or max; the trilinear walk takes groups of four samples whose sum is
-// ((v0+v1)+v2)+v3, then single samples) and writes to_u8 (:793-795).  Every
+// mean, max, alpha or stddev; the trilinear walk takes groups of four
+// samples -- the mean adds each group as ((v0+v1)+v2)+v3 -- then single
+// samples) and writes to_u8 (:793-795).  Every
 // floating-point operation is the reference's, in its order: the library is
 // built without contraction and with correctly rounded division and square
 // root, so each pixel equals the reference renderer's bit for bit.
@@ -232,9 +233,20 @@ __device__ uint8_t render_pixel(const RenderDesc &d, const Src &s, int row, int 
     }
     const float sz_ = d.dt * nz, sy_ = d.dt * ny, sx_ = d.dt * nx;
     float pz = P[0] + d.t0 * nz, py = P[1] + d.t0 * ny, px = P[2] + d.t0 * nx;
-    float mn = 255.0f, mx = 0.0f, sum = 0.0f;
+    float mn = 255.0f, mx = 0.0f, sum = 0.0f, sum2 = 0.0f, acc = 0.0f, A = 0.0f;
     int it = 0;
     bool done = false;
+    // front-to-back alpha (comp 4): opacity from the value above a_th,
+    // stopping at 98 % accumulated opacity
+    auto alpha_step = [&](float v) {
+        float a = (v * (1.0f / 255.0f) - d.a_th) * d.a_sc;
+        if (a > 0.0f) {
+            if (a > 1.0f) a = 1.0f;
+            acc += (1.0f - A) * a * v;
+            A += (1.0f - A) * a;
+            if (A >= 0.98f) done = true;
+        }
+    };
     if (d.filter == 1) {
         for (; it + 4 <= d.nsteps && !done; it += 4) {
             float v4[4];
@@ -250,8 +262,13 @@ __device__ uint8_t render_pixel(const RenderDesc &d, const Src &s, int row, int 
                 #pragma unroll
                 for (int k = 0; k < 4; k++) if (v4[k] > mx) mx = v4[k];
                 if (mx >= 255.0f) done = true;
-            } else {
+            } else if (d.comp == 2) {
                 sum += v4[0] + v4[1] + v4[2] + v4[3];
+            } else if (d.comp == 5) {
+                #pragma unroll
+                for (int k = 0; k < 4; k++) { sum += v4[k]; sum2 += v4[k] * v4[k]; }
+            } else {
+                for (int k = 0; k < 4 && !done; k++) alpha_step(v4[k]);
             }
         }
     }
@@ -259,12 +276,21 @@ __device__ uint8_t render_pixel(const RenderDesc &d, const Src &s, int row, int 
         const float v = sample(s, pz, py, px, d.filter);
         if (d.comp == 1) { if (v < mn) mn = v; }
         else if (d.comp == 3) { if (v > mx) mx = v; }
-        else sum += v;
+        else if (d.comp == 2) sum += v;
+        else if (d.comp == 5) { sum += v; sum2 += v * v; }
+        else alpha_step(v);
         pz += sz_; py += sy_; px += sx_;
     }
     if (d.comp == 1) return to_u8(mn);
     if (d.comp == 3) return to_u8(mx);
-    return to_u8(d.nsteps ? sum / (float)d.nsteps : 0.0f);
+    if (d.comp == 2) return to_u8(d.nsteps ? sum / (float)d.nsteps : 0.0f);
+    if (d.comp == 5) {
+        if (!d.nsteps) return 0;
+        const float m = sum / (float)d.nsteps;
+        const float var = sum2 / (float)d.nsteps - m * m;
+        return to_u8(var > 0.0f ? sqrtf(var) : 0.0f);
+    }
+    return to_u8(acc);
 }
 
 template <class Src>
@@ -405,7 +431,7 @@ static int pick_lod(int nlods, float vpp) {
 int render_describe(RenderDesc &d, int nlods, const int32_t *dims, const float *grid_host, const float *grid_dev,
                     int gw, int gh, float x0, float y0, float step, int w, int h, int filter, int comp, float t0,
                     float t1, float dt) {
-    if ((filter != 0 && filter != 1) || comp < 0 || comp > 3) return -1;
+    if ((filter != 0 && filter != 1) || comp < 0 || comp > 5) return -1;
     const int L = pick_lod(nlods, step * quad_spacing(grid_host, gw, gh));
     const float inv = 1.0f / (float)(1 << L);
     if (L > 0) { t0 = t0 * inv; t1 = t1 * inv; }       // render_lod (src/mc_render.c:2276-2288)
@@ -420,6 +446,10 @@ int render_describe(RenderDesc &d, int nlods, const int32_t *dims, const float *
     d.x0 = x0; d.y0 = y0; d.step = step;
     d.w = w; d.h = h;
     d.filter = filter; d.comp = comp;
+    // alpha parameters at the reference's defaults (alpha_min and
+    // alpha_opacity zero: threshold 0, scale 1 / (1 - 0))
+    d.a_th = 0.0f;
+    d.a_sc = 1.0f / (1.0f - 0.0f);
     d.lod = L;
     d.a = inv;
     d.b = 0.5f * inv - 0.5f;

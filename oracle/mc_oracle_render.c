@@ -7,9 +7,10 @@
  * (src/mc_render.c:1981-2052) as driven row by row by quad_rowgen
  * (:2097-2102), the LOD remap of lod_rowgen / render_lod (:2255-2288),
  * mc_render_pick_lod and mc_quad_spacing (:2222-2250), make_cfg's step count
- * (:827-835), render_pixel for composites NONE, MIN, MEAN and MAX
- * (:1150-1281; the trilinear filter walks the ray in groups of four with the
- * group sum ((v0+v1)+v2)+v3, then one sample at a time), the trilinear and
+ * (:827-840), render_pixel for composites NONE, MIN, MEAN, MAX, ALPHA (with
+ * zero alpha_min / alpha_opacity, the defaults) and STDDEV (:1150-1281; the
+ * trilinear filter walks the ray in groups of four -- the mean takes the
+ * group sum ((v0+v1)+v2)+v3 -- then one sample at a time), the trilinear and
  * nearest samplers (:176-217, :86-89, :58-67; every path of the reference's
  * sampler -- dense, in-block, straddling, edge, and the 4-wide groups --
  * computes the same corner values and the same mul/add sequence) and to_u8
@@ -187,8 +188,21 @@ static uint8_t render_pixel(const orc_level *L, const float *P, const float *Nn,
     }
     const float sz_ = dt * nz, sy_ = dt * ny, sx_ = dt * nx;
     float pz = P[0] + t0 * nz, py = P[1] + t0 * ny, px = P[2] + t0 * nx;
-    float mn = 255.0f, mx = 0.0f, sum = 0.0f;
+    /* alpha with the reference's defaults for zero alpha_min / alpha_opacity
+     * (make_cfg, src/mc_render.c:838-840) */
+    const float a_th = 0.0f, a_sc = 1.0f / (1.0f - 0.0f);
+    float mn = 255.0f, mx = 0.0f, sum = 0.0f, sum2 = 0.0f, acc = 0.0f, A = 0.0f;
     int it = 0, done = 0;
+#define ORC_ALPHA_STEP(v)                                  \
+    do {                                                   \
+        float a = ((v) * (1.0f / 255.0f) - a_th) * a_sc;   \
+        if (a > 0.0f) {                                    \
+            if (a > 1.0f) a = 1.0f;                        \
+            acc += (1.0f - A) * a * (v);                   \
+            A += (1.0f - A) * a;                           \
+            if (A >= 0.98f) done = 1;                      \
+        }                                                  \
+    } while (0)
     if (filter == ORC_FILTER_TRILINEAR) {
         for (; it + 4 <= nsteps && !done; it += 4) {
             float v4[4];
@@ -201,8 +215,12 @@ static uint8_t render_pixel(const orc_level *L, const float *P, const float *Nn,
             } else if (comp == ORC_COMP_MAX) {
                 for (int k = 0; k < 4; k++) if (v4[k] > mx) mx = v4[k];
                 if (mx >= 255.0f) done = 1;
-            } else {
+            } else if (comp == ORC_COMP_MEAN) {
                 sum += v4[0] + v4[1] + v4[2] + v4[3];
+            } else if (comp == ORC_COMP_STDDEV) {
+                for (int k = 0; k < 4; k++) { sum += v4[k]; sum2 += v4[k] * v4[k]; }
+            } else {
+                for (int k = 0; k < 4 && !done; k++) ORC_ALPHA_STEP(v4[k]);
             }
         }
     }
@@ -210,12 +228,22 @@ static uint8_t render_pixel(const orc_level *L, const float *P, const float *Nn,
         const float v = sample(L, pz, py, px, filter);
         if (comp == ORC_COMP_MIN) { if (v < mn) mn = v; }
         else if (comp == ORC_COMP_MAX) { if (v > mx) mx = v; }
-        else sum += v;
+        else if (comp == ORC_COMP_MEAN) sum += v;
+        else if (comp == ORC_COMP_STDDEV) { sum += v; sum2 += v * v; }
+        else ORC_ALPHA_STEP(v);
         pz += sz_; py += sy_; px += sx_;
     }
+#undef ORC_ALPHA_STEP
     if (comp == ORC_COMP_MIN) return to_u8(mn);
     if (comp == ORC_COMP_MAX) return to_u8(mx);
-    return to_u8(nsteps ? sum / (float)nsteps : 0.0f);
+    if (comp == ORC_COMP_MEAN) return to_u8(nsteps ? sum / (float)nsteps : 0.0f);
+    if (comp == ORC_COMP_STDDEV) {
+        if (!nsteps) return 0;
+        const float m = sum / (float)nsteps;
+        const float var = sum2 / (float)nsteps - m * m;
+        return to_u8(var > 0.0f ? sqrtf(var) : 0.0f);
+    }
+    return to_u8(acc);
 }
 
 int orc_render_steps(float t0, float t1, float dt) {
