@@ -53,14 +53,23 @@ __device__ __forceinline__ uint32_t wg_exclusive_scan(uint32_t v, uint32_t *lds4
     return base + x - v;
 }
 
+// Lane order of the entropy kernel: present blocks by decreasing payload
+// length in 8-byte buckets, so each wave holds blocks of similar length and
+// the longest chains start first.
+constexpr uint32_t kLenBuckets = 1024;
+__device__ __forceinline__ uint32_t len_bucket(uint32_t len) {
+    return kLenBuckets - 1u - min(len >> 3, kLenBuckets - 1u);
+}
+
 __global__ __launch_bounds__(256) void k_index(const uint8_t *__restrict__ blobs,
                                                const DescRef cd,
                                                BlkRec *__restrict__ rec,
                                                uint16_t *__restrict__ plist,
                                                ChunkInfo *__restrict__ cinfo) {
     __shared__ uint32_t lds4[4];
-    __shared__ uint32_t keys[kNVox];
+    __shared__ uint32_t hist[kLenBuckets];
     const int c = blockIdx.x, t = threadIdx.x;
+    for (uint32_t i = t; i < kLenBuckets; i += 256) hist[i] = 0;
     const uint8_t *b = blobs + cd[c].blob_off;
     const uint64_t L = cd[c].blob_len;
     bool ok = L >= 14;
@@ -69,7 +78,7 @@ __global__ __launch_bounds__(256) void k_index(const uint8_t *__restrict__ blobs
     ok = ok && bm + 512 <= L;
     uint32_t bits = ok ? (blob_byte(b, L, bm + 2 * t) | (blob_byte(b, L, bm + 2 * t + 1) << 8)) : 0u;
     uint32_t np;
-    uint32_t slot0 = wg_exclusive_scan(__popc(bits), lds4, np);
+    uint32_t slot0 = wg_exclusive_scan(__popc(bits), lds4, np);   // (its barriers also order the hist clear)
     uint64_t lens = bm + 512;
     if (!ok || lens + 2ull * np > L) { bits = 0; np = 0; }
     uint32_t sum = 0;
@@ -82,38 +91,43 @@ __global__ __launch_bounds__(256) void k_index(const uint8_t *__restrict__ blobs
     uint32_t pbase = wg_exclusive_scan(sum, lds4, ptot);
     uint64_t pay0 = lens + 2ull * np;
     uint64_t off = pay0 + pbase;
+    // counting sort of the present blocks by length bucket: a block's place is
+    // its bucket's base plus its rank among the bucket's blocks (the order of
+    // the LDS atomics, which decides only which lane decodes the block)
+    uint32_t place[16];
+    #pragma unroll
+    for (int j = 0; j < 16; ++j) place[j] = 0;
+    #pragma unroll
     for (int j = 0, s = (int)slot0; j < 16; ++j) {
         int bi = t * 16 + j;
         BlkRec r = {0, 0, 0, 0};
-        uint32_t key = 0xFFFFFFFFu;                  // absent blocks sort last
         if ((bits >> j) & 1) {
             uint64_t a = lens + 2ull * s;
             uint32_t len = b[a] | ((uint32_t)b[a + 1] << 8);
             if (off + len <= L) { r.off = (uint32_t)off; r.len = len; r.slot = (uint32_t)s; }
-            key = ((65535u - len) << 12) | (uint32_t)bi;   // longest payload first
+            const uint32_t k = len_bucket(len);
+            place[j] = (k << 16) | atomicAdd(&hist[k], 1u);
             off += len;
             s++;
         }
-        keys[bi] = key;
         rec[(size_t)c * kNVox + bi] = r;
     }
-    // Lane order of the entropy kernel: present blocks by decreasing payload
-    // length, so each wave holds blocks of similar length and the longest
-    // chains start first (bitonic sort of 4096 keys in LDS).
     __syncthreads();
-    for (uint32_t k = 2; k <= (uint32_t)kNVox; k <<= 1)
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = t; i < (uint32_t)kNVox; i += 256) {
-                const uint32_t ixj = i ^ j;
-                if (ixj > i) {
-                    const uint32_t a = keys[i], bb = keys[ixj];
-                    const bool up = (i & k) == 0;
-                    if ((a > bb) == up) { keys[i] = bb; keys[ixj] = a; }
-                }
-            }
-            __syncthreads();
+    // exclusive scan of the bucket counts, four buckets per thread
+    const uint32_t h0 = hist[4 * t], h1 = hist[4 * t + 1], h2 = hist[4 * t + 2], h3 = hist[4 * t + 3];
+    uint32_t htot;
+    const uint32_t hb = wg_exclusive_scan(h0 + h1 + h2 + h3, lds4, htot);
+    hist[4 * t] = hb;
+    hist[4 * t + 1] = hb + h0;
+    hist[4 * t + 2] = hb + h0 + h1;
+    hist[4 * t + 3] = hb + h0 + h1 + h2;
+    __syncthreads();
+    #pragma unroll
+    for (int j = 0; j < 16; ++j)
+        if ((bits >> j) & 1) {
+            const uint32_t at = hist[place[j] >> 16] + (place[j] & 0xFFFFu);
+            plist[(size_t)c * kNVox + at] = (uint16_t)(t * 16 + j);
         }
-    for (uint32_t i = t; i < np; i += 256) plist[(size_t)c * kNVox + i] = (uint16_t)(keys[i] & 4095u);
     if (t == 0) { cinfo[c].np = np; cinfo[c].pay0 = (uint32_t)pay0; }
 }
 
