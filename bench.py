@@ -190,6 +190,78 @@ def cpu_reference_baseline(chunks, blobs, man, nthreads):
     return res
 
 
+def dropin_leg(chunks, blobs, nthreads):
+    """The drop-in calls as a caller of the reference's API makes them, with
+    host buffers (so PCIe transfers are included): mc_archive_decode_chunk of
+    every blob from an archive into pageable host memory, from one thread and
+    from nthreads threads (each with its own device context), and
+    mc_archive_append_chunk_par of the given chunks from one thread.  GB/s of
+    decoded (or encoded) voxels."""
+    import tempfile
+    import threading
+    import shutil
+    L = mc_amd.api()
+    n = len(blobs)
+    tmp = tempfile.mkdtemp(prefix="mcamd_")
+    res = {}
+    try:
+        a = L.mc_archive_open_dims(os.path.join(tmp, "d.mca").encode(), 256 * n, 256, 256, C.c_float(6.0))
+        assert a, "archive open failed"
+        for i, b in enumerate(blobs):
+            bb = np.frombuffer(b, np.uint8)
+            assert L.mc_archive_append_chunk_compressed(a, 0, 0, 0, i, bb.ctypes.data, bb.size) == 0
+        offs = [L.mc_archive_chunk_offset(a, 0, 0, 0, i) for i in range(n)]
+        out = np.empty((nthreads, 256, 256, 256), np.uint8)
+
+        def run(t, stride):
+            for i in range(t, n, stride):
+                L.mc_archive_decode_chunk(a, offs[i], out[t].ctypes.data, 0)
+
+        # a pool of caller threads that live across the rounds: a thread's
+        # device context is created by its first call and kept until it exits
+        rounds = 2
+        start, done = threading.Barrier(nthreads + 1), threading.Barrier(nthreads + 1)
+
+        def worker(t):
+            for _ in range(rounds):
+                start.wait()
+                run(t, nthreads)
+                done.wait()
+
+        pool = [threading.Thread(target=worker, args=(t,)) for t in range(nthreads)]
+        for x in pool:
+            x.start()
+        gb = n * mc_amd.CHUNK_BYTES / 1e9
+        for r in range(rounds):                # round 0 creates the contexts
+            t0 = time.perf_counter()
+            start.wait()
+            done.wait()
+            dt = time.perf_counter() - t0
+        for x in pool:
+            x.join()
+        res[f"decode_chunk_{nthreads}_threads_gbs"] = round(gb / dt, 3)
+        run(0, 1)
+        t0 = time.perf_counter()
+        run(0, 1)
+        res["decode_chunk_1_thread_gbs"] = round(gb / (time.perf_counter() - t0), 3)
+        L.mc_archive_close(a)
+        e = L.mc_archive_open_dims(os.path.join(tmp, "e.mca").encode(), 256 * len(chunks), 256, 256,
+                                   C.c_float(6.0))
+        assert e, "archive open failed"
+        L.mc_archive_append_chunk_par(e, 0, 0, 0, 0, chunks[0].ctypes.data, C.c_float(6.0), 0)   # warm
+        t0 = time.perf_counter()
+        for i in range(len(chunks)):
+            L.mc_archive_append_chunk_par(e, 0, 0, 0, i, chunks[i].ctypes.data, C.c_float(6.0), 0)
+        res["append_chunk_par_1_thread_gbs"] = round(len(chunks) * mc_amd.CHUNK_BYTES / 1e9
+                                                     / (time.perf_counter() - t0), 3)
+        L.mc_archive_close(e)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    res["note"] = (f"host buffers, PCIe included: decode of {n} blobs from an archive into pageable memory; "
+                   f"append of {len(chunks)} chunks")
+    return res
+
+
 def cpu_record(ref, port_dec, port_enc, ns, n, nthreads):
     """The cpu_baseline object: the reference's shipped build when it was
     timed (its faster decode policy is the value), else the oracle port."""
@@ -760,6 +832,9 @@ def main():
         ref_cpu = cpu_reference_baseline(chunks[:ns], blobs[:ns], man, nthreads)
     if ref_cpu:
         log("[bench] reference CPU build: " + ", ".join(f"{k} {v:.3f}" for k, v in ref_cpu.items()))
+    log("[bench] drop-in calls with host buffers")
+    dropin = dropin_leg(chunks[:ns], blobs, nthreads)
+    log(f"[bench] drop-in: {dropin}")
     del chunks
 
     offs = np.zeros(n, np.uint64)
@@ -921,6 +996,7 @@ def main():
                    "launch": f"{min(n, enc_batch)} chunks per launch",
                    "kernels_ms_per_chunk": {k: round(v[0] / (esteps * n), 4) for k, v in enc_k.items()}},
         "verify": verify,
+        "dropin_host_buffers": dropin,
     }
     rec.update(extra)
     d_blobs.free()
