@@ -19,9 +19,14 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 # Per-chunk launches overlap across streams only as far as the process has
-# hardware queues (HIP's default is 4); the runtime reads this variable when
-# it starts, so it is set before the library is loaded.
-QUEUES = 32
+# hardware queues (HIP's default is 4): a hardware queue runs one kernel at a
+# time, so two streams on one queue take turns.  The runtime's null stream
+# holds one queue, so the per-chunk leg uses QUEUES - 1 streams, one per
+# remaining queue.  Measured on MI355X (64 chunks of q=6): 73 GB/s with 32
+# queues shared by 64 streams, 92 with 16 queues, 111 with 22, and 73 again
+# with 24.  The runtime reads the variable when it starts, so it is set
+# before the library is loaded.
+QUEUES = 22
 # The serving leg runs in a child process of its own (--serve-only) with as
 # many hardware queues as it has streams: idle queues left by the per-chunk
 # leg would otherwise share the hardware's queue slots with the serving
@@ -727,9 +732,9 @@ def main():
     ap.add_argument("--workload", default="plain64_q6")
     ap.add_argument("--no-batched", action="store_true", help="skip the all-chunks-in-one-launch measurement")
     ap.add_argument("--batched", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--streams", type=int, default=2 * QUEUES, help="HIP streams the per-chunk launches rotate over")
+    ap.add_argument("--streams", type=int, default=QUEUES - 1, help="HIP streams the per-chunk launches rotate over")
     ap.add_argument("--tune", action="append", default=[], help="key=value launch tuning (mchip_set_tuning)")
-    ap.add_argument("--timed-streams", type=int, default=8,
+    ap.add_argument("--timed-streams", type=int, default=4,
                     help="streams whose kernels carry HIP timing events during the timed steps")
     ap.add_argument("--serve-batches", default="1000,1000,1000",
                     help="batches served at 20, 50 and 100 batches/s (serving leg); 0 skips the leg")
@@ -832,9 +837,7 @@ def main():
         ref_cpu = cpu_reference_baseline(chunks[:ns], blobs[:ns], man, nthreads)
     if ref_cpu:
         log("[bench] reference CPU build: " + ", ".join(f"{k} {v:.3f}" for k, v in ref_cpu.items()))
-    log("[bench] drop-in calls with host buffers")
-    dropin = dropin_leg(chunks[:ns], blobs, nthreads)
-    log(f"[bench] drop-in: {dropin}")
+    drop_chunks = chunks[:ns].copy()
     del chunks
 
     offs = np.zeros(n, np.uint64)
@@ -901,20 +904,37 @@ def main():
               "cpu_oracle_1_core": round(cpu_v, 3)}
     log(f"[bench] verify resident {verify['device_resident']} GB/s, host {verify['host_blobs']} GB/s, "
         f"oracle 1 core {verify['cpu_oracle_1_core']} GB/s")
+    # The per-chunk leg's streams are created after the single-stream legs
+    # (idle hardware queues slow single-stream work) and before the drop-in
+    # leg, so that each takes a hardware queue of its own: the drop-in leg's
+    # caller threads create streams that share queues with them, but those
+    # are gone or idle when the per-chunk leg runs.
     ctxs = [ctx] + [mc_amd.Ctx(0) for _ in range(args.streams - 1)]
+    log("[bench] drop-in calls with host buffers")
+    dropin = dropin_leg(drop_chunks, blobs, nthreads)
+    log(f"[bench] drop-in: {dropin}")
+    del drop_chunks
     # per-chunk call arguments built once: each step issues exactly the C calls
     # mchip_decode_chunks(ctx, blobs, &off[i], &len[i], &q[i], 1, out_i, 16 MiB)
     Lib = mc_amd.load()
     argv = []
     for i in range(n):
         cargs = (np.array([offs[i]], np.uint64), np.array([lens[i]], np.uint64), np.array([qs[i]], np.float32))
-        argv.append((ctxs[i % len(ctxs)].h, C.c_void_p(d_blobs.ptr), *[a.ctypes.data_as(C.c_void_p) for a in cargs],
+        argv.append((C.c_void_p(d_blobs.ptr), *[a.ctypes.data_as(C.c_void_p) for a in cargs],
                      C.c_void_p(views[i].ptr), cargs))
+    handles = [c_.h for c_ in ctxs]
+    # the chunks go to the streams in one rotation that continues from step to
+    # step, so every stream (and its hardware queue) gets the same number of
+    # chunks over the run rather than a fixed share of each step
+    turn = [0]
 
     def step():
-        for h_, b_, o_, l_, q_, out_, _keep in argv:
-            if Lib.mchip_decode_chunks(h_, b_, o_, l_, q_, 1, out_, mc_amd.CHUNK_BYTES) != 0:
+        k = turn[0]
+        for b_, o_, l_, q_, out_, _keep in argv:
+            if Lib.mchip_decode_chunks(handles[k], b_, o_, l_, q_, 1, out_, mc_amd.CHUNK_BYTES) != 0:
                 raise mc_amd.MchipError(mc_amd.last_error())
+            k = k + 1 if k + 1 < len(handles) else 0
+        turn[0] = k
 
     log(f"[bench] warmup {args.warmup}")
     for _ in range(args.warmup):
@@ -985,7 +1005,7 @@ def main():
         "bit_exact": bad == 0,
         "config": {"workload": args.workload, "chunks": n, "chunk": "256^3 u8", "q": 6.0, "air": 0.419,
                    "decoded_bytes_per_step": n * mc_amd.CHUNK_BYTES, "blob_bytes": blob_bytes,
-                   "launch": "one chunk per launch", "streams": args.streams, "tuning": args.tune,
+                   "launch": "one chunk per launch", "streams": args.streams, "hw_queues": QUEUES, "tuning": args.tune,
                    "kernel_events_on_streams": len(timed_ctxs)},
         "kernels_ms_avg": {k: round(v, 4) for k, v in kt.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
