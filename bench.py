@@ -741,6 +741,8 @@ def main():
     ap.add_argument("--serve-only", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--mixed-only", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-quality batched leg")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the drop-in calls with host buffers (their caller threads stop rocprofv3's tracer)")
     args = ap.parse_args()
     if args.serve_only:
         nbs = [int(x) for x in args.serve_batches.split(",")]
@@ -910,9 +912,11 @@ def main():
     # caller threads create streams that share queues with them, but those
     # are gone or idle when the per-chunk leg runs.
     ctxs = [ctx] + [mc_amd.Ctx(0) for _ in range(args.streams - 1)]
-    log("[bench] drop-in calls with host buffers")
-    dropin = dropin_leg(drop_chunks, blobs, nthreads)
-    log(f"[bench] drop-in: {dropin}")
+    dropin = None
+    if not args.no_dropin:
+        log("[bench] drop-in calls with host buffers")
+        dropin = dropin_leg(drop_chunks, blobs, nthreads)
+        log(f"[bench] drop-in: {dropin}")
     del drop_chunks
     # per-chunk call arguments built once: each step issues exactly the C calls
     # mchip_decode_chunks(ctx, blobs, &off[i], &len[i], &q[i], 1, out_i, 16 MiB)
