@@ -200,7 +200,11 @@ uint64_t mc_reader_chunk_blob_len(mc_reader *r, uint64_t chunk_off) {
 
 int mc_reader_read_blob(mc_reader *r, uint64_t chunk_off, size_t len, uint8_t *dst) {
     if (!r || !chunk_off || !len || !dst) return -1;
-    if (r->arc) { memcpy(dst, r->arc + chunk_off, len); return 0; }
+    if (r->arc) {                /* a range past the archive's end is refused, not read */
+        if (chunk_off > r->len || len > r->len - chunk_off) return -1;
+        memcpy(dst, r->arc + chunk_off, len);
+        return 0;
+    }
     for (size_t done = 0; done < len;) {
         const uint32_t n = len - done > 0x40000000u ? 0x40000000u : (uint32_t)(len - done);
         if (rd(r, chunk_off + done, n, dst + done) != 0) return -1;

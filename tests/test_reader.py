@@ -138,6 +138,12 @@ def test_reader_index_and_accessors_match_reference(archive):
                 assert ours.mc_reader_read_blob(a, co, n, ob.p(x)) == 0
                 assert ref.mc_reader_read_blob(b, co, n, ob.p(y)) == 0
                 assert np.array_equal(x, y) and np.array_equal(x, archive[co:co + n])
+        # a flat read past the archive's end is refused (the reference would
+        # read out of bounds here, so it is not called)
+        z = np.zeros(64, np.uint8)
+        assert ours.mc_reader_read_blob(ro[0], archive.size - 32, 64, ob.p(z)) == -1
+        assert ours.mc_reader_read_blob(ro[0], archive.size - 64, 64, ob.p(z)) == 0
+        assert np.array_equal(z, archive[-64:])
     finally:
         for a in ro:
             ours.mc_close(a)
