@@ -4,7 +4,8 @@ sources (oracle/_ref/libmcref.so, which holds src/mc_stream.c and
 src/mc_archive.c).
 
 CPU: chunk resolution (present, air and absent chunks, failing reads), blob
-lengths and bytes, header fields, priors, through flat and streaming readers.
+lengths and bytes, header fields, priors, through flat and streaming readers;
+the archive and reader error paths of tests/mc_archive_errpaths_test.c.
 GPU: block decodes through every reader mode equal the reference reader's
 (including a corrupted archive), the batched decode equals per-block decodes,
 partial fetch moves far fewer bytes for scattered blocks
@@ -161,6 +162,45 @@ def test_reader_index_and_accessors_match_reference(archive):
     junk = np.zeros(300, np.uint8)
     assert not ours.mc_open(ob.p(junk), junk.size) and not ref.mc_open(ob.p(junk), junk.size)
     assert not ours.mc_open(ob.p(junk), 100)
+
+
+def test_archive_and_reader_error_paths():
+    """The error paths of the reference's tests/mc_archive_errpaths_test.c on
+    this library: a reopen with other dims is refused and leaves the file
+    usable; a file or buffer without the archive magic, or shorter than the
+    header, is refused by the writer and the flat reader; priors stored with
+    mc_archive_set_priors come back through a flat reader."""
+    L = _ours()
+    blobs, _ = _blobs()
+    path = _tmp("e.mca")
+    a = _build_ours(path, blobs)
+    offs = [L.mc_archive_chunk_offset(a, *pl) for pl in PLACES]
+    L.mc_archive_close(a)
+    assert not L.mc_archive_open_dims(path.encode(), 128, 128, 128, C.c_float(6.0))
+    a = L.mc_archive_open_dims(path.encode(), 512, 512, 256, C.c_float(6.0))
+    assert a and [L.mc_archive_chunk_offset(a, *pl) for pl in PLACES] == offs
+    plo = np.full((8, 32), 900, np.uint16)
+    phi = np.full((8, 32), 3100, np.uint16)
+    try:
+        assert L.mc_archive_set_priors(a, ob.p(plo), ob.p(phi)) == 0
+        L.mc_archive_close(a)
+        data = np.fromfile(path, np.uint8)
+        r = L.mc_open(ob.p(data), data.size)
+        lo, hi = C.c_void_p(), C.c_void_p()
+        assert r and L.mc_reader_priors(r, C.byref(lo), C.byref(hi)) == 1
+        got_lo = np.ctypeslib.as_array(C.cast(lo, C.POINTER(C.c_uint16)), (8 * 32,))
+        got_hi = np.ctypeslib.as_array(C.cast(hi, C.POINTER(C.c_uint16)), (8 * 32,))
+        assert np.array_equal(got_lo, plo.ravel()) and np.array_equal(got_hi, phi.ravel())
+        L.mc_close(r)
+    finally:
+        L.mc_codec_set_priors(None, None)          # priors are process-wide: back to the defaults
+    garbage = _tmp("g.mca")
+    with open(garbage, "wb") as f:
+        f.write(b"\xab" * 300)
+    assert not L.mc_archive_open_dims(garbage.encode(), 256, 256, 256, C.c_float(6.0))
+    junk = np.full(300, 0xAB, np.uint8)
+    assert not L.mc_open(ob.p(junk), junk.size)
+    assert not L.mc_open(ob.p(data), 255) and not L.mc_open(None, 4096)
 
 
 BLOCKS = [(z, y, x) for z in (0, 3, 8, 15) for y in (0, 7, 15) for x in (0, 5, 12, 15)]
