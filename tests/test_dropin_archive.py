@@ -262,3 +262,37 @@ def test_gpu_read_regions_match_oracle_crops():
     L.mc_archive_decode_block(a, co, 3, 4, 5, ob.p(blk))
     assert np.array_equal(blk.reshape(16, 16, 16), vol[48:64, 64:80, 256 + 80:256 + 96])
     L.mc_archive_close(a)
+
+
+def test_concurrent_appends_share_index_nodes():
+    """tests/mc_archive_concurrent_test.c: 16 threads append disjoint chunks
+    whose index paths share root, inner and shard nodes, released together by
+    a barrier; every appended chunk must be reachable afterwards, on each of a
+    number of fresh archives."""
+    import threading
+    L = _api()
+    coords = [0, 16, 48, 80, 272, 288, 528, 800]          # index nibbles vary at every level
+    dim = (800 + 1) * 256
+    blob = np.full(64, 0xAB, np.uint8)                    # the index does not parse the blob
+    places = [(z, y, x) for z in coords for y in coords for x in coords]
+    nthread = 16
+    for trial in range(12):
+        a = L.mc_archive_open_dims(_tmp(f"c{trial}.mca").encode(), dim, dim, dim, C.c_float(6.0))
+        assert a
+        start = threading.Barrier(nthread)
+        fails = [0] * nthread
+
+        def worker(t):
+            start.wait()
+            for k in range(t, len(places), nthread):
+                if L.mc_archive_append_chunk_compressed(a, 0, *places[k], ob.p(blob), blob.size) != 0:
+                    fails[t] += 1
+
+        th = [threading.Thread(target=worker, args=(t,)) for t in range(nthread)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        missing = sum(L.mc_archive_chunk_offset(a, 0, *p) <= 1 for p in places)
+        L.mc_archive_close(a)
+        assert sum(fails) == 0 and missing == 0, f"trial {trial}: {sum(fails)} failed, {missing} missing"
