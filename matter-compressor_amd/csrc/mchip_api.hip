@@ -78,7 +78,6 @@ struct Tuning {
     int persistent = 1;       // batched decode (>= persist_min chunks): persistent entropy launch
     int persist_min = 2;
     int persist_wg = 0;       // workgroups of the persistent launch (0: occupancy x CUs)
-    int wave_prio = 0;        // issue priority of one-chunk decodes' longest chains (DecodeArgs::prio)
 };
 Tuning g_tune;
 
@@ -419,7 +418,6 @@ static int prepare_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
     a.nwg = 0;
     a.order = nullptr;
     a.pctl = nullptr;
-    a.prio = g_tune.wave_prio;
     if (a.persistent) {
         if (ensure(c, &c->d_order, &c->cap_order, sizeof(uint32_t) * ((size_t)n * 64 + 64))) return -1;
         a.order = (uint32_t *)c->d_order;
@@ -474,11 +472,6 @@ int mchip_set_tuning(const char *key, int value) {
     if (!strcmp(key, "enc_batch")) {
         if (value < 1 || value > 1024) return fail("enc_batch must be in [1, 1024]");
         g_tune.enc_batch = value;
-        return 0;
-    }
-    if (!strcmp(key, "wave_prio")) {
-        if (value < 0 || value > 3) return fail("wave_prio must be in [0, 3]");
-        g_tune.wave_prio = value;
         return 0;
     }
     if (!strcmp(key, "decoder")) {

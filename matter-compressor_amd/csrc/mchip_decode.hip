@@ -65,8 +65,7 @@ __global__ __launch_bounds__(256) void k_index(const uint8_t *__restrict__ blobs
                                                const DescRef cd,
                                                BlkRec *__restrict__ rec,
                                                uint16_t *__restrict__ plist,
-                                               ChunkInfo *__restrict__ cinfo, int prio) {
-    if (prio >= 3) __builtin_amdgcn_s_setprio(3);
+                                               ChunkInfo *__restrict__ cinfo) {
     __shared__ uint32_t lds4[4];
     __shared__ uint32_t hist[kLenBuckets];
     const int c = blockIdx.x, t = threadIdx.x;
@@ -165,7 +164,7 @@ __global__ __launch_bounds__(64) void k_entropy(const uint8_t *__restrict__ blob
                                                 const uint16_t *__restrict__ plist,
                                                 const ChunkInfo *__restrict__ cinfo,
                                                 uint32_t *__restrict__ scratch,
-                                                uint2 *__restrict__ binfo, int prio) {
+                                                uint2 *__restrict__ binfo) {
     __shared__ uint32_t stage[kMode == kDecLane ? 1 : kStage * 64];
     __shared__ uint32_t sctx[kMode == kDecPosition ? 1 : 17 * 64];
     __shared__ uint32_t pwin[kMode == kDecLane ? kWinAlloc * 64 : 1];
@@ -174,15 +173,6 @@ __global__ __launch_bounds__(64) void k_entropy(const uint8_t *__restrict__ blob
     const int c = blockIdx.y;
     const uint32_t np = cinfo[c].np;
     if (blockIdx.x * 64 >= np) return;
-    // the chunk finishes with its longest chains (the first groups): their
-    // waves take issue slots first when they share a SIMD
-    if (prio == 2) {
-        if (blockIdx.x < 8) __builtin_amdgcn_s_setprio(3);
-    } else if (prio) {
-        if (blockIdx.x < 8) __builtin_amdgcn_s_setprio(3);
-        else if (blockIdx.x < 16) __builtin_amdgcn_s_setprio(2);
-        else if (blockIdx.x < 24) __builtin_amdgcn_s_setprio(1);
-    }
     {   // the chunk's priors and band starts into LDS (off the global-memory path)
         const uint32_t *src = (const uint32_t *)(qtabs + cd[c].qtab);
         uint32_t *dst = (uint32_t *)&sq;
@@ -462,8 +452,7 @@ __global__ __launch_bounds__(256) void k_recon(const uint8_t *__restrict__ blobs
                                                const ChunkInfo *__restrict__ cinfo,
                                                const uint32_t *__restrict__ scratch,
                                                const uint2 *__restrict__ binfo,
-                                               uint8_t *__restrict__ out, int nchunks, int prio) {
-    if (prio >= 3) __builtin_amdgcn_s_setprio(3);
+                                               uint8_t *__restrict__ out, int nchunks) {
     __shared__ float tile[256 * kPad];
     __shared__ __attribute__((aligned(16))) uint8_t u8t[kNVox];
     const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
@@ -505,7 +494,7 @@ hipError_t upload_constants(const float *cmf, const uint16_t *band_start, const 
 }
 
 void launch_index(const DecodeArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_index, dim3(a.n), dim3(256), 0, s, a.blobs, a.cd, a.rec, a.plist, a.cinfo, a.prio);
+    hipLaunchKernelGGL(k_index, dim3(a.n), dim3(256), 0, s, a.blobs, a.cd, a.rec, a.plist, a.cinfo);
 }
 hipError_t occupancy_persist(int *per_cu) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_entropy_persist, 64, 0);
@@ -520,17 +509,17 @@ void launch_entropy(const DecodeArgs &a, hipStream_t s) {
     }
     if (a.decoder == kDecState)
         hipLaunchKernelGGL(k_entropy<kDecState>, dim3(kNVox / 64, a.n), dim3(64), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
-                           a.plist, a.cinfo, a.scratch, a.binfo, a.prio);
+                           a.plist, a.cinfo, a.scratch, a.binfo);
     else if (a.decoder == kDecLane)
         hipLaunchKernelGGL(k_entropy<kDecLane>, dim3(kNVox / 64, a.n), dim3(64), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
-                           a.plist, a.cinfo, a.scratch, a.binfo, a.prio);
+                           a.plist, a.cinfo, a.scratch, a.binfo);
     else
         hipLaunchKernelGGL(k_entropy<kDecPosition>, dim3(kNVox / 64, a.n), dim3(64), 0, s, a.blobs, a.cd, a.qtabs,
-                           a.rec, a.plist, a.cinfo, a.scratch, a.binfo, a.prio);
+                           a.rec, a.plist, a.cinfo, a.scratch, a.binfo);
 }
 void launch_recon(const DecodeArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_recon, dim3((unsigned)a.n * kNVox), dim3(256), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
-                       a.cinfo, a.scratch, a.binfo, a.out, a.n, a.prio);
+                       a.cinfo, a.scratch, a.binfo, a.out, a.n);
 }
 
 // ---------------------------------------------------------------------------

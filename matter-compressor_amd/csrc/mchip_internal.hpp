@@ -60,11 +60,6 @@ struct DecodeArgs {
     int nwg;
     uint32_t *order;        // [n*64]
     uint32_t *pctl;         // [2]
-    // wave issue priority of one-chunk-per-launch decodes (tuning): 0 none;
-    // 1 the entropy workgroups holding the longest blocks raise theirs
-    // (groups 0-7: 3, 8-15: 2, 16-23: 1); 2 only groups 0-7, to 3;
-    // 3 as 1, and k_index and k_recon at 3
-    int prio;
 };
 
 // ---- encode ----------------------------------------------------------------
