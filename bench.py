@@ -267,14 +267,33 @@ def dropin_leg(chunks, blobs, nthreads):
     return res
 
 
+def host_cpu():
+    """The host's CPU model and core counts (as /proc/cpuinfo and the
+    scheduler report them)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except Exception:
+        usable = None
+    return {"model": model, "nproc": os.cpu_count(), "usable": usable}
+
+
 def cpu_record(ref, port_dec, port_enc, ns, n, nthreads):
     """The cpu_baseline object: the reference's shipped build when it was
     timed (its faster decode policy is the value), else the oracle port."""
     port = {"port_decode_gbs": round(port_dec, 3), "port_encode_gbs": round(port_enc, 3),
-            "port_sample": f"oracle decode of {ns} of the {n} chunks, {nthreads} threads over blocks"}
+            "port_sample": f"oracle decode of {ns} of the {n} chunks, {nthreads} threads over blocks",
+            "host": host_cpu()}
     if not ref:
         return {"value": round(port_dec, 3), "unit": "GB/s", "cores": nthreads, "kind": "port",
-                "sample": port["port_sample"], "encode_gbs": round(port_enc, 3)}
+                "sample": port["port_sample"], "encode_gbs": round(port_enc, 3), "host": port["host"]}
     r3 = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in ref.items()}
     return {"value": round(max(ref["fast_api"], ref["fast_workers"]), 3), "unit": "GB/s", "cores": nthreads,
             "kind": "reference",
