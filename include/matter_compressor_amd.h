@@ -172,7 +172,8 @@ mc_serve *mc_serve_open(mc_archive *a, int lod, uint64_t arena_bytes, int lanes)
 mc_serve *mc_serve_open_levels(mc_archive *a, int lod0, int nlods, const int32_t *dims, uint64_t arena_bytes,
                                int lanes);
 /* Quad render over the levels (mc_render_quad_lod, src/mc_render.c:2302-2314,
- * filters trilinear/nearest, composites none/min/mean/max/alpha/stddev; see
+ * filters trilinear/nearest, composites none/min/mean/max/alpha/stddev/
+ * percentile/depth; see
  * mchip_server_submit_render) into the w x h image at d_out (device).
  * Pixels equal the reference renderer's fed the same decoded levels. */
 int64_t   mc_serve_render_quad(mc_serve *s, const float *grid, int gw, int gh, float x0, float y0, float step,

@@ -262,8 +262,9 @@ int64_t      mchip_server_submit(mchip_server *sv, const int64_t *origins, int n
  * gw x gh control grid (host, (z,y,x) per point in level-0 voxels, copied) is
  * rendered over the store's levels into the w x h image at d_out (device).
  * filter: 0 nearest, 1 trilinear; comp: 0 none, 1 min, 2 mean, 3 max,
- * 4 alpha, 5 stddev (the reference's mc_filter / mc_comp values; alpha with
- * the default alpha_min / alpha_opacity); the slab [t0, t1] along the
+ * 4 alpha, 5 stddev, 7 percentile, 8 depth (the reference's mc_filter /
+ * mc_comp values, with the reference's default alpha_min, alpha_opacity and
+ * percentile); the slab [t0, t1] along the
  * normal is walked at dt.  The level is chosen and the points remapped as the
  * reference does; the render reads only decoded blocks of that level, which
  * are admitted and decoded like a box request's.  Returns a ticket, or -1. */

@@ -137,7 +137,8 @@ struct RenderDesc {
     int gw, gh;
     float x0, y0, step;     // pixel (i,j) maps to grid (x0 + j*step, y0 + i*step)
     int w, h;
-    int filter, comp;       // 0 nearest / 1 trilinear; 0 none, 1 min, 2 mean, 3 max, 4 alpha, 5 stddev
+    int filter, comp;       // 0 nearest / 1 trilinear; 0 none, 1 min, 2 mean, 3 max, 4 alpha, 5 stddev,
+                            // 7 percentile, 8 depth
     float a_th, a_sc;       // alpha threshold and scale
     float t0, dt;           // first ray offset and step along the normal (level units)
     int nsteps;

@@ -5,9 +5,9 @@
 // quad's control grid (mc_quad_gen, :1981-2052), remaps it to the chosen
 // level (lod_rowgen, :2262-2274), walks the ray along the normal and reduces
 // the samples (render_pixel, :1150-1281: trilinear or nearest; none, min,
-// mean, max, alpha or stddev; the trilinear walk takes groups of four
-// samples -- the mean adds each group as ((v0+v1)+v2)+v3 -- then single
-// samples) and writes to_u8 (:793-795).  Every
+// mean, max, alpha, stddev, percentile or depth; the trilinear walk takes
+// groups of four samples -- the mean adds each group as ((v0+v1)+v2)+v3 --
+// then single samples) and writes to_u8 (:793-795).  Every
 // floating-point operation is the reference's, in its order: the library is
 // built without contraction and with correctly rounded division and square
 // root, so each pixel equals the reference renderer's bit for bit.
@@ -55,6 +55,7 @@ __device__ __forceinline__ float lerp3(float c000, float c001, float c010, float
 
 // ---- voxel sources ------------------------------------------------------
 struct DenseSrc {
+    static constexpr bool kTouch = false;
     const uint8_t *v;
     int nz, ny, nx;
     __device__ float voxel(int z, int y, int x) const {
@@ -75,6 +76,7 @@ struct DenseSrc {
 };
 
 struct TableSrc {
+    static constexpr bool kTouch = false;
     TableLevel t;
     int nz, ny, nx;
     __device__ const uint8_t *block(int bz, int by, int bx) const {
@@ -109,6 +111,7 @@ struct TableSrc {
 // bitmap in global memory takes one atomic per distinct block per workgroup.
 constexpr int kTouchSet = 1024;
 struct TouchSrc {
+    static constexpr bool kTouch = true;
     uint32_t *bits;
     TableLevel t;
     int nz, ny, nx;
@@ -211,7 +214,67 @@ __device__ void quad_point(const RenderDesc &d, int row, int j, float *P, float 
 }
 
 // ---- one pixel (src/mc_render.c:1150-1281) --------------------------------
+// The value of rank k (0-based, ascending) among a[0..n-1]: a quickselect
+// with Lomuto partitions around the last element (a is permuted).
+__device__ inline float select_rank(float *a, int n, int k) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const float piv = a[hi];
+        int st = lo;
+        for (int i = lo; i < hi; ++i)
+            if (a[i] < piv) { const float t = a[i]; a[i] = a[st]; a[st] = t; st++; }
+        { const float t = a[hi]; a[hi] = a[st]; a[st] = t; }
+        if (k == st) return a[st];
+        if (k < st) hi = st - 1; else lo = st + 1;
+    }
+    return a[k];
+}
+
+// PERCENTILE with the default rank 0.9 (pct_ray, src/mc_render.c:896-914):
+// up to 1024 samples along the ray (every stride-th step for deeper slabs),
+// the value at rank round(0.9 * (n - 1)).  kStore false walks the same
+// samples without keeping them (the touch pass).
+template <bool kStore, class Src>
+__device__ uint8_t percentile_ray(const RenderDesc &d, const Src &s, const float *P, float nz, float ny, float nx) {
+    float buf[kStore ? 1024 : 1];
+    int stride = (d.nsteps + 1023) / 1024;
+    if (stride < 1) stride = 1;
+    const float sz_ = d.dt * nz * stride, sy_ = d.dt * ny * stride, sx_ = d.dt * nx * stride;
+    float pz = P[0] + d.t0 * nz, py = P[1] + d.t0 * ny, px = P[2] + d.t0 * nx;
+    int n = 0;
+    for (int it = 0; it < d.nsteps; it += stride) {
+        const float v = sample(s, pz, py, px, d.filter);
+        if constexpr (kStore) buf[n] = v;
+        n++;
+        pz += sz_; py += sy_; px += sx_;
+    }
+    if constexpr (!kStore) return 0;
+    if (!n) return 0;
+    const int k = (int)(0.9f * (float)(n - 1) + 0.5f);
+    return to_u8(select_rank(buf, n, k));
+}
+
+// DEPTH with the default threshold 0 (depth_ray, src/mc_render.c:917-933):
+// the first step whose value is above 0, as 1 + 254 * step / (nsteps - 1)
+// rounded, or 0 when no step is.
 template <class Src>
+__device__ uint8_t depth_ray(const RenderDesc &d, const Src &s, const float *P, float nz, float ny, float nx) {
+    const float sz_ = d.dt * nz, sy_ = d.dt * ny, sx_ = d.dt * nx;
+    float pz = P[0] + d.t0 * nz, py = P[1] + d.t0 * ny, px = P[2] + d.t0 * nx;
+    for (int it = 0; it < d.nsteps; it++) {
+        const float v = sample(s, pz, py, px, d.filter);
+        if (v * (1.0f / 255.0f) > 0.0f) {
+            const float frac = d.nsteps > 1 ? (float)it / (float)(d.nsteps - 1) : 0.0f;
+            return (uint8_t)(1.0f + 254.0f * frac + 0.5f);
+        }
+        pz += sz_; py += sy_; px += sx_;
+    }
+    return 0;
+}
+
+// kPct: the instantiation that renders PERCENTILE (it keeps the ray's
+// samples in private memory; the other composites need none).
+template <class Src, bool kPct = false>
 __device__ uint8_t render_pixel(const RenderDesc &d, const Src &s, int row, int j) {
     float P[3], N[3];
     quad_point(d, row, j, P, N);
@@ -231,6 +294,12 @@ __device__ uint8_t render_pixel(const RenderDesc &d, const Src &s, int row, int 
         const float nl = 1.0f / sqrtf(n2);
         nz *= nl; ny *= nl; nx *= nl;
     }
+    if (d.comp == 7) {
+        if constexpr (Src::kTouch) return percentile_ray<false>(d, s, P, nz, ny, nx);
+        else if constexpr (kPct) return percentile_ray<true>(d, s, P, nz, ny, nx);
+        else return 0;                          // rendered by the kPct instantiation only
+    }
+    if (d.comp == 8) return depth_ray(d, s, P, nz, ny, nx);
     const float sz_ = d.dt * nz, sy_ = d.dt * ny, sx_ = d.dt * nx;
     float pz = P[0] + d.t0 * nz, py = P[1] + d.t0 * ny, px = P[2] + d.t0 * nx;
     float mn = 255.0f, mx = 0.0f, sum = 0.0f, sum2 = 0.0f, acc = 0.0f, A = 0.0f;
@@ -293,12 +362,12 @@ __device__ uint8_t render_pixel(const RenderDesc &d, const Src &s, int row, int 
     return to_u8(acc);
 }
 
-template <class Src>
+template <class Src, bool kPct>
 __global__ __launch_bounds__(256) void k_render(RenderDesc d, Src s, uint8_t *__restrict__ out) {
     const int64_t pix = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (pix >= (int64_t)d.w * d.h) return;
     const int row = (int)(pix / d.w), j = (int)(pix % d.w);
-    out[pix] = render_pixel(d, s, row, j);
+    out[pix] = render_pixel<Src, kPct>(d, s, row, j);
 }
 
 __global__ __launch_bounds__(256) void k_render_touch(RenderDesc d, TouchSrc s) {
@@ -431,7 +500,7 @@ static int pick_lod(int nlods, float vpp) {
 int render_describe(RenderDesc &d, int nlods, const int32_t *dims, const float *grid_host, const float *grid_dev,
                     int gw, int gh, float x0, float y0, float step, int w, int h, int filter, int comp, float t0,
                     float t1, float dt) {
-    if ((filter != 0 && filter != 1) || comp < 0 || comp > 5) return -1;
+    if ((filter != 0 && filter != 1) || comp < 0 || comp > 8 || comp == 6) return -1;   // no SHADED
     const int L = pick_lod(nlods, step * quad_spacing(grid_host, gw, gh));
     const float inv = 1.0f / (float)(1 << L);
     if (L > 0) { t0 = t0 * inv; t1 = t1 * inv; }       // render_lod (src/mc_render.c:2276-2288)
@@ -461,11 +530,19 @@ static unsigned grid_of(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 void launch_render_dense(const RenderDesc &d, const uint8_t *vol, uint8_t *out, hipStream_t s) {
     DenseSrc src{vol, d.nz, d.ny, d.nx};
-    hipLaunchKernelGGL(k_render<DenseSrc>, dim3(grid_of((int64_t)d.w * d.h)), dim3(256), 0, s, d, src, out);
+    if (d.comp == 7)
+        hipLaunchKernelGGL((k_render<DenseSrc, true>), dim3(grid_of((int64_t)d.w * d.h)), dim3(256), 0, s, d, src, out);
+    else
+        hipLaunchKernelGGL((k_render<DenseSrc, false>), dim3(grid_of((int64_t)d.w * d.h)), dim3(256), 0, s, d, src,
+                           out);
 }
 void launch_render_table(const RenderDesc &d, const TableLevel &t, uint8_t *out, hipStream_t s) {
     TableSrc src{t, d.nz, d.ny, d.nx};
-    hipLaunchKernelGGL(k_render<TableSrc>, dim3(grid_of((int64_t)d.w * d.h)), dim3(256), 0, s, d, src, out);
+    if (d.comp == 7)
+        hipLaunchKernelGGL((k_render<TableSrc, true>), dim3(grid_of((int64_t)d.w * d.h)), dim3(256), 0, s, d, src, out);
+    else
+        hipLaunchKernelGGL((k_render<TableSrc, false>), dim3(grid_of((int64_t)d.w * d.h)), dim3(256), 0, s, d, src,
+                           out);
 }
 void launch_render_touch(const RenderDesc &d, const TableLevel &t, uint32_t *bits, hipStream_t s) {
     TouchSrc src{bits, t, d.nz, d.ny, d.nx, -1, nullptr};

@@ -113,7 +113,7 @@ void     orc_decimate(const uint8_t *src, int dz, int dy, int dx, uint8_t *out);
 
 enum { ORC_FILTER_NEAREST = 0, ORC_FILTER_TRILINEAR = 1 };
 enum { ORC_COMP_NONE = 0, ORC_COMP_MIN = 1, ORC_COMP_MEAN = 2, ORC_COMP_MAX = 3, ORC_COMP_ALPHA = 4,
-       ORC_COMP_STDDEV = 5 };
+       ORC_COMP_STDDEV = 5, ORC_COMP_PERCENTILE = 7, ORC_COMP_DEPTH = 8 };
 
 /* One level of a pyramid: a dense z,y,x raster, or (dense == NULL) 16^3
  * blocks at blk_data + blk_off[(bz*gby + by)*gbx + bx] (-1 = absent, reads 0)
@@ -133,8 +133,8 @@ float    orc_quad_spacing(const float *grid, int gw, int gh);
 /* Ray steps of a [t0, t1] walk at dt (make_cfg, src/mc_render.c:827-835). */
 int      orc_render_steps(float t0, float t1, float dt);
 /* mc_render_quad_lod for filters NEAREST/TRILINEAR and composites
- * NONE/MIN/MEAN/MAX/ALPHA/STDDEV (other render parameters zero); returns the
- * level rendered, or -1. */
+ * NONE/MIN/MEAN/MAX/ALPHA/STDDEV/PERCENTILE/DEPTH (other render parameters
+ * zero: the defaults); returns the level rendered, or -1. */
 int      orc_render_quad_lod(int nlods, const orc_level *levels, const float *grid, int gw, int gh, float x0,
                              float y0, float step, int w, int h, int filter, int comp, float t0, float t1,
                              float dt, uint8_t *out);

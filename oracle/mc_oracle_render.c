@@ -174,7 +174,61 @@ float orc_quad_spacing(const float *grid, int gw, int gh) {
     return n ? (float)(sum / n) : 1.0f;
 }
 
-/* ---- one pixel (src/mc_render.c:1150-1281, composites NONE/MIN/MEAN/MAX) -- */
+/* ---- percentile and depth rays (src/mc_render.c:896-933) ------------------ */
+/* The value of rank k (0-based, ascending) among a[0..n-1]: a quickselect
+ * with Lomuto partitions around the last element (a is permuted). */
+static float select_rank(float *a, int n, int k) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const float piv = a[hi];
+        int st = lo;
+        for (int i = lo; i < hi; ++i)
+            if (a[i] < piv) { const float t = a[i]; a[i] = a[st]; a[st] = t; st++; }
+        { const float t = a[hi]; a[hi] = a[st]; a[st] = t; }
+        if (k == st) return a[st];
+        if (k < st) hi = st - 1; else lo = st + 1;
+    }
+    return a[k];
+}
+
+/* PERCENTILE with the default rank 0.9: up to 1024 samples along the ray
+ * (every stride-th step for deeper slabs), the value at rank
+ * round(0.9 * (n - 1)). */
+static uint8_t percentile_ray(const orc_level *L, const float *P, float nz, float ny, float nx, int filter,
+                              float t0, float dt, int nsteps) {
+    float buf[1024];
+    int stride = (nsteps + 1023) / 1024;
+    if (stride < 1) stride = 1;
+    const float sz_ = dt * nz * stride, sy_ = dt * ny * stride, sx_ = dt * nx * stride;
+    float pz = P[0] + t0 * nz, py = P[1] + t0 * ny, px = P[2] + t0 * nx;
+    int n = 0;
+    for (int it = 0; it < nsteps; it += stride) {
+        buf[n++] = sample(L, pz, py, px, filter);
+        pz += sz_; py += sy_; px += sx_;
+    }
+    if (!n) return 0;
+    const int k = (int)(0.9f * (float)(n - 1) + 0.5f);
+    return to_u8(select_rank(buf, n, k));
+}
+
+/* DEPTH with the default threshold 0: the first step whose value is above
+ * 0, as 1 + 254 * step / (nsteps - 1) rounded, or 0 when no step is. */
+static uint8_t depth_ray(const orc_level *L, const float *P, float nz, float ny, float nx, int filter, float t0,
+                         float dt, int nsteps) {
+    const float sz_ = dt * nz, sy_ = dt * ny, sx_ = dt * nx;
+    float pz = P[0] + t0 * nz, py = P[1] + t0 * ny, px = P[2] + t0 * nx;
+    for (int it = 0; it < nsteps; it++) {
+        const float v = sample(L, pz, py, px, filter);
+        if (v * (1.0f / 255.0f) > 0.0f) {
+            const float frac = nsteps > 1 ? (float)it / (float)(nsteps - 1) : 0.0f;
+            return (uint8_t)(1.0f + 254.0f * frac + 0.5f);
+        }
+        pz += sz_; py += sy_; px += sx_;
+    }
+    return 0;
+}
+
+/* ---- one pixel (src/mc_render.c:1150-1281) ----------------------------------- */
 static uint8_t render_pixel(const orc_level *L, const float *P, const float *Nn, int filter, int comp, float t0,
                             float dt, int nsteps) {
     if (!pt_valid(P)) return 0;
@@ -186,6 +240,8 @@ static uint8_t render_pixel(const orc_level *L, const float *P, const float *Nn,
         const float nl = 1.0f / sqrtf(n2);
         nz *= nl; ny *= nl; nx *= nl;
     }
+    if (comp == ORC_COMP_PERCENTILE) return percentile_ray(L, P, nz, ny, nx, filter, t0, dt, nsteps);
+    if (comp == ORC_COMP_DEPTH) return depth_ray(L, P, nz, ny, nx, filter, t0, dt, nsteps);
     const float sz_ = dt * nz, sy_ = dt * ny, sx_ = dt * nx;
     float pz = P[0] + t0 * nz, py = P[1] + t0 * ny, px = P[2] + t0 * nx;
     /* alpha with the reference's defaults for zero alpha_min / alpha_opacity

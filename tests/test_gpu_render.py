@@ -91,7 +91,7 @@ def _render_cases(levels, n, seed):
         else:
             x0, y0 = float(rng.uniform(-2, gw * 0.5)), float(rng.uniform(-2, gh * 0.5))
         filt = 1 if k % 5 else 0
-        comp = [2, 4, 0, 5, 1, 3, 2, 5, 4, 2][k % 10]
+        comp = [2, 4, 0, 5, 1, 3, 7, 8, 2, 5, 7, 8][k % 12]
         t0, t1, dt = [(-4.0, 4.0, 1.0), (-2.5, 3.0, 0.75), (3.0, -3.0, 0.5), (0.0, 0.0, 1.0),
                       (-7.0, 7.0, 1.0)][k % 5]
         out.append((g, x0, y0, step, w, h, filt, comp, t0, t1, dt))

@@ -112,7 +112,7 @@ def _cases():
             x0 = float((gw - 1) / 2 - w * step / 2 + rng.uniform(-0.5, 0.5))
             y0 = float((gh - 1) / 2 - h * step / 2 + rng.uniform(-0.5, 0.5))
         filt = 1 if k % 5 else 0
-        comp = [2, 4, 5, 0, 1, 3, 2, 4, 2, 5][k % 10]
+        comp = [2, 4, 5, 0, 1, 3, 7, 8, 2, 5, 7, 8][k % 12]
         t0, t1, dt = [(-4.0, 4.0, 1.0), (-2.5, 3.0, 0.75), (3.0, -3.0, 0.5), (0.0, 0.0, 1.0),
                       (-7.0, 7.0, 1.0)][k % 5]
         out.append((levels, g, x0, y0, step, w, h, filt, comp, t0, t1, dt))
