@@ -791,7 +791,7 @@ def main():
         log("[bench] serving leg (child process)")
         serving = child("--serve-only", "--serve-batches", args.serve_batches)
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); "
               "the codec shards by chunk with no cross-GPU exchange", file=sys.stderr)
         sys.exit(2)
 

@@ -802,7 +802,7 @@ void orc_decode_chunk(const uint8_t *blob, uint64_t blob_len, uint8_t *out) {
 }
 
 /* ------------------------------------------------------------------------
- * Multi-threaded CPU baseline drivers.
+ * Multi-threaded CPU baseline runners.
  * ------------------------------------------------------------------------ */
 typedef struct {
     int n; const uint8_t *const *blobs; const uint64_t *lens; uint8_t *out;
