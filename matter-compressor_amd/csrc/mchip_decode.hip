@@ -431,9 +431,19 @@ __device__ __forceinline__ void recon_from_list(const uint2 inf, const uint32_t 
         }
     }
     if (flags & 2) {
-        __syncthreads();
-        if (t == 0)
+        // Corrections in list order (src/mc_codec.c:770-781).  Their
+        // positions are a running sum of gaps; when they strictly increase
+        // (as in every stream the encoder writes) no voxel is corrected
+        // twice, so the threads apply them in parallel; otherwise thread 0
+        // applies them one after another.
+        int dup = 0;
+        for (uint32_t i = t; i + 1 < nc; i += 256) dup |= w[nw + 2 * i] >= w[nw + 2 * i + 2];
+        dup = __syncthreads_or(dup);
+        if (!dup) {
+            for (uint32_t i = t; i < nc; i += 256) apply_correction(u8t, w[nw + 2 * i], w[nw + 2 * i + 1]);
+        } else if (t == 0) {
             for (uint32_t i = 0; i < nc; ++i) apply_correction(u8t, w[nw + 2 * i], w[nw + 2 * i + 1]);
+        }
     }
     __syncthreads();
 }
