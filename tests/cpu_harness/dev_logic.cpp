@@ -73,6 +73,29 @@ uint32_t dl_encode(const QTab *qt, const uint16_t *band_start, uint32_t dc, uint
     return e.n;
 }
 
+// A block payload with no coefficients (eob 0) and the given corrections:
+// gap[i] (position step, 0 repeats the position), neg[i], mag[i] >= 1, in the
+// bin order of the reference's encoder.  Lets tests build correction lists
+// the encoder itself never writes (repeated positions).
+uint32_t dl_encode_corrections(const QTab *qt, uint32_t dc, uint32_t n, const uint32_t *gap, const int *neg,
+                               const uint32_t *mag, uint8_t *out, uint32_t cap) {
+    RcEnc e;
+    uint32_t stage[kEncStage];
+    e.init((uint32_t *)out, cap, stage, 1);
+    e.bit_once(qt->flag[0], 0);
+    e.bit_once(qt->flag[1], 1);
+    for (int b = 7; b >= 0; --b) e.bit_once(qt->dc[7 - b], (dc >> b) & 1);
+    e.bit_once(qt->eob[0], 0);
+    e.expgolomb(n - 1);
+    for (uint32_t i = 0; i < n; ++i) {
+        e.expgolomb(gap[i]);
+        e.bypass(neg[i]);
+        e.expgolomb(mag[i] - 1);
+    }
+    e.finish();
+    return e.n;
+}
+
 // IDCT of one 16-point line with the device kernel's arithmetic.
 void dl_idct_line(float *v, const float *cmf) { idct16_line(v, cmf); }
 void dl_fdct_line(float *v, const float *cmf) { fdct16_line(v, cmf); }

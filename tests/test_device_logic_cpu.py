@@ -36,6 +36,8 @@ def harness():
     H.dl_decode_levels_full.restype = C.c_uint32
     H.dl_encode.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, C.c_uint32, vp, vp, C.c_uint32]
     H.dl_encode.restype = C.c_uint32
+    H.dl_encode_corrections.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp, vp, vp, C.c_uint32]
+    H.dl_encode_corrections.restype = C.c_uint32
     H.dl_idct_line.argtypes = [vp, vp]
     H.dl_fdct_line.argtypes = [vp, vp]
     H.dl_set_sm.argtypes = [C.c_int]
@@ -206,3 +208,29 @@ def test_device_line_transforms_match_oracle_dct():
                 g[idx] = line
             got = np.moveaxis(g, -1, axis)
         assert np.array_equal(got.ravel().view(np.uint32), want_f.view(np.uint32)), f"forward trial {trial}"
+
+
+def test_crafted_correction_payloads_decode_like_the_reference():
+    """Block payloads with hand-made correction lists (dl_encode_corrections)
+    decode in the oracle exactly as in the reference build: repeated
+    positions apply one after another, each clamped."""
+    if not ob.ref_available():
+        pytest.skip("reference build not present")
+    H = harness()
+    R = ob.ref()
+    for gaps, negs, mags, dc, q in [([5, 0, 0, 3, 0], [0, 0, 1, 1, 0], [200, 100, 150, 90, 7], 128, 6.0),
+                                    ([0, 0, 4095], [1, 0, 0], [1, 300, 2], 3, 1.0)]:
+        t = qtab(q)
+        n = len(gaps)
+        out = np.zeros(4096, np.uint8)
+        k = H.dl_encode_corrections(ob.p(t), dc, n, ob.p(np.array(gaps, np.uint32)), ob.p(np.array(negs, np.int32)),
+                                    ob.p(np.array(mags, np.uint32)), ob.p(out), out.size)
+        pay = out[:k].tobytes()
+        ctx = R.mc_codec_ctx_new()
+        R.mc_codec_ctx_set_quality(ctx, C.c_float(q))
+        want = np.zeros(4096, np.uint8)
+        R.mc_dec_block(ctx, pay, len(pay), ob.p(want))
+        R.mc_codec_ctx_free(ctx)
+        assert np.array_equal(ob.dec_block(pay, q), want)
+        if dc == 128:   # voxel 5: 128 +200 -> 255, +100 -> 255, -150 -> 105; voxel 8: 128 -90 +7 = 45
+            assert want[5] == 105 and want[8] == 45 and want[0] == 128

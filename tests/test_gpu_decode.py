@@ -200,3 +200,57 @@ def test_autotune_keeps_results_exact(gpu_ctx):
             mc_amd.set_tuning(k, v)
         d_in.free()
         d_out.free()
+
+
+def _replace_payloads(blob, repl):
+    """The chunk blob with the payloads of some present blocks replaced
+    (repl: block index -> payload bytes); header, fraction map and bitmap
+    kept, lengths rewritten."""
+    b = np.frombuffer(blob, np.uint8)
+    fml = int(b[12]) | (int(b[13]) << 8)
+    bm = 14 + fml
+    bits = np.unpackbits(b[bm:bm + 512], bitorder="little")
+    present = np.nonzero(bits)[0]
+    lens = b[bm + 512:bm + 512 + 2 * len(present)].view("<u2").astype(np.int64)
+    offs = bm + 512 + 2 * len(present) + np.concatenate([[0], np.cumsum(lens)[:-1]])
+    pays = [repl.get(int(bi), b[offs[k]:offs[k] + lens[k]].tobytes()) for k, bi in enumerate(present)]
+    new_lens = np.array([len(p) for p in pays], "<u2")
+    return bytes(b[:bm + 512]) + new_lens.tobytes() + b"".join(pays)
+
+
+def test_chunk_decode_parallel_and_ordered_corrections(gpu_ctx):
+    """k_recon applies a block's corrections in parallel when their positions
+    strictly increase and in list order otherwise: blocks with 300 spread
+    corrections, with repeated positions whose clamps depend on the order, and
+    with positions that wrap past 2^32 decode exactly as in the oracle."""
+    import ctypes as C
+    from test_device_logic_cpu import harness, qtab
+    import mc_amd
+    H = harness()
+    ch = mc_amd.synth_chunks([77], [0.3])[0]
+    q = 6.0
+    blob = ob.encode_chunk(ch, q, 0)
+    t = qtab(q)
+    rng = np.random.default_rng(21)
+
+    def craft(gaps, negs, mags, dc):
+        out = np.zeros(1 << 16, np.uint8)
+        n = H.dl_encode_corrections(ob.p(t), dc, len(gaps), ob.p(np.array(gaps, np.uint32)),
+                                    ob.p(np.array(negs, np.int32)), ob.p(np.array(mags, np.uint32)), ob.p(out),
+                                    out.size)
+        return out[:n].tobytes()
+
+    bits = np.unpackbits(np.frombuffer(blob, np.uint8)[14 + (blob[12] | (blob[13] << 8)):][:512], bitorder="little")
+    present = [int(i) for i in np.nonzero(bits)[0]]
+    spread = sorted(rng.choice(4096, 300, replace=False))
+    repl = {
+        present[0]: craft([int(spread[0])] + [int(b - a) for a, b in zip(spread, spread[1:])],
+                          list(rng.integers(0, 2, 300)), list(rng.integers(1, 200, 300)), 90),
+        present[1]: craft([5, 0, 0, 3, 0], [0, 0, 1, 1, 0], [200, 100, 150, 90, 7], 128),
+        present[2]: craft([4000, 0xFFFFFFFF - 3999, 10, 0], [0, 1, 0, 1], [50, 60, 70, 80], 200),
+    }
+    crafted = _replace_payloads(blob, repl)
+    print("decoding a chunk with crafted correction lists", flush=True)
+    out = gpu_ctx.decode_chunks_host([crafted])
+    want = ob.decode_chunk(crafted)
+    assert np.array_equal(out[0], want.reshape(out[0].shape))
