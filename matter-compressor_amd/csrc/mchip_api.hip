@@ -74,7 +74,8 @@ constexpr int kMaxQTabs = 4096;
 struct Tuning {
     int decoder = 2;          // entropy decoder: 0 per scan position, 1 branching state machine,
                               // 2 branch-free lane decoder over an LDS payload window
-    int enc_batch = 64;       // chunks per encode launch sequence (device workspace ~170 MB each)
+    int enc_batch = 0;        // chunks per encode launch sequence (device workspace ~165 MiB each);
+                              // 0: sized from the free device memory (enc_batch_for)
     int persistent = 1;       // batched decode (>= persist_min chunks): persistent entropy launch
     int persist_min = 2;
     int persist_wg = 0;       // workgroups of the persistent launch (0: occupancy x CUs)
@@ -470,7 +471,7 @@ int mchip_set_tuning(const char *key, int value) {
         return 0;
     }
     if (!strcmp(key, "enc_batch")) {
-        if (value < 1 || value > 1024) return fail("enc_batch must be in [1, 1024]");
+        if (value < 0 || value > 1024) return fail("enc_batch must be in [0, 1024] (0: automatic)");
         g_tune.enc_batch = value;
         return 0;
     }
@@ -746,9 +747,24 @@ int mchip_decode_blocks_host(mchip_ctx *c, const uint8_t *const *payloads, const
     return mchip_ctx_sync(c);
 }
 
-// Chunks per encode launch: bounds the workspace (about 160 MiB per chunk of
-// levels, corrections and payload slots).
+// Blocks per launch of the independent-block encode.
 constexpr int kEncBatch = 32;
+
+// Chunks per encode launch sequence.  Each chunk of the sequence needs about
+// 165 MiB of workspace (levels 32 MiB, corrections 64 MiB, payload slots
+// 68 MiB); larger sequences keep the device busy through each chunk's tail
+// (measured on 256 mixed-quality chunks: 26 GB/s at 64 per sequence, 37 at
+// 128, 49 at 256).  Automatic: a quarter of the free device memory, from 16
+// to 256 chunks.
+static int enc_batch_for() {
+    if (g_tune.enc_batch > 0) return g_tune.enc_batch;
+    constexpr size_t kPerChunk = (size_t)kNVox * kNVox * (sizeof(int16_t) + sizeof(uint32_t)) +
+                                 (size_t)kNVox * kSlotBytes + (size_t)kNVox * 32 + kFmapStride;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 64;
+    size_t eb = free_b / 4 / kPerChunk;
+    return (int)(eb < 16 ? 16 : (eb > 256 ? 256 : eb));
+}
 
 int mchip_encode_chunks(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride, const float *q, const int *tau,
                         const float *q_store, int n, uint8_t *d_out, const uint64_t *out_off,
@@ -757,7 +773,7 @@ int mchip_encode_chunks(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride, c
     if (n <= 0) return 0;
     if (!d_in || !q || !d_out || !out_off || !out_cap || !blob_len) return fail("null argument");
     if (in_stride % 16 || ((uintptr_t)d_in) % 16) return fail("input rasters must be 16-byte aligned");
-    const int eb = g_tune.enc_batch;
+    const int eb = enc_batch_for();
     for (int b0 = 0; b0 < n; b0 += eb) {
         const int m = n - b0 < eb ? n - b0 : eb;
         void *h = nullptr;
@@ -840,7 +856,7 @@ int mchip_encode_chunks_host(mchip_ctx *c, const uint8_t *chunks, const float *q
         tot += (caps[i] + 255) & ~255ull;
     }
     std::vector<int> st(n, 0);
-    const int eb = g_tune.enc_batch;
+    const int eb = enc_batch_for();
     for (int b0 = 0; b0 < n; b0 += eb) {
         const int m = n - b0 < eb ? n - b0 : eb;
         uint64_t base = off[b0], span = 0;
