@@ -541,16 +541,19 @@ struct Prefetch4 {
         cur = ngrp > 0 ? p[0] : make_uint4(0, 0, 0, 0);
         nxt = ngrp > 1 ? p[1] : make_uint4(0, 0, 0, 0);
     }
-    // word i; calls must come in non-decreasing i
+    // word i; each call's i is the previous call's i or the one after it.
+    // The group only ever steps to nxt: a "nxt or p[g]" choice lets the
+    // compiler load through a selected pointer, which puts the whole reader
+    // in scratch memory and a scratch round trip on every coded bin.
     __device__ __forceinline__ uint32_t get(uint32_t i) {
-        const uint32_t g = i >> 2;
-        if (g != grp) {
-            cur = g == grp + 1 ? nxt : p[g];
-            grp = g;
-            nxt = g + 1 < ngrp ? p[g + 1] : make_uint4(0, 0, 0, 0);
+        if ((i >> 2) != grp) {
+            cur = nxt;
+            grp++;
+            nxt = make_uint4(0, 0, 0, 0);
+            if (grp + 1 < ngrp) nxt = p[grp + 1];
         }
-        const uint32_t k = i & 3;
-        return k == 0 ? cur.x : k == 1 ? cur.y : k == 2 ? cur.z : cur.w;
+        const uint32_t lo = (i & 1) ? cur.y : cur.x, hi = (i & 1) ? cur.w : cur.z;
+        return (i & 2) ? hi : lo;
     }
 };
 

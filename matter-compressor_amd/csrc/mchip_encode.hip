@@ -253,13 +253,23 @@ __device__ void encode_fracmap_wg(const uint4 *__restrict__ ei, uint32_t *__rest
     if (threadIdx.x != 0) return;
     RcEnc e;
     e.init(fm + 1, kFmapCap);
-    uint32_t cx[3][4];
-    for (int b = 0; b < 3; ++b)
-        for (int i = 0; i < 4; ++i) cx[b][i] = 2048;
+    // the three context sets (previous nibble 0, 1-14, 15) stay in registers:
+    // the bin picks its set by selects, never by an indexed (scratch) array
+    uint32_t c0[4], c1[4], c2[4];
+    #pragma unroll
+    for (int b = 0; b < 4; ++b) c0[b] = c1[b] = c2[b] = 2048;
     int prev = 0;
     for (int i = 0; i < kNVox; ++i) {
-        const int v = frac[i], pb = prev == 0 ? 0 : (prev == 15 ? 2 : 1);
-        for (int b = 3; b >= 0; --b) e.bit(cx[pb][b], (v >> b) & 1);
+        const int v = frac[i];
+        const bool z = prev == 0, f = prev == 15;
+        #pragma unroll
+        for (int b = 3; b >= 0; --b) {
+            uint32_t p = z ? c0[b] : f ? c2[b] : c1[b];
+            e.bit(p, (v >> b) & 1);
+            c0[b] = z ? p : c0[b];
+            c2[b] = f ? p : c2[b];
+            c1[b] = (z || f) ? c1[b] : p;
+        }
         prev = v;
     }
     e.finish();
@@ -278,12 +288,12 @@ __global__ __launch_bounds__(64) void k_enc_entropy(const EncChunkDesc *__restri
     __shared__ uint32_t stage[kEncStage * 64];
     __shared__ QTab sq;
     __shared__ uint16_t sband[9];
-    if (!kBlocks && blockIdx.x == kNVox / 64) {   // one extra workgroup per chunk: the fraction map
+    if (!kBlocks && blockIdx.x == 0) {   // one extra workgroup per chunk, dispatched first: the fraction map
         encode_fracmap_wg(einfo + (size_t)blockIdx.y * kNVox, fmapbuf + (size_t)blockIdx.y * (kFmapStride / 4),
                           stage);
         return;
     }
-    const int bi = blockIdx.x * 64 + threadIdx.x;
+    const int bi = (kBlocks ? blockIdx.x : blockIdx.x - 1) * 64 + threadIdx.x;
     const QTab *qt;
     if (!kBlocks) {   // one chunk per workgroup: its priors and band starts into LDS
         const uint32_t *src = (const uint32_t *)(qtabs + cd[blockIdx.y].qtab);
