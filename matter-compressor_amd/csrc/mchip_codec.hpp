@@ -438,8 +438,11 @@ __device__ __forceinline__ uint32_t decode_coefs_lane(LaneCoder &d, const QTab *
             dn = dn < 3u ? dn : 3u;
             slot = st == 0 ? dn : (st + 3u < 16u ? st + 3u : 16u);
             // a distinct (volatile) read keeps the compiler from merging it
-            // with the hot path's read and sinking both to the loop top
-            p0 = *(volatile uint32_t *)(ctx + slot * kLaneStride);
+            // with the hot path's read and sinking both to the loop top; the
+            // contexts are in LDS (every caller passes its __shared__ array),
+            // and naming that address space makes the read a ds_read, where a
+            // volatile generic pointer became a flat load waiting on vmcnt
+            p0 = *(volatile __attribute__((address_space(3))) uint32_t *)(ctx + slot * kLaneStride);
         }
     }
     return sink.count();
