@@ -67,13 +67,17 @@ __device__ __forceinline__ uint64_t xxh64_wave(const uint8_t *p, uint64_t len, u
     const int lane = threadIdx.x & 63;
     const uint64_t nstripe = len / 32;
     const uint64_t ntiles = (nstripe + kHashTile - 1) / kHashTile;
-    const uint32_t *wb = (const uint32_t *)((uintptr_t)p & ~(uintptr_t)3);
+    // global address space: through a generic pointer the tile loads are flat
+    // loads, which also count against lgkmcnt, so the chains' waits on their
+    // LDS reads would wait for the next tile's loads too
+    typedef const __attribute__((address_space(1))) uint32_t gword;
+    gword *wb = (gword *)((uintptr_t)p & ~(uintptr_t)3);
     const uint32_t sh = (uint32_t)((uintptr_t)p & 3) * 8;
     uint64_t v = lane == 0 ? seed + XP1 + XP2 : lane == 1 ? seed + XP2 : lane == 2 ? seed : seed - XP1;
 
     uint32_t r0[kHashPerLane], r1[kHashPerLane], r2[kHashPerLane];
     auto load_tile = [&](uint64_t t) {
-        const uint32_t *b = wb + 2 * t * kHashWords + 2 * lane;
+        gword *b = wb + 2 * t * kHashWords + 2 * lane;
         if ((t + 1) * kHashTile <= nstripe) {            // full tile: every word is in range
             #pragma unroll
             for (int i = 0; i < kHashPerLane; ++i) {
