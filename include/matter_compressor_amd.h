@@ -61,6 +61,9 @@ int   mc_enc_block(mc_codec_ctx *ctx, const mc_u8 *vox, mc_buf *out, uint32_t *l
 void  mc_dec_block(mc_codec_ctx *ctx, const mc_u8 *payload, uint32_t plen, mc_u8 *dst);
 /* decode by archive block codec id (CABAC only; src/mc_codec.h:20) */
 void  mc_dec_block_codec(mc_codec_ctx *ctx, uint32_t codec, const mc_u8 *payload, uint32_t plen, mc_u8 *dst);
+/* decode-side deblocking of a decoded nz x ny x nx volume in place; runs on
+ * the GPU (:106) */
+void  mc_deblock(mc_u8 *vol, int nz, int ny, int nx, float quality);
 /* per-chunk material-fraction map (:117-118) */
 uint32_t mc_enc_fracmap(const mc_u8 *frac, mc_u8 *out, size_t cap);
 void     mc_dec_fracmap(const mc_u8 *in, uint32_t len, mc_u8 *frac);

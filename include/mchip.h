@@ -176,6 +176,15 @@ int mchip_render_quad(mchip_ctx *ctx, int nlods, const uint8_t *const *d_levels,
 int mchip_decimate(mchip_ctx *ctx, const uint8_t *d_src, int dz, int dy, int dx, uint8_t *d_out);
 int mchip_decimate_chunks(mchip_ctx *ctx, const uint8_t *const *d_src, uint8_t *d_out);
 
+/* Decode-side deblocking (mc_deblock, src/mc_codec.c:561-595) of a dense
+ * nz x ny x nx z,y,x raster in device memory, in place, on the context's
+ * stream (asynchronous).  A face whose far neighbour lies past the end of the
+ * raster reads that neighbour as 0. */
+int mchip_deblock(mchip_ctx *ctx, uint8_t *d_vol, int nz, int ny, int nx, float quality);
+/* The same for a raster in host memory (uploaded, filtered, copied back;
+ * synchronous). */
+int mchip_deblock_host(mchip_ctx *ctx, uint8_t *vol, int nz, int ny, int nx, float quality);
+
 /* Kernel timing: when enabled, every kernel launched on the context is
  * bracketed by HIP events on the context's stream; stats accumulate per
  * kernel name ("index", "entropy", "recon", ...). */

@@ -68,27 +68,47 @@ const uint16_t kHiHead[8][12] = {
 };
 const int kHiN[8] = {12, 12, 11, 8, 4, 2, 4, 8};
 
-constexpr int kMaxQTabs = 4096;
+// Quality tables are never overwritten: a slot, once handed out, keeps its
+// table for the life of the process, because launches in flight, captured
+// graphs and serving stores hold slot numbers.  The device array grows by
+// doubling (the replaced array stays allocated for the same reason).
+constexpr uint32_t kInitQTabs = 4096;
+constexpr uint32_t kMaxQTabs = 1u << 20;     // 504 MiB of tables
 
-// Launch-shape tuning knobs (mchip_set_tuning); process-wide.
+// Launch-shape tuning knobs (mchip_set_tuning); process-wide, read as one
+// snapshot per call (tuning()).
 struct Tuning {
-    int decoder = 2;          // entropy decoder: 0 per scan position, 1 branching state machine,
-                              // 2 branch-free lane decoder over an LDS payload window
+    int decoder = 2;          // entropy decoder: 0 per scan position, 2 branch-free lane decoder
+                              // over an LDS payload window
     int enc_batch = 0;        // chunks per encode launch sequence (device workspace ~165 MiB each);
                               // 0: sized from the free device memory (enc_batch_for)
     int persistent = 1;       // batched decode (>= persist_min chunks): persistent entropy launch
     int persist_min = 2;
     int persist_wg = 0;       // workgroups of the persistent launch (0: occupancy x CUs)
 };
+std::mutex g_tune_mu;
 Tuning g_tune;
+Tuning tuning() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    return g_tune;
+}
+
+// One prior configuration: the q=1 and q=12 endpoint tables.
+struct PriorSet {
+    uint16_t lo[8][32], hi[8][32];
+};
 
 struct Global {
     std::mutex mu;
     int device = -1;
     bool ready = false;
-    uint16_t plo[8][32], phi[8][32];
-    std::map<uint32_t, uint32_t> qidx;      // sanitised-q bit pattern -> slot
-    QTab *d_qtabs = nullptr;
+    std::vector<PriorSet> priors;           // every configuration seen; [0] = trained defaults
+    uint32_t cur = 0;                       // index of the process priors in priors[]
+    std::map<uint64_t, uint32_t> qidx;      // (prior set << 32 | sanitised-q bits) -> slot
+    std::vector<QTab> tabs;                 // host copy of every slot handed out
+    std::atomic<QTab *> d_qtabs{nullptr};   // device array of cap slots
+    uint32_t cap = 0;
+    std::vector<QTab *> retired;            // replaced device arrays (still read by older launches)
 };
 Global &G() {
     static Global g;
@@ -136,7 +156,10 @@ QTab make_qtab(float q, const uint16_t lo[8][32], const uint16_t hi[8][32]) {
 }
 
 int init_locked(Global &g, int device) {
-    if (g.ready && (device < 0 || device == g.device)) return 0;
+    if (g.ready) {
+        if (device < 0 || device == g.device) return 0;
+        return fail("the library already runs on device %d (one device per process)", g.device);
+    }
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
     if (ndev <= 0) return fail("no HIP device");
@@ -161,35 +184,54 @@ int init_locked(Global &g, int device) {
     band_start[8] = kNVox;
     for (int i = 0; i < kNVox; ++i) scan[count[(i >> 8) + ((i >> 4) & 15) + (i & 15)]++] = (uint16_t)i;
     HIPCHK(upload_constants(cmf, band_start, scan));
-    if (!g.d_qtabs) HIPCHK(hipMalloc((void **)&g.d_qtabs, sizeof(QTab) * kMaxQTabs));
-    default_priors(g.plo, g.phi);
-    g.qidx.clear();
+    QTab *d = nullptr;
+    HIPCHK(hipMalloc((void **)&d, sizeof(QTab) * kInitQTabs));
+    g.d_qtabs.store(d, std::memory_order_release);
+    g.cap = kInitQTabs;
+    PriorSet def;
+    default_priors(def.lo, def.hi);
+    g.priors.assign(1, def);
+    g.cur = 0;
     g.device = device;
     g.ready = true;
     return 0;
 }
 
-// Slot of the quality table for q (built and uploaded on first use).
-int qtab_slot(float q, uint32_t *slot) {
+// Slot of the quality table for q under the process priors (built and
+// uploaded on first use).  may_grow = false refuses to allocate a larger
+// device array (callers that must not allocate).
+int qtab_slot(float q, uint32_t *slot, bool may_grow = true) {
     Global &g = G();
     std::lock_guard<std::mutex> lk(g.mu);
     if (init_locked(g, -1)) return -1;
     float qs = sanitize_q(q);
-    uint32_t key;
-    memcpy(&key, &qs, 4);
+    uint32_t bits;
+    memcpy(&bits, &qs, 4);
+    const uint64_t key = ((uint64_t)g.cur << 32) | bits;
     auto it = g.qidx.find(key);
     if (it != g.qidx.end()) { *slot = it->second; return 0; }
-    if ((int)g.qidx.size() >= kMaxQTabs) {
-        HIPCHK(hipDeviceSynchronize());
-        g.qidx.clear();
+    const uint32_t s = (uint32_t)g.tabs.size();
+    if (s >= kMaxQTabs) return fail("more than %u distinct (priors, quality) tables", kMaxQTabs);
+    const QTab t = make_qtab(qs, g.priors[g.cur].lo, g.priors[g.cur].hi);
+    if (s >= g.cap) {
+        if (!may_grow) return fail("the quality-table array is full (%u tables) and this call may not allocate", g.cap);
+        const uint32_t ncap = std::min(kMaxQTabs, 2 * g.cap);
+        QTab *nd = nullptr;
+        HIPCHK(hipMalloc((void **)&nd, sizeof(QTab) * ncap));
+        HIPCHK(hipMemcpy(nd, g.tabs.data(), sizeof(QTab) * s, hipMemcpyHostToDevice));
+        g.retired.push_back(g.d_qtabs.load(std::memory_order_relaxed));
+        g.d_qtabs.store(nd, std::memory_order_release);
+        g.cap = ncap;
     }
-    uint32_t s = (uint32_t)g.qidx.size();
-    QTab t = make_qtab(qs, g.plo, g.phi);
-    HIPCHK(hipMemcpy(g.d_qtabs + s, &t, sizeof t, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(g.d_qtabs.load(std::memory_order_relaxed) + s, &t, sizeof t, hipMemcpyHostToDevice));
+    g.tabs.push_back(t);
     g.qidx[key] = s;
     *slot = s;
     return 0;
 }
+
+// The device table array; holds every slot handed out before the call.
+const QTab *qtabs_now() { return G().d_qtabs.load(std::memory_order_acquire); }
 
 }  // namespace
 
@@ -204,8 +246,8 @@ int rt_fail(const char *fmt, ...) {
     return -1;
 }
 int rt_qtab_slot(float q, uint32_t *slot) { return qtab_slot(q, slot); }
-const QTab *rt_qtabs() { return G().d_qtabs; }
-int rt_decoder() { return g_tune.decoder; }
+const QTab *rt_qtabs() { return qtabs_now(); }
+int rt_decoder() { return tuning().decoder; }
 }  // namespace mchip
 
 // ---------------------------------------------------------------------------
@@ -342,15 +384,16 @@ int collect_timing(mchip_ctx *c) {
 // Workgroups of the persistent entropy launch: as many as the device keeps
 // resident at once (occupancy per CU x CUs).
 int persistent_grid() {
-    static int cached = 0;
-    if (cached) return cached;
+    static std::atomic<int> cached{0};
+    const int have = cached.load(std::memory_order_relaxed);
+    if (have) return have;
     int dev = 0, cus = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 1024;
     if (occupancy_persist(&per) != hipSuccess || per <= 0) per = 4;
-    cached = cus * per;
-    return cached;
+    cached.store(cus * per, std::memory_order_relaxed);
+    return cus * per;
 }
 
 }  // namespace
@@ -358,7 +401,8 @@ int persistent_grid() {
 // Descriptors, workspace and launch arguments of a decode of n resident
 // chunk blobs (everything but the launches).
 static int prepare_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *blob_off, const uint64_t *blob_len,
-                   const float *q, int n, uint8_t *d_out, uint64_t out_stride, DecodeArgs &a) {
+                          const float *q, int n, uint8_t *d_out, uint64_t out_stride, const Tuning &tn,
+                          DecodeArgs &a) {
     if (!c) return fail("null context");
     if (n <= 0) return 0;
     if (!d_blobs || !blob_off || !blob_len || !q || !d_out) return fail("null argument");
@@ -406,7 +450,7 @@ static int prepare_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
     }
     a.blobs = d_blobs;
     a.cd = dref;
-    a.qtabs = G().d_qtabs;
+    a.qtabs = qtabs_now();
     a.rec = (BlkRec *)c->d_rec;
     a.plist = (uint16_t *)c->d_plist;
     a.cinfo = (ChunkInfo *)c->d_cinfo;
@@ -414,8 +458,8 @@ static int prepare_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
     a.binfo = (uint2 *)c->d_binfo;
     a.out = d_out;
     a.n = n;
-    a.decoder = g_tune.decoder;
-    a.persistent = g_tune.persistent && n >= g_tune.persist_min && a.decoder == 2;
+    a.decoder = tn.decoder;
+    a.persistent = tn.persistent && n >= tn.persist_min && a.decoder == 2;
     a.nwg = 0;
     a.order = nullptr;
     a.pctl = nullptr;
@@ -423,7 +467,7 @@ static int prepare_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
         if (ensure(c, &c->d_order, &c->cap_order, sizeof(uint32_t) * ((size_t)n * 64 + 64))) return -1;
         a.order = (uint32_t *)c->d_order;
         a.pctl = a.order + (size_t)n * 64;
-        a.nwg = g_tune.persist_wg ? g_tune.persist_wg : persistent_grid();
+        a.nwg = tn.persist_wg > 0 ? tn.persist_wg : persistent_grid();
         if (a.nwg > n * 64) a.nwg = n * 64;
     }
     return 0;
@@ -450,19 +494,26 @@ int mchip_set_priors(const uint16_t *plo, const uint16_t *phi) {
     Global &g = G();
     std::lock_guard<std::mutex> lk(g.mu);
     if (init_locked(g, -1)) return -1;
-    HIPCHK(hipDeviceSynchronize());
+    PriorSet ps;
     if (plo && phi) {
-        memcpy(g.plo, plo, sizeof g.plo);
-        memcpy(g.phi, phi, sizeof g.phi);
+        memcpy(ps.lo, plo, sizeof ps.lo);
+        memcpy(ps.hi, phi, sizeof ps.hi);
     } else {
-        default_priors(g.plo, g.phi);
+        default_priors(ps.lo, ps.hi);
     }
-    g.qidx.clear();
+    // a configuration seen before keeps its index (and so its quality-table
+    // slots); the tables of other configurations stay valid for their holders
+    uint32_t id = (uint32_t)g.priors.size();
+    for (uint32_t i = 0; i < g.priors.size(); ++i)
+        if (!memcmp(&g.priors[i], &ps, sizeof ps)) { id = i; break; }
+    if (id == g.priors.size()) g.priors.push_back(ps);
+    g.cur = id;
     return 0;
 }
 
 int mchip_set_tuning(const char *key, int value) {
     if (!key) return fail("null key");
+    std::lock_guard<std::mutex> lk(g_tune_mu);
     if (!strcmp(key, "persistent")) { g_tune.persistent = value != 0; return 0; }
     if (!strcmp(key, "persist_min")) { g_tune.persist_min = value < 1 ? 1 : value; return 0; }
     if (!strcmp(key, "persist_wg")) {
@@ -476,7 +527,7 @@ int mchip_set_tuning(const char *key, int value) {
         return 0;
     }
     if (!strcmp(key, "decoder")) {
-        if (value < 0 || value > 2) return fail("decoder must be 0, 1 or 2");
+        if (value != 0 && value != 2) return fail("decoder must be 0 or 2");
         g_tune.decoder = value;
         return 0;
     }
@@ -564,7 +615,7 @@ int mchip_decode_chunks(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *bl
     if (!c) return fail("null context");
     if (n <= 0) return 0;
     DecodeArgs a;
-    if (prepare_decode(c, d_blobs, blob_off, blob_len, q, n, d_out, out_stride, a)) return -1;
+    if (prepare_decode(c, d_blobs, blob_off, blob_len, q, n, d_out, out_stride, tuning(), a)) return -1;
     launch_decode(c, a);
     HIPCHK(hipGetLastError());
     return 0;
@@ -585,7 +636,8 @@ mchip_graph *mchip_decode_graph_create(mchip_ctx *caller, const uint8_t *d_blobs
     g->stream = caller->stream;
     g->ctx = mchip_ctx_create();
     DecodeArgs a;
-    bool ok = g->ctx && !prepare_decode(g->ctx, d_blobs, blob_off, blob_len, q, n, d_out, out_stride, a) &&
+    bool ok = g->ctx &&
+              !prepare_decode(g->ctx, d_blobs, blob_off, blob_len, q, n, d_out, out_stride, tuning(), a) &&
               hipStreamSynchronize(g->ctx->stream) == hipSuccess;
     if (ok) {
         // the launches only (no copies, no memsets): k_group_order resets the
@@ -626,7 +678,9 @@ void mchip_graph_destroy(mchip_graph *g) {
 int mchip_autotune_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *blob_off, const uint64_t *blob_len,
                           const float *q, int n, uint8_t *d_out, uint64_t out_stride, char *report, size_t cap) {
     if (!c || n <= 0) return fail("mchip_autotune_decode: bad argument");
-    const Tuning saved = g_tune;
+    // the candidates run with private tunings; the process tuning changes
+    // only at the end, in one step, so concurrent decodes never see a mix
+    const Tuning base = tuning();
     struct Cand { int decoder, persistent, wg_div; };
     std::vector<Cand> cands = {{2, 1, 1}, {2, 1, 2}, {2, 0, 1}, {0, 0, 1}};
     if (n < 2) cands = {{2, 0, 1}, {0, 0, 1}};
@@ -635,19 +689,20 @@ int mchip_autotune_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
     HIPCHK(hipEventCreate(&e1));
     std::string rep = "[";
     float best = 1e30f;
-    Cand pick = cands[0];
+    Tuning pick = base;
     for (size_t k = 0; k < cands.size(); ++k) {
-        g_tune.decoder = cands[k].decoder;
-        g_tune.persistent = cands[k].persistent;
-        g_tune.persist_min = 1;
-        g_tune.persist_wg = cands[k].persistent ? persistent_grid() / cands[k].wg_div : 0;
+        Tuning tn = base;
+        tn.decoder = cands[k].decoder;
+        tn.persistent = cands[k].persistent;
+        tn.persist_min = 1;
+        tn.persist_wg = cands[k].persistent ? persistent_grid() / cands[k].wg_div : 0;
         float t = 1e30f;
         for (int rep_i = 0; rep_i < 2; ++rep_i) {                // best of two runs
+            DecodeArgs a;
             HIPCHK(hipEventRecord(e0, c->stream));
-            if (mchip_decode_chunks(c, d_blobs, blob_off, blob_len, q, n, d_out, out_stride)) {
-                g_tune = saved;
-                return -1;
-            }
+            if (prepare_decode(c, d_blobs, blob_off, blob_len, q, n, d_out, out_stride, tn, a)) return -1;
+            launch_decode(c, a);
+            HIPCHK(hipGetLastError());
             HIPCHK(hipEventRecord(e1, c->stream));
             HIPCHK(hipEventSynchronize(e1));
             float ms = 0.0f;
@@ -656,19 +711,25 @@ int mchip_autotune_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
         }
         char buf[160];
         snprintf(buf, sizeof buf, "%s{\"decoder\": %d, \"persistent\": %d, \"persist_wg\": %d, \"ms\": %.3f}",
-                 k ? ", " : "", cands[k].decoder, cands[k].persistent, g_tune.persist_wg, t);
+                 k ? ", " : "", tn.decoder, tn.persistent, tn.persist_wg, t);
         rep += buf;
-        if (t < best) { best = t; pick = cands[k]; }
+        if (t < best) {
+            best = t;
+            pick = tn;
+            pick.persist_min = n < base.persist_min ? n : base.persist_min;
+        }
     }
     rep += "]";
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    g_tune = saved;
-    g_tune.decoder = pick.decoder;
-    g_tune.persistent = pick.persistent;
-    if (pick.persistent) {
-        g_tune.persist_min = n < saved.persist_min ? n : saved.persist_min;
-        g_tune.persist_wg = persistent_grid() / pick.wg_div;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune.decoder = pick.decoder;
+        g_tune.persistent = pick.persistent;
+        if (pick.persistent) {
+            g_tune.persist_min = pick.persist_min;
+            g_tune.persist_wg = pick.persist_wg;
+        }
     }
     if (report && cap) {
         snprintf(report, cap, "%s", rep.c_str());
@@ -740,7 +801,7 @@ int mchip_decode_blocks_host(mchip_ctx *c, const uint8_t *const *payloads, const
     const uint32_t *dlen = (const uint32_t *)(doff + n);
     const uint32_t *dq = dlen + n;
     timed(c, "block_single", [&] {
-        launch_block_single(din, doff, dlen, dq, G().d_qtabs, (uint8_t *)c->d_stage_out, n, c->stream);
+        launch_block_single(din, doff, dlen, dq, qtabs_now(), (uint8_t *)c->d_stage_out, n, c->stream);
     });
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, c->d_stage_out, (size_t)n * kNVox, hipMemcpyDeviceToHost, c->stream));
@@ -757,7 +818,8 @@ constexpr int kEncBatch = 32;
 // 128, 49 at 256).  Automatic: a quarter of the free device memory, from 16
 // to 256 chunks.
 static int enc_batch_for() {
-    if (g_tune.enc_batch > 0) return g_tune.enc_batch;
+    const int fixed = tuning().enc_batch;
+    if (fixed > 0) return fixed;
     constexpr size_t kPerChunk = (size_t)kNVox * kNVox * (sizeof(int16_t) + sizeof(uint32_t)) +
                                  (size_t)kNVox * kSlotBytes + (size_t)kNVox * 32 + kFmapStride;
     size_t free_b = 0, total_b = 0;
@@ -814,7 +876,7 @@ int mchip_encode_chunks(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride, c
         EncodeArgs a;
         a.in = d_in + (size_t)b0 * in_stride;
         a.cd = (const EncChunkDesc *)c->d_ecd;
-        a.qtabs = G().d_qtabs;
+        a.qtabs = qtabs_now();
         a.einfo = (uint4 *)c->d_einfo;
         a.levels = (int16_t *)c->d_levels;
         a.corr = (uint32_t *)c->d_corr;
@@ -915,7 +977,7 @@ int mchip_encode_blocks_host(mchip_ctx *c, const uint8_t *vox, const float *q, c
         EncodeArgs a{};
         a.in = (const uint8_t *)c->d_stage_in;
         a.cd = (const EncChunkDesc *)c->d_ecd;
-        a.qtabs = G().d_qtabs;
+        a.qtabs = qtabs_now();
         a.einfo = (uint4 *)c->d_einfo;
         a.levels = (int16_t *)c->d_levels;
         a.corr = (uint32_t *)c->d_corr;
@@ -963,8 +1025,8 @@ int mchip_decode_block_jobs(mchip_ctx *c, const uint8_t *d_pay, const uint64_t *
     HIPCHK(hipEventRecord(c->ev_desc, c->stream));
     c->desc_pending = true;
     timed(c, "block_jobs", [&] {
-        launch_jobs(d_pay, (const BlockJob *)c->d_jobs, n, G().d_qtabs, (uint32_t *)c->d_jscratch,
-                    (uint2 *)c->d_jinfo, d_arena, g_tune.decoder, c->stream);
+        launch_jobs(d_pay, (const BlockJob *)c->d_jobs, n, qtabs_now(), (uint32_t *)c->d_jscratch,
+                    (uint2 *)c->d_jinfo, d_arena, tuning().decoder, c->stream);
     });
     HIPCHK(hipGetLastError());
     return 0;
@@ -1228,6 +1290,25 @@ int mchip_decimate_chunks(mchip_ctx *c, const uint8_t *const *d_src, uint8_t *d_
     timed(c, "decimate", [&] { launch_decimate_group(g, d_out, c->stream); });
     HIPCHK(hipGetLastError());
     return 0;
+}
+
+int mchip_deblock(mchip_ctx *c, uint8_t *d_vol, int nz, int ny, int nx, float quality) {
+    if (!c) return fail("null context");
+    if (!d_vol || nz <= 0 || ny <= 0 || nx <= 0) return fail("mchip_deblock: bad argument");
+    timed(c, "deblock", [&] { launch_deblock(d_vol, nz, ny, nx, quality, c->stream); });
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mchip_deblock_host(mchip_ctx *c, uint8_t *vol, int nz, int ny, int nx, float quality) {
+    if (!c) return fail("null context");
+    if (!vol || nz <= 0 || ny <= 0 || nx <= 0) return fail("mchip_deblock_host: bad argument");
+    const size_t n = (size_t)nz * ny * nx;
+    if (ensure(c, &c->d_stage_out, &c->cap_stage_out, n)) return -1;
+    HIPCHK(hipMemcpyAsync(c->d_stage_out, vol, n, hipMemcpyHostToDevice, c->stream));
+    if (mchip_deblock(c, (uint8_t *)c->d_stage_out, nz, ny, nx, quality)) return -1;
+    HIPCHK(hipMemcpyAsync(vol, c->d_stage_out, n, hipMemcpyDeviceToHost, c->stream));
+    return mchip_ctx_sync(c);
 }
 
 int mchip_ctx_enable_timing(mchip_ctx *c, int on) {

@@ -72,89 +72,6 @@ __device__ __forceinline__ uint32_t decode_coefs(RcDec &d, const QTab *__restric
     return nsig;
 }
 
-// Coefficient decode with one bin per loop iteration.  Each lane walks its
-// own state machine (significance, magnitude ladder, Exp-Golomb prefix and
-// suffix, sign), so a wave spends max(bins) iterations instead of the union
-// of its lanes' per-coefficient work.  The 16 adaptive contexts of the
-// block (4 significance contexts of the current band + 12 magnitude rungs)
-// live at ctx[i * cstride] (LDS on the GPU).  Same bins, same order, same
-// results as decode_coefs (src/mc_codec.c:260-375).
-enum : uint32_t { kPhSig = 0, kPhMag = 1, kPhEgPrefix = 2, kPhEgSuffix = 3, kPhSign = 4, kPhDone = 5 };
-
-template <class Sink>
-__device__ __forceinline__ uint32_t decode_coefs_sm(RcDec &d, const QTab *__restrict__ qt, const uint16_t *band_start,
-                                                    Sink &sink, uint32_t *ctx, uint32_t cstride) {
-    int k0 = 0;
-    #pragma unroll
-    for (int i = 0; i < 13; ++i)
-        if (k0 == i && d.bit_once(qt->eob[i])) k0 = i + 1;
-    uint32_t eob = k0 == 0 ? 0u : (k0 == 1 ? 1u : ((1u << (k0 - 1)) | d.bypass_n(k0 - 1)));
-    if (eob > (uint32_t)kNVox) eob = kNVox;
-    for (int i = 0; i < 12; ++i) ctx[(4 + i) * cstride] = qt->mag[i];
-    uint32_t band = 0, next_band = band_start[1];
-    for (int i = 0; i < 4; ++i) ctx[i * cstride] = qt->sig[i];
-    uint32_t ph = eob ? kPhSig : kPhDone;
-    uint32_t p = 0, hist = 0, k = 0, v = 0, nb = 0, mag = 0, nsig = 0;
-    while (ph != kPhDone) {
-        if (ph == kPhSig) {
-            if (p >= next_band) {
-                band++;
-                next_band = band_start[band + 1];
-                for (int i = 0; i < 4; ++i) ctx[i * cstride] = qt->sig[band * 4 + i];
-            }
-            if (p + 1 == eob) { hist = (hist << 1) | 1u; ph = kPhMag; k = 0; v = 0; }   // implied significant
-        }
-        if (ph == kPhEgSuffix) {
-            const uint32_t x = ((1u << (nb & 31)) | d.bypass_n((int)nb)) - 1u;
-            mag = v + x + 1;
-            ph = kPhSign;
-            continue;
-        }
-        const bool adapt = ph <= kPhMag;
-        uint32_t dens = (uint32_t)__popc(hist & 0xFFFFu);
-        dens = dens < 3u ? dens : 3u;
-        const uint32_t ci = ph == kPhSig ? dens : 4u + k;
-        uint32_t p0 = adapt ? ctx[ci * cstride] : 0u;
-        const uint32_t r0 = adapt ? __umul24(d.range >> 12, p0) : d.range >> 1;
-        const uint32_t b = d.code >= r0;
-        d.code = b ? d.code - r0 : d.code;
-        d.range = (b && adapt) ? d.range - r0 : r0;
-        d.norm();
-        if (adapt) {
-            p0 = b ? p0 - (p0 >> 4) : p0 + ((4096u - p0) >> 4);
-            ctx[ci * cstride] = p0;
-        }
-        if (ph == kPhSig) {
-            hist = (hist << 1) | b;
-            if (b) { ph = kPhMag; k = 0; v = 0; }
-            else if (++p == eob) ph = kPhDone;
-        } else if (ph == kPhMag) {
-            if (k < 11) {
-                if (b) { v++; k++; }
-                else { mag = v + 1; ph = kPhSign; }
-            } else if (!b) {
-                mag = v + 1; ph = kPhSign;
-            } else {
-                ph = kPhEgPrefix; nb = 0;
-            }
-        } else if (ph == kPhEgPrefix) {
-            if (b) {
-                if (++nb == 64) ph = kPhEgSuffix;     // corrupt-input bound, as decode_coefs
-            } else if (nb == 0) {
-                mag = v + 1; ph = kPhSign;
-            } else {
-                ph = kPhEgSuffix;
-            }
-        } else {                                      // sign
-            sink.coef(p, (uint16_t)(b ? 0u - mag : mag));
-            nsig++;
-            ph = ++p == eob ? kPhDone : kPhSig;
-        }
-    }
-    return nsig;
-}
-
-
 // Word list of one block (coefficients as scan position | level << 16, then
 // corrections as pos, delta word pairs).  Words go through a small staging
 // area (LDS on the GPU, lane-interleaved) and reach global memory in 16-byte
@@ -484,19 +401,16 @@ __device__ __forceinline__ uint2 entropy_to_list_lane(const uint8_t *payload, ui
 // Entropy-decode one payload into a (raster index, level) list at w (cap
 // words), corrections appended as (pos, delta) word pairs.  Returns the
 // block record: x = dc | flags << 8 | status << 16, y = nsig | ncorr << 16.
-template <bool kSM>
 __device__ __forceinline__ uint2 entropy_to_list(const uint8_t *payload, uint32_t len, const QTab *__restrict__ qt,
                                                  const uint16_t *band_start, uint32_t *w, uint32_t cap,
-                                                 uint32_t *stage, uint32_t stage_stride, uint32_t *ctx,
-                                                 const uint32_t *win = nullptr, int32_t nwin = 0) {
+                                                 uint32_t *stage, uint32_t stage_stride) {
     RcDec d;
-    d.init(payload, 0, len, win, nwin, (int32_t)stage_stride);
+    d.init(payload, 0, len);
     int dc, flags;
     decode_header(d, qt, dc, flags);
     ListSink sink;
     sink.init(w, cap, stage, stage_stride);
-    const uint32_t nsig = kSM ? decode_coefs_sm(d, qt, band_start, sink, ctx, stage_stride)
-                              : decode_coefs(d, qt, band_start, sink);
+    const uint32_t nsig = decode_coefs(d, qt, band_start, sink);
     uint32_t nc = 0;
     if (flags & 2) {
         uint32_t n = d.expgolomb() + 1, pos = 0;

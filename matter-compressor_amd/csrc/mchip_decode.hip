@@ -134,16 +134,15 @@ __global__ __launch_bounds__(256) void k_index(const uint8_t *__restrict__ blobs
 // ---------------------------------------------------------------------------
 // k_entropy: one lane per present block of one chunk (blockIdx.y = chunk).
 // ---------------------------------------------------------------------------
-// Decoder variants (a tuning knob; all give identical results):
+// Decoder variants (a tuning knob; both give identical results):
 //   kDecPosition  one loop iteration per scan position (mchip_codec.hpp
 //                 decode_coefs), payload read through a prefetched bit buffer;
-//   kDecState     one bin per iteration with branches per phase;
 //   kDecLane      one bin per iteration, branch-free, payload staged in a
 //                 per-lane LDS window (LaneDec).  In SIMT every branch body
 //                 taken by any lane costs the whole wave, so the branch-free
 //                 loop pays one bin body per bin instead of the union of the
 //                 lanes' ladders.
-constexpr int kDecPosition = 0, kDecState = 1, kDecLane = 2;
+constexpr int kDecPosition = 0, kDecLane = 2;
 
 template <int kMode>
 __device__ __forceinline__ uint2 entropy_lane(const uint8_t *pay, uint32_t len, const QTab *qt, const uint16_t *band,
@@ -152,8 +151,7 @@ __device__ __forceinline__ uint2 entropy_lane(const uint8_t *pay, uint32_t len, 
     if constexpr (kMode == kDecLane)
         return entropy_to_list_lane(pay, len, qt, band, w, cap, ctx, win);
     else
-        return entropy_to_list<kMode == kDecState>(pay, len, qt, band, w, cap, stage, 64,
-                                                   kMode == kDecState ? ctx : nullptr, nullptr, 0);
+        return entropy_to_list(pay, len, qt, band, w, cap, stage, 64);
 }
 
 template <int kMode>
@@ -166,7 +164,7 @@ __global__ __launch_bounds__(64) void k_entropy(const uint8_t *__restrict__ blob
                                                 uint32_t *__restrict__ scratch,
                                                 uint2 *__restrict__ binfo) {
     __shared__ uint32_t stage[kMode == kDecLane ? 1 : kStage * 64];
-    __shared__ uint32_t sctx[kMode == kDecPosition ? 1 : 17 * 64];
+    __shared__ uint32_t sctx[kMode == kDecLane ? 17 * 64 : 1];
     __shared__ uint32_t pwin[kMode == kDecLane ? kWinAlloc * 64 : 1];
     __shared__ QTab sq;
     __shared__ uint16_t sband[9];
@@ -517,10 +515,7 @@ void launch_entropy(const DecodeArgs &a, hipStream_t s) {
                            a.cinfo, a.scratch, a.binfo, a.order, a.pctl);
         return;
     }
-    if (a.decoder == kDecState)
-        hipLaunchKernelGGL(k_entropy<kDecState>, dim3(kNVox / 64, a.n), dim3(64), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
-                           a.plist, a.cinfo, a.scratch, a.binfo);
-    else if (a.decoder == kDecLane)
+    if (a.decoder == kDecLane)
         hipLaunchKernelGGL(k_entropy<kDecLane>, dim3(kNVox / 64, a.n), dim3(64), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
                            a.plist, a.cinfo, a.scratch, a.binfo);
     else
@@ -542,7 +537,7 @@ __global__ __launch_bounds__(64) void k_entropy_jobs(const uint8_t *__restrict__
                                                      const QTab *__restrict__ qtabs,
                                                      uint32_t *__restrict__ scratch, uint2 *__restrict__ binfo) {
     __shared__ uint32_t stage[kMode == kDecLane ? 1 : kStage * 64];
-    __shared__ uint32_t sctx[kMode == kDecPosition ? 1 : 17 * 64];
+    __shared__ uint32_t sctx[kMode == kDecLane ? 17 * 64 : 1];
     __shared__ uint32_t pwin[kMode == kDecLane ? kWinAlloc * 64 : 1];
     const int i = blockIdx.x * 64 + threadIdx.x;
     if (i >= n) return;
@@ -572,9 +567,6 @@ void launch_jobs(const uint8_t *base, const BlockJob *jobs, int n, const QTab *q
                  uint2 *binfo, uint8_t *arena, int decoder, hipStream_t s) {
     if (decoder == kDecLane)
         hipLaunchKernelGGL(k_entropy_jobs<kDecLane>, dim3((n + 63) / 64), dim3(64), 0, s, base, jobs, n, qtabs,
-                           scratch, binfo);
-    else if (decoder == kDecState)
-        hipLaunchKernelGGL(k_entropy_jobs<kDecState>, dim3((n + 63) / 64), dim3(64), 0, s, base, jobs, n, qtabs,
                            scratch, binfo);
     else
         hipLaunchKernelGGL(k_entropy_jobs<kDecPosition>, dim3((n + 63) / 64), dim3(64), 0, s, base, jobs, n, qtabs,

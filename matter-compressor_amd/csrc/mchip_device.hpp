@@ -47,8 +47,6 @@ __device__ __forceinline__ int32_t payload_words(const uint8_t *p, uint32_t n) {
 }
 
 struct RcDec {
-    const uint32_t *lwin;    // optional LDS copy of the first lwords words (lane-interleaved)
-    int32_t lwords, lstride;
     const uint32_t *wp;      // aligned word holding payload byte 4*k - lead
     int32_t k;               // index of the word in nextw
     int32_t last;            // index of the last word that holds payload bytes
@@ -62,7 +60,7 @@ struct RcDec {
     // word i with bytes outside [0, len) cleared, first payload byte on top
     __device__ __forceinline__ uint32_t fetch(int32_t i) const {
         const int32_t ii = i > last ? last : i;
-        uint32_t w = ii < lwords ? lwin[ii * lstride] : wp[ii];
+        uint32_t w = wp[ii];
         const int32_t b0 = 4 * i - (int32_t)lead;           // payload index of byte 0 of word i
         uint32_t keep = 0xFFFFFFFFu;
         if (b0 < 0) keep &= 0xFFFFFFFFu << (8 * (-b0));
@@ -76,11 +74,7 @@ struct RcDec {
         k++;
         nextw = fetch(k);
     }
-    // Payload words [0, nw) are also at win[j * stride] (staged in LDS by the
-    // caller); the decode then reads global memory only beyond them.
-    __device__ __forceinline__ void init(const uint8_t *base, uint64_t off, uint32_t n,
-                                         const uint32_t *win = nullptr, int32_t nw = 0, int32_t stride = 1) {
-        lwin = win; lwords = win ? nw : 0; lstride = stride;
+    __device__ __forceinline__ void init(const uint8_t *base, uint64_t off, uint32_t n) {
         const uint64_t a = (uint64_t)(uintptr_t)(base + off);
         wp = (const uint32_t *)(uintptr_t)(a & ~3ull);
         lead = (uint32_t)(a & 3);

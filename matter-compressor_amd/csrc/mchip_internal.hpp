@@ -177,6 +177,10 @@ struct ChunkGroup {
 };
 void launch_decimate_group(const ChunkGroup &g, uint8_t *out, hipStream_t s);
 
+// Decode-side deblocking of a dense nz*ny*nx raster in place
+// (src/mc_codec.c:561-595); three passes on stream s.
+void launch_deblock(uint8_t *v, int nz, int ny, int nx, float quality, hipStream_t s);
+
 // Scratch words a chunk needs: 8 per payload byte + 32 per block.
 inline uint64_t scratch_words_for(uint64_t blob_len) { return 8ull * blob_len + 32ull * 4096ull; }
 

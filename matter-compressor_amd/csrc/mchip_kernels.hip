@@ -4,3 +4,4 @@
 #include "mchip_decode.hip"
 #include "mchip_encode.hip"
 #include "mchip_hash.hip"
+#include "mchip_deblock.hip"

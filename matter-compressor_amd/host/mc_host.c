@@ -42,13 +42,6 @@ void mc_dec_block_codec(mc_codec_ctx *c, uint32_t codec, const mc_u8 *p, uint32_
 static pthread_key_t g_dev_key;
 static pthread_once_t g_dev_once = PTHREAD_ONCE_INIT;
 
-/* Calls from several threads overlap on the GPU only as far as the process
- * has HIP hardware queues, one kernel at a time per queue; HIP's default is
- * 4 (one of them held by the null stream).  Unless the process chose a
- * count, ask for 22 (measured best on MI355X: 24 and more queues no longer
- * run concurrently).  The HIP runtime reads the variable at its first call,
- * so this takes effect when no HIP call came before the library was loaded. */
-__attribute__((constructor)) static void default_hw_queues(void) { setenv("GPU_MAX_HW_QUEUES", "22", 0); }
 static void dev_dtor(void *p) { if (p) mchip_ctx_destroy((mchip_ctx *)p); }
 static void dev_key_init(void) { pthread_key_create(&g_dev_key, dev_dtor); }
 
@@ -175,6 +168,14 @@ void mc_dec_block(mc_codec_ctx *c, const mc_u8 *p, uint32_t plen, mc_u8 *dst) {
         mch_dev_error("mc_dec_block");
         memset(dst, 0, 4096);
     }
+}
+
+/* Decode-side deblocking on the GPU (src/mc_codec.c:561-595).  On a device
+ * failure the volume is left unfiltered and the failure is reported. */
+void mc_deblock(mc_u8 *v, int nz, int ny, int nx, float quality) {
+    if (!v || nz <= 0 || ny <= 0 || nx <= 0) return;
+    mchip_ctx *d = mch_dev();
+    if (!d || mchip_deblock_host(d, v, nz, ny, nx, quality) != 0) mch_dev_error("mc_deblock");
 }
 
 /* ------------------------------------------------------------------------
@@ -442,6 +443,12 @@ mc_archive *mc_archive_open_dims(const char *path, int nx, int ny, int nz, float
     uint8_t *base = mmap(NULL, reserve, PROT_READ | PROT_WRITE, MAP_SHARED | MAP_NORESERVE, fd, 0);
     if (base == MAP_FAILED) { perror("mmap"); close(fd); return NULL; }
     mc_archive *a = calloc(1, sizeof *a);
+    if (!a) {
+        fprintf(stderr, "mc_archive_open: out of memory\n");
+        munmap(base, reserve);
+        close(fd);
+        return NULL;
+    }
     a->fd = fd; a->base = base; a->reserve = reserve;
     a->nx = nx; a->ny = ny; a->nz = nz; a->dim = nx; a->quality = quality; a->cov = NULL;
     pthread_mutex_init(&a->grow_mu, NULL);
@@ -481,7 +488,6 @@ mc_archive *mc_archive_open_dims(const char *path, int nx, int ny, int nz, float
 
 void mc_archive_close(mc_archive *a) {
     if (!a) return;
-    mch_region_release(a);
     uint64_t cur = atomic_load(&a->cursor);
     memcpy(a->base + MCH_TOTLEN, &cur, 8);
     msync(a->base, cur, MS_SYNC);
@@ -785,8 +791,9 @@ static const uint8_t *fracmap_of(mc_archive *a, uint64_t co) {
     uint64_t tag; memcpy(&tag, a->base + co + 4, 8);
     pthread_mutex_lock(&a->aux_mu);
     if (!a->fr_cache) {
+        a->fr_cache = calloc(1024, sizeof *a->fr_cache);
+        if (!a->fr_cache) { pthread_mutex_unlock(&a->aux_mu); return NULL; }
         a->fr_cap = 1024;
-        a->fr_cache = calloc(a->fr_cap, sizeof *a->fr_cache);
     }
     uint64_t h = (co * 0x9E3779B97F4A7C15ull) >> 20;
     for (uint32_t p = 0; p < a->fr_cap; ++p) {
@@ -814,6 +821,7 @@ float mc_archive_block_fraction(mc_archive *a, int lod, int bz, int by, int bx) 
     uint16_t fml; memcpy(&fml, a->base + co + 12, 2);
     if (!fml) return 0.0f;
     const uint8_t *fr = fracmap_of(a, co);
+    if (!fr) return 0.0f;
     return (float)fr[((bz & 15) * 16 + (by & 15)) * 16 + (bx & 15)] / 15.0f;
 }
 

@@ -30,8 +30,6 @@ typedef struct {
     uint8_t frac[4096];
 } mch_frac_entry;
 
-struct mch_region;          /* device-resident reader state (mc_host_region.c) */
-
 /* The leading fields keep the order and types of the reference's private
  * struct (src/mc_archive.h:12-25), so that modules compiled against that
  * header (mc_cache.c and mc_volume.c read a->gen) see the same offsets when
@@ -50,10 +48,9 @@ struct mc_archive {
     _Atomic uint64_t *cov;         /* unused here (always NULL) */
     _Atomic uint64_t gen;          /* bumped on every publish */
     int nx, ny, nz;
-    pthread_mutex_t aux_mu;        /* fraction-map cache, region reader */
+    pthread_mutex_t aux_mu;        /* fraction-map cache */
     mch_frac_entry *fr_cache;
     uint32_t fr_cap;
-    struct mch_region *region;
 };
 
 mchip_ctx *mch_dev(void);
@@ -62,6 +59,5 @@ uint64_t mch_xxh64(const void *data, size_t len, uint64_t seed);
 uint64_t mch_blob_len_bounded(const uint8_t *arc, uint64_t off, uint64_t end);
 uint64_t mch_resolve(const uint8_t *arc, uint64_t root, int cz, int cy, int cx, uint64_t end);
 int mch_block_range(mc_archive *a, uint64_t co, int bi, uint64_t *off, uint32_t *len);
-void mch_region_release(mc_archive *a);
 
 #endif

@@ -32,11 +32,12 @@ typedef struct {
 typedef struct {
     serve_chunk *v;
     int n, cap;
+    int oom;                     /* an allocation failed: the list is incomplete */
 } chunk_list;
 
 static void walk(const uint8_t *arc, uint64_t end, uint64_t node, int lv, int cz, int cy, int cx, chunk_list *out) {
     if (!node || node > end || MCH_NODE_BYTES > end - node) return;
-    for (int idx = 0; idx < 4096; ++idx) {
+    for (int idx = 0; idx < 4096 && !out->oom; ++idx) {
         uint64_t child;
         memcpy(&child, arc + node + (uint64_t)idx * 8, 8);
         if (!child) continue;
@@ -45,8 +46,11 @@ static void walk(const uint8_t *arc, uint64_t end, uint64_t node, int lv, int cz
             walk(arc, end, child, lv - 1, z, y, x, out);
         } else if (child > MCH_SLOT_ZERO) {
             if (out->n == out->cap) {
-                out->cap = out->cap ? out->cap * 2 : 256;
-                out->v = realloc(out->v, sizeof *out->v * out->cap);
+                const int nc = out->cap ? out->cap * 2 : 256;
+                serve_chunk *t = realloc(out->v, sizeof *out->v * nc);
+                if (!t) { out->oom = 1; return; }
+                out->v = t;
+                out->cap = nc;
             }
             out->v[out->n++] = (serve_chunk){z, y, x, child};
         }
@@ -75,6 +79,10 @@ mc_serve *mc_serve_open_levels(mc_archive *a, int lod0, int nlods, const int32_t
         uint64_t root;
         memcpy(&root, arc + MCH_ROOTOFF + (uint64_t)(lod0 + l) * 8, 8);
         walk(arc, end, root, 2, 0, 0, 0, &L->cl);
+        if (L->cl.oom) {
+            fprintf(stderr, "mc_serve_open: out of host memory\n");
+            ok = 0;
+        }
         for (int k = 0; k < 3; ++k) { L->lo[k] = 4096; L->hi[k] = -1; }
         for (int i = 0; i < L->cl.n; ++i) {
             const int c[3] = {L->cl.v[i].cz, L->cl.v[i].cy, L->cl.v[i].cx};

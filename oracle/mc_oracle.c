@@ -590,6 +590,44 @@ void orc_dec_block(float q, const uint8_t *p, uint32_t plen, uint8_t dst[NVOX]) 
  * Fraction map (src/mc_codec.c:600-621): 4096 nibbles, 4 adaptive bins each
  * MSB first, context by previous nibble bucket {0, 1..14, 15}, p0 = 2048.
  * ------------------------------------------------------------------------ */
+/* ---- decode-side deblocking (src/mc_codec.c:561-595) ----------------------
+ * Every 16-aligned block face of the raster, X faces first, then Y, then Z,
+ * in place.  A face at coordinate c of a line moves the voxels at c-1 and c
+ * toward each other by a clamped delta when both are material and their step
+ * is below beta; c-2 and c+1 (air replaced by the side's own value) shape the
+ * delta.  The neighbour at c+1 is taken at the flat offset the reference
+ * uses; one past the end of the raster reads as 0. */
+static void orc_db_face(uint8_t *v, size_t n, size_t at, size_t step, int beta, int tc) {
+    const int a = v[at - step], b = v[at];
+    if (a == 0 || b == 0) return;
+    const int d = b - a;
+    if ((d < 0 ? -d : d) >= beta) return;
+    const int far_p = v[at - 2 * step], far_q = at + step < n ? v[at + step] : 0;
+    const int pp = far_p ? far_p : a, qq = far_q ? far_q : b;
+    int delta = (4 * (b - a) + (pp - qq) + 4) >> 3;
+    if (delta > tc) delta = tc;
+    if (delta < -tc) delta = -tc;
+    int na = a + delta, nb = b - delta;
+    na = na < 1 ? 1 : na > 255 ? 255 : na;
+    nb = nb < 1 ? 1 : nb > 255 ? 255 : nb;
+    v[at - step] = (uint8_t)na;
+    v[at] = (uint8_t)nb;
+}
+
+void orc_deblock(uint8_t *v, int nz, int ny, int nx, float quality) {
+    const int beta = (int)(3.0f * quality + 6.0f), tc = (int)(0.5f * quality + 1.0f);
+    const size_t sy = (size_t)nx, sz = (size_t)ny * nx, n = sz * (size_t)nz;
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 16; x < nx; x += 16) orc_db_face(v, n, z * sz + y * sy + x, 1, beta, tc);
+    for (int z = 0; z < nz; ++z)
+        for (int y = 16; y < ny; y += 16)
+            for (int x = 0; x < nx; ++x) orc_db_face(v, n, z * sz + y * sy + x, sy, beta, tc);
+    for (int z = 16; z < nz; z += 16)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) orc_db_face(v, n, z * sz + y * sy + x, sz, beta, tc);
+}
+
 uint32_t orc_enc_fracmap(const uint8_t frac[NVOX], uint8_t *out, size_t cap) {
     renc e; re_begin(&e, out, cap);
     uint16_t cx[3][4];

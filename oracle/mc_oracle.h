@@ -77,6 +77,8 @@ void     orc_dec_block_levels(float q, const uint8_t *p, uint32_t plen, int16_t 
                               uint32_t corr_pos[4096], int32_t corr_delta[4096]);
 
 /* ---- fraction map, hash ---------------------------------------------------- */
+/* decode-side deblocking of a nz*ny*nx raster in place (src/mc_codec.c:561-595) */
+void     orc_deblock(uint8_t *v, int nz, int ny, int nx, float quality);
 uint32_t orc_enc_fracmap(const uint8_t frac[4096], uint8_t *out, size_t cap);
 void     orc_dec_fracmap(const uint8_t *in, uint32_t len, uint8_t frac[4096]);
 uint64_t orc_xxh64(const void *data, size_t len, uint64_t seed);

@@ -10,30 +10,18 @@ extern "C" {
 
 // Entropy-decode a payload with the device decoder; expands the list into
 // raster levels (via scan) and returns corrections.  Returns the status.
-static int use_sm = 1, use_win = 0;
-void dl_set_sm(int on) { use_sm = on; }
-void dl_set_window(int words) { use_win = words; }
+// Variant 2: the lane decoder (one bin per step); 0: one step per position.
+static int variant = 2;
+void dl_set_variant(int v) { variant = v; }
 
 int dl_decode(const QTab *qt, const uint16_t *band_start, const uint16_t *scan, const uint8_t *payload, uint32_t len,
               int16_t *levels, int *dc, int *flags, uint32_t *ncorr, uint32_t *cpos, int32_t *cdel) {
     static thread_local uint32_t w[8 * 17408 + 64 + 3 * 4096 * 2];
     uint32_t cap = 8u * len + 32u;
     uint32_t stage[kStage], ctx[17];
-    // optional window: the first use_win aligned payload words staged separately
-    uint32_t win[64];
-    int32_t nwin = 0;
-    if (use_win) {
-        const uint32_t *aw = (const uint32_t *)((uintptr_t)payload & ~(uintptr_t)3);
-        nwin = payload_words(payload, len);
-        if (nwin > use_win) nwin = use_win;
-        if (nwin > 64) nwin = 64;
-        for (int j = 0; j < nwin; ++j) win[j] = aw[j];
-    }
-    const uint32_t *wp = use_win ? win : nullptr;
     static thread_local uint32_t lwin[kWinAlloc];
-    uint2 r = use_sm == 2 ? entropy_to_list_lane(payload, len, qt, band_start, w, cap, ctx, lwin)
-            : use_sm ? entropy_to_list<true>(payload, len, qt, band_start, w, cap, stage, 1, ctx, wp, nwin)
-                     : entropy_to_list<false>(payload, len, qt, band_start, w, cap, stage, 1, ctx, wp, nwin);
+    uint2 r = variant == 2 ? entropy_to_list_lane(payload, len, qt, band_start, w, cap, ctx, lwin)
+                           : entropy_to_list(payload, len, qt, band_start, w, cap, stage, 1);
     memset(levels, 0, 4096 * sizeof(int16_t));
     *dc = r.x & 255;
     *flags = (r.x >> 8) & 255;

@@ -55,6 +55,7 @@ def oracle():
     O.orc_decode_chunks_mt.argtypes = [C.c_int, vp, vp, vp, C.c_int]
     O.orc_encode_chunks_mt.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, C.c_int]
     O.orc_decimate.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp]
+    O.orc_deblock.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_float]
     O.orc_pick_lod.argtypes = [C.c_int, C.c_float]
     O.orc_pick_lod.restype = C.c_int
     O.orc_quad_spacing.argtypes = [vp, C.c_int, C.c_int]
@@ -208,6 +209,45 @@ def ref_build(vol, q):
     return data
 
 
+def default_priors():
+    """The trained q=1 and q=12 endpoint tables, u16 [8, 32] each: the
+    interpolated tables at q <= 1 (weight 0) and at q >= 2^3.585 (weight 1,
+    src/mc_codec.c:156-167) are the endpoints themselves.  Restores the
+    oracle's default tables first."""
+    oracle().orc_set_priors(None, None)
+    lo = np.zeros((8, 32), np.uint16)
+    hi = np.zeros((8, 32), np.uint16)
+    oracle().orc_priors_for_q(C.c_float(0.5), p(lo))
+    oracle().orc_priors_for_q(C.c_float(16.0), p(hi))
+    return lo, hi
+
+
+def custom_priors(seed=4242, spread=300):
+    """Deterministic non-default endpoint tables: the trained tables moved by
+    up to +-spread (seeded), kept inside [32, 4064]."""
+    lo, hi = default_priors()
+    rng = np.random.default_rng(seed)
+    lo = np.clip(lo.astype(np.int64) + rng.integers(-spread, spread + 1, lo.shape), 32, 4064).astype(np.uint16)
+    hi = np.clip(hi.astype(np.int64) + rng.integers(-spread, spread + 1, hi.shape), 32, 4064).astype(np.uint16)
+    return lo, hi
+
+
+def set_priors(lo=None, hi=None):
+    """Override the oracle's endpoint tables (None restores the defaults)."""
+    if lo is None:
+        oracle().orc_set_priors(None, None)
+    else:
+        a, b = np.ascontiguousarray(lo, np.uint16), np.ascontiguousarray(hi, np.uint16)
+        oracle().orc_set_priors(p(a), p(b))
+
+
+def deblock(vol, q):
+    """The oracle's mc_deblock of a copy of vol (z, y, x)."""
+    v = np.ascontiguousarray(vol, np.uint8).copy()
+    oracle().orc_deblock(p(v), v.shape[0], v.shape[1], v.shape[2], C.c_float(q))
+    return v
+
+
 def ref_available():
     return os.path.exists(REF_SO)
 
@@ -244,6 +284,10 @@ def ref():
     R.mc_archive_sample_boxes.argtypes = [vp, C.c_int, C.c_uint64, C.c_int, C.c_long, C.c_long, C.c_long,
                                           C.c_float, vp]
     R.mc_archive_sample_boxes.restype = C.c_int
+    R.mc_archive_set_priors.argtypes = [vp, vp, vp]
+    R.mc_archive_set_priors.restype = C.c_int
+    R.mc_codec_set_priors.argtypes = [vp, vp]
+    R.mc_deblock.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_float]
     R.mc_codec_init()
     _r = R
     return R

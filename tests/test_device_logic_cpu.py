@@ -40,8 +40,7 @@ def harness():
     H.dl_encode_corrections.restype = C.c_uint32
     H.dl_idct_line.argtypes = [vp, vp]
     H.dl_fdct_line.argtypes = [vp, vp]
-    H.dl_set_sm.argtypes = [C.c_int]
-    H.dl_set_window.argtypes = [C.c_int]
+    H.dl_set_variant.argtypes = [C.c_int]
     _h = H
     return H
 
@@ -115,14 +114,11 @@ def device_decode(payload, q, scan, band_start):
     return lv, dc.value, fl.value, cp[: nc.value].copy(), cd[: nc.value].copy()
 
 
-@pytest.fixture(params=[(2, 0), (1, 0), (0, 0), (1, 7), (0, 3)],
-                ids=["branch_free", "state_machine", "per_position", "state_machine_window", "per_position_window"])
+@pytest.fixture(params=[2, 0], ids=["lane", "per_position"])
 def decoder_variant(request):
-    harness().dl_set_sm(request.param[0])
-    harness().dl_set_window(request.param[1])
+    harness().dl_set_variant(request.param)
     yield request.param
-    harness().dl_set_sm(1)
-    harness().dl_set_window(0)
+    harness().dl_set_variant(2)
 
 
 def test_device_decoder_matches_oracle_on_reference_kat(kat, decoder_variant):
