@@ -73,6 +73,23 @@ int mchip_decode_chunks(mchip_ctx *ctx, const uint8_t *d_blobs, const uint64_t *
                         const uint64_t *blob_len, const float *q, int n,
                         uint8_t *d_out, uint64_t out_stride);
 
+/* The decode of mchip_decode_chunks with no allocation of any kind: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  Its workspace is a fixed device array of 128 MiB that
+ * the library holds on each device it runs on until the process ends; the
+ * quality tables and chunk descriptors travel in the kernel arguments, so
+ * the call issues kernel launches only.  At most 4 chunks per call, and
+ * their workspace (about 0.6 MiB per chunk plus 32 bytes per blob byte, so
+ * three q=6 chunks of the workload) must fit in the array: larger calls are
+ * refused with an error.  mchip_init must have run.  Work is enqueued on
+ * stream (a hipStream_t; NULL = the null stream) and is asynchronous.
+ * Every call on a device shares that device's array, which keeps the last
+ * call's contents: calls that overlap on one device race and give wrong
+ * voxels, so serialise them (one stream, or one call at a time). */
+int mchip_decode_chunks_static(const uint8_t *d_blobs, const uint64_t *blob_off, const uint64_t *blob_len,
+                               const float *q, int n, uint8_t *d_out, uint64_t out_stride, void *stream);
+
 /* Host-memory convenience: upload n blobs, decode, download the rasters to
  * out + i*out_stride, synchronise. */
 int mchip_decode_chunks_host(mchip_ctx *ctx, const uint8_t *const *blobs, const uint64_t *blob_len,

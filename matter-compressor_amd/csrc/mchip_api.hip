@@ -98,10 +98,13 @@ struct PriorSet {
     uint16_t lo[8][32], hi[8][32];
 };
 
+constexpr int kMaxDevices = 64;
+
 struct Global {
     std::mutex mu;
     int device = -1;
     bool ready = false;
+    void *static_ws[kMaxDevices] = {};      // device address of the fixed workspace, per device
     std::vector<PriorSet> priors;           // every configuration seen; [0] = trained defaults
     uint32_t cur = 0;                       // index of the process priors in priors[]
     std::map<uint64_t, uint32_t> qidx;      // (prior set << 32 | sanitised-q bits) -> slot
@@ -184,6 +187,12 @@ int init_locked(Global &g, int device) {
     band_start[8] = kNVox;
     for (int i = 0; i < kNVox; ++i) scan[count[(i >> 8) + ((i >> 4) & 15) + (i & 15)]++] = (uint16_t)i;
     HIPCHK(upload_constants(cmf, band_start, scan));
+    if (device >= kMaxDevices) return fail("device %d: at most %d devices", device, kMaxDevices);
+    if (!g.static_ws[device]) {
+        void *ws = nullptr;
+        HIPCHK(static_scratch_address(&ws));
+        g.static_ws[device] = ws;
+    }
     QTab *d = nullptr;
     HIPCHK(hipMalloc((void **)&d, sizeof(QTab) * kInitQTabs));
     g.d_qtabs.store(d, std::memory_order_release);
@@ -617,6 +626,73 @@ int mchip_decode_chunks(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *bl
     DecodeArgs a;
     if (prepare_decode(c, d_blobs, blob_off, blob_len, q, n, d_out, out_stride, tuning(), a)) return -1;
     launch_decode(c, a);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int mchip_decode_chunks_static(const uint8_t *d_blobs, const uint64_t *blob_off, const uint64_t *blob_len,
+                               const float *q, int n, uint8_t *d_out, uint64_t out_stride, void *stream) {
+    if (n <= 0) return 0;
+    if (n > kInlineDesc) return fail("mchip_decode_chunks_static: %d chunks (at most %d per call)", n, kInlineDesc);
+    if (!d_blobs || !blob_off || !blob_len || !q || !d_out) return fail("null argument");
+    if (out_stride < (uint64_t)16777216) return fail("out_stride %llu < 256^3", (unsigned long long)out_stride);
+    Global &g = G();
+    PriorSet pri;
+    uint8_t *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g.mu);
+        if (!g.ready) return fail("mchip_decode_chunks_static: call mchip_init first");
+        pri = g.priors[g.cur];
+        base = (uint8_t *)g.static_ws[g.device];
+    }
+    if (!base) return fail("mchip_decode_chunks_static: no fixed workspace on device %d", g.device);
+    // workspace layout inside the fixed array, every part 256-byte aligned
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t nb = (size_t)n * kNVox;
+    const size_t o_rec = 0, o_plist = al(o_rec + sizeof(BlkRec) * nb), o_cinfo = al(o_plist + sizeof(uint16_t) * nb),
+                 o_binfo = al(o_cinfo + sizeof(ChunkInfo) * n), o_order = al(o_binfo + sizeof(uint2) * nb),
+                 o_qt = al(o_order + sizeof(uint32_t) * ((size_t)n * 64 + 64)),
+                 o_lists = al(o_qt + sizeof(QTab) * n);
+    DescRef dref{};
+    dref.ptr = nullptr;
+    QTab tabs[kInlineDesc];
+    uint64_t sw = 0;
+    for (int i = 0; i < n; ++i) {
+        if (blob_len[i] > 0xFFFFFFF0ull) return fail("blob %d too long", i);
+        tabs[i] = make_qtab(sanitize_q(q[i]), pri.lo, pri.hi);
+        dref.inl[i] = ChunkDesc{blob_off[i], blob_len[i], (uint64_t)i * out_stride, sw, (uint32_t)i, 0};
+        sw += scratch_words_for(blob_len[i]);
+    }
+    const size_t need = o_lists + sizeof(uint32_t) * sw;
+    if (need > kStaticScratchBytes)
+        return fail("mchip_decode_chunks_static: these blobs need %zu bytes of workspace; the fixed workspace "
+                    "holds %zu", need, kStaticScratchBytes);
+    const Tuning tn = tuning();
+    DecodeArgs a;
+    a.blobs = d_blobs;
+    a.cd = dref;
+    a.qtabs = (const QTab *)(base + o_qt);
+    a.rec = (BlkRec *)(base + o_rec);
+    a.plist = (uint16_t *)(base + o_plist);
+    a.cinfo = (ChunkInfo *)(base + o_cinfo);
+    a.binfo = (uint2 *)(base + o_binfo);
+    a.scratch = (uint32_t *)(base + o_lists);
+    a.out = d_out;
+    a.n = n;
+    a.decoder = tn.decoder;
+    a.persistent = tn.persistent && n >= tn.persist_min && a.decoder == 2;
+    a.order = (uint32_t *)(base + o_order);
+    a.pctl = a.order + (size_t)n * 64;
+    a.nwg = 0;
+    if (a.persistent) {
+        a.nwg = tn.persist_wg > 0 ? tn.persist_wg : persistent_grid();
+        if (a.nwg > n * 64) a.nwg = n * 64;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    launch_put_qtabs(tabs, n, (QTab *)(base + o_qt), s);
+    launch_index(a, s);
+    launch_entropy(a, s);
+    launch_recon(a, s);
     HIPCHK(hipGetLastError());
     return 0;
 }

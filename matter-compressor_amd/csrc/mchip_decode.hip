@@ -654,4 +654,31 @@ void launch_block_single(const uint8_t *base, const uint64_t *off, const uint32_
     hipLaunchKernelGGL(k_block_single, dim3(n), dim3(256), 0, s, base, off, len, qi, qtabs, out);
 }
 
+// ---------------------------------------------------------------------------
+// Fixed device workspace of the decode entry point that allocates nothing
+// (mchip_decode_chunks_static): block records, lane order, block records of
+// the entropy stage, the persistent launch's group order, the call's
+// quality tables and the coefficient lists all live in this array.  Host
+// code takes its device address once per device (static_scratch_address)
+// and passes that address to the kernels like any workspace pointer.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+
+hipError_t static_scratch_address(void **p) { return hipGetSymbolAddress(p, HIP_SYMBOL(scratch)); }
+
+// The quality tables of one call travel in the kernel arguments.
+struct QTabPack {
+    QTab t[kInlineDesc];
+};
+__global__ __launch_bounds__(128) void k_put_qtabs(const QTabPack pack, int n, QTab *__restrict__ dst) {
+    constexpr int kWords = (int)(sizeof(QTab) / 4);
+    for (int i = threadIdx.x; i < n * kWords; i += 128)
+        ((uint32_t *)dst)[i] = ((const uint32_t *)pack.t)[i];
+}
+void launch_put_qtabs(const QTab *tabs, int n, QTab *dst, hipStream_t s) {
+    QTabPack pk;
+    for (int i = 0; i < kInlineDesc; ++i) pk.t[i] = tabs[i < n ? i : 0];
+    hipLaunchKernelGGL(k_put_qtabs, dim3(1), dim3(128), 0, s, pk, n, dst);
+}
+
 }  // namespace mchip

@@ -185,6 +185,13 @@ void launch_deblock(uint8_t *v, int nz, int ny, int nx, float quality, hipStream
 inline uint64_t scratch_words_for(uint64_t blob_len) { return 8ull * blob_len + 32ull * 4096ull; }
 
 hipError_t upload_constants(const float *cmf, const uint16_t *band_start, const uint16_t *scan);
+// The fixed workspace of mchip_decode_chunks_static (a file-scope device
+// array of kStaticScratchBytes in the kernels' translation unit): its device
+// address on the current device, and a launch that writes n (<= kInlineDesc)
+// quality tables passed in the kernel arguments to dst.
+constexpr size_t kStaticScratchBytes = 128ull << 20;
+hipError_t static_scratch_address(void **p);
+void launch_put_qtabs(const QTab *tabs, int n, QTab *dst, hipStream_t s);
 void launch_index(const DecodeArgs &a, hipStream_t s);
 hipError_t occupancy_persist(int *per_cu);   // resident persistent-entropy workgroups per CU
 void launch_entropy(const DecodeArgs &a, hipStream_t s);

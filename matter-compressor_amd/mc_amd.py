@@ -67,6 +67,9 @@ def load():
         "mchip_server_destroy": (None, [vp]),
         "mchip_decode_chunks": (i, [vp, u8p, vp, vp, vp, i, u8p, C.c_uint64]),
         "mchip_decode_chunks_host": (i, [vp, vp, vp, i, u8p, C.c_uint64]),
+        "mchip_decode_chunks_static": (i, [u8p, vp, vp, vp, i, u8p, C.c_uint64, vp]),
+        "mchip_deblock": (i, [vp, vp, i, i, i, C.c_float]),
+        "mchip_deblock_host": (i, [vp, vp, i, i, i, C.c_float]),
         "mchip_decode_blocks_host": (i, [vp, vp, vp, vp, i, u8p]),
         "mchip_encode_chunks": (i, [vp, u8p, C.c_uint64, vp, vp, vp, i, u8p, vp, vp, vp, vp]),
         "mchip_encode_chunks_host": (i, [vp, vp, vp, vp, vp, i, vp, vp, vp, vp]),
