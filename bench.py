@@ -32,7 +32,12 @@ QUEUES = 22
 # leg would otherwise share the hardware's queue slots with the serving
 # lanes and delay their dispatches.
 SERVE_LANES = 8
-os.environ["GPU_MAX_HW_QUEUES"] = str(2 * SERVE_LANES if "--serve-only" in sys.argv else QUEUES)
+# The drop-in leg (--dropin-only) runs at HIP's default queue count, as a
+# program that links the library and sets nothing would.
+if "--dropin-only" in sys.argv:
+    os.environ.pop("GPU_MAX_HW_QUEUES", None)
+else:
+    os.environ["GPU_MAX_HW_QUEUES"] = str(2 * SERVE_LANES if "--serve-only" in sys.argv else QUEUES)
 sys.path.insert(0, os.path.join(ROOT, "matter-compressor_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
@@ -195,20 +200,33 @@ def cpu_reference_baseline(chunks, blobs, man, nthreads):
     return res
 
 
-def dropin_leg(chunks, blobs, nthreads):
+def dropin_leg(nthreads, nchunks=64, nappend=16):
     """The drop-in calls as a caller of the reference's API makes them, with
-    host buffers (so PCIe transfers are included): mc_archive_decode_chunk of
-    every blob from an archive into pageable host memory, from one thread and
-    from nthreads threads (each with its own device context), and
-    mc_archive_append_chunk_par of the given chunks from one thread.  GB/s of
-    decoded (or encoded) voxels."""
+    host buffers (so PCIe transfers are included), in a process at HIP's
+    default hardware-queue count: the plain64_q6 chunks are generated and
+    GPU-encoded (blobs checked against the reference manifest) into an
+    archive; then mc_archive_decode_chunk of every chunk into pageable host
+    memory from nthreads caller threads (merged into batched decodes, and,
+    for comparison, with merging off: each thread's own decode on its own
+    stream) and from one thread, mc_archive_append_chunk_par from one thread,
+    and one chunk per mchip_decode_chunks launch rotated over the streams the
+    default queues allow.  Every decoded chunk is hashed against the
+    manifest.  GB/s of decoded (or encoded) voxels."""
     import tempfile
     import threading
     import shutil
     L = mc_amd.api()
-    n = len(blobs)
+    specs = workloads.WORKLOADS["plain64_q6"]()[:nchunks]
+    man = json.load(open(os.path.join(ROOT, "tests", "golden", "chunk_manifest.json")))["plain64_q6"]
+    n = len(specs)
+    chunks = mc_amd.synth_chunks([s_["seed"] for s_ in specs], [s_["air"] for s_ in specs], nthreads=nthreads)
+    ctx = mc_amd.Ctx(0)
+    blobs = ctx.encode_chunks_host(chunks, [6.0] * n)
+    enc_ok = all(len(b) == m["blob_len"] and xxh64_hex(np.frombuffer(b, np.uint8)) == m["blob_xxh64"]
+                 for b, m in zip(blobs, man))
     tmp = tempfile.mkdtemp(prefix="mcamd_")
-    res = {}
+    res = {"hw_queues": "HIP default (GPU_MAX_HW_QUEUES unset)", "encode_byte_identical": enc_ok}
+    bad = []
     try:
         a = L.mc_archive_open_dims(os.path.join(tmp, "d.mca").encode(), 256 * n, 256, 256, C.c_float(6.0))
         assert a, "archive open failed"
@@ -218,53 +236,122 @@ def dropin_leg(chunks, blobs, nthreads):
         offs = [L.mc_archive_chunk_offset(a, 0, 0, 0, i) for i in range(n)]
         out = np.empty((nthreads, 256, 256, 256), np.uint8)
 
-        def run(t, stride):
+        def run(t, stride, check):
             for i in range(t, n, stride):
                 L.mc_archive_decode_chunk(a, offs[i], out[t].ctypes.data, 0)
+                if check and xxh64_hex(out[t]) != man[i]["decoded_xxh64"]:
+                    bad.append(i)
 
-        # a pool of caller threads that live across the rounds: a thread's
-        # device context is created by its first call and kept until it exits
-        rounds = 2
-        start, done = threading.Barrier(nthreads + 1), threading.Barrier(nthreads + 1)
+        def threaded(rounds=3):
+            # a pool of caller threads that live across the rounds: a thread's
+            # device context is created by its first call and kept until it
+            # exits; the last round hashes every output (outside the timing)
+            start, done = threading.Barrier(nthreads + 1), threading.Barrier(nthreads + 1)
 
-        def worker(t):
-            for _ in range(rounds):
+            def worker(t):
+                for r in range(rounds):
+                    start.wait()
+                    run(t, nthreads, r == rounds - 1)
+                    done.wait()
+
+            pool = [threading.Thread(target=worker, args=(t,)) for t in range(nthreads)]
+            for x in pool:
+                x.start()
+            best = None
+            for r in range(rounds):        # round 0 creates the contexts
+                t0 = time.perf_counter()
                 start.wait()
-                run(t, nthreads)
                 done.wait()
+                dt = time.perf_counter() - t0
+                if 0 < r < rounds - 1:
+                    best = dt if best is None else min(best, dt)
+            for x in pool:
+                x.join()
+            return n * mc_amd.CHUNK_BYTES / 1e9 / best
 
-        pool = [threading.Thread(target=worker, args=(t,)) for t in range(nthreads)]
-        for x in pool:
-            x.start()
-        gb = n * mc_amd.CHUNK_BYTES / 1e9
-        for r in range(rounds):                # round 0 creates the contexts
-            t0 = time.perf_counter()
-            start.wait()
-            done.wait()
-            dt = time.perf_counter() - t0
-        for x in pool:
-            x.join()
-        res[f"decode_chunk_{nthreads}_threads_gbs"] = round(gb / dt, 3)
-        run(0, 1)
+        b0 = C.c_uint64()
+        c0 = C.c_uint64()
+        res[f"decode_chunk_{nthreads}_threads_gbs"] = round(threaded(4), 3)
+        mc_amd.check(mc_amd.load().mchip_merge_stats(C.byref(b0), C.byref(c0)))
+        res["merged_chunks_per_batch"] = round(c0.value / max(1, b0.value), 2)
+        mc_amd.set_tuning("merge", 0)
+        res[f"decode_chunk_{nthreads}_threads_unmerged_gbs"] = round(threaded(3), 3)
+        mc_amd.set_tuning("merge", 1)
+        run(0, 1, True)
         t0 = time.perf_counter()
-        run(0, 1)
-        res["decode_chunk_1_thread_gbs"] = round(gb / (time.perf_counter() - t0), 3)
+        run(0, 1, False)
+        res["decode_chunk_1_thread_gbs"] = round(n * mc_amd.CHUNK_BYTES / 1e9 / (time.perf_counter() - t0), 3)
         L.mc_archive_close(a)
-        e = L.mc_archive_open_dims(os.path.join(tmp, "e.mca").encode(), 256 * len(chunks), 256, 256,
-                                   C.c_float(6.0))
+        e = L.mc_archive_open_dims(os.path.join(tmp, "e.mca").encode(), 256 * nappend, 256, 256, C.c_float(6.0))
         assert e, "archive open failed"
         L.mc_archive_append_chunk_par(e, 0, 0, 0, 0, chunks[0].ctypes.data, C.c_float(6.0), 0)   # warm
         t0 = time.perf_counter()
-        for i in range(len(chunks)):
+        for i in range(nappend):
             L.mc_archive_append_chunk_par(e, 0, 0, 0, i, chunks[i].ctypes.data, C.c_float(6.0), 0)
-        res["append_chunk_par_1_thread_gbs"] = round(len(chunks) * mc_amd.CHUNK_BYTES / 1e9
+        res["append_chunk_par_1_thread_gbs"] = round(nappend * mc_amd.CHUNK_BYTES / 1e9
                                                      / (time.perf_counter() - t0), 3)
         L.mc_archive_close(e)
+        # one chunk per launch, device-resident, over the streams the default
+        # queues allow (4 queues, one held by the null stream)
+        res["per_chunk_resident_default_queues_gbs"] = round(per_chunk_rate(ctx, blobs, man, 3, 3, bad), 3)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
-    res["note"] = (f"host buffers, PCIe included: decode of {n} blobs from an archive into pageable memory; "
-                   f"append of {len(chunks)} chunks")
+        ctx.close()
+    res["bit_exact"] = not bad
+    res["note"] = (f"host buffers, PCIe included: decode of {n} blobs from an archive into pageable memory "
+                   f"(every output hashed against the reference manifest); append of {nappend} chunks")
     return res
+
+
+def per_chunk_rate(ctx, blobs, man, nstreams, steps, bad):
+    """GB/s of one chunk per mchip_decode_chunks launch over resident blobs,
+    rotated over nstreams contexts (ctx first); outputs hashed afterwards."""
+    n = len(blobs)
+    offs = np.zeros(n, np.uint64)
+    tot = 0
+    for i, b in enumerate(blobs):
+        offs[i] = tot
+        tot += (len(b) + 255) // 256 * 256
+    host = np.zeros(tot, np.uint8)
+    for i, b in enumerate(blobs):
+        host[int(offs[i]):int(offs[i]) + len(b)] = np.frombuffer(b, np.uint8)
+    d_b = mc_amd.DeviceBuffer(tot)
+    d_o = mc_amd.DeviceBuffer(n * mc_amd.CHUNK_BYTES)
+    ctx.h2d(d_b, host)
+    ctxs = [ctx] + [mc_amd.Ctx(0) for _ in range(nstreams - 1)]
+    Lb = mc_amd.load()
+    lens = [np.array([len(b)], np.uint64) for b in blobs]
+    qv = [np.array([np.frombuffer(b[:4], np.float32)[0]], np.float32) for b in blobs]
+    ov = [np.array([offs[i]], np.uint64) for i in range(n)]
+
+    def step(k):
+        for i in range(n):
+            mc_amd.check(Lb.mchip_decode_chunks(ctxs[k].h, C.c_void_p(d_b.ptr), ov[i].ctypes.data_as(C.c_void_p),
+                                                lens[i].ctypes.data_as(C.c_void_p),
+                                                qv[i].ctypes.data_as(C.c_void_p), 1,
+                                                C.c_void_p(d_o.ptr + i * mc_amd.CHUNK_BYTES), mc_amd.CHUNK_BYTES))
+            k = k + 1 if k + 1 < len(ctxs) else 0
+        return k
+
+    k = step(0)
+    for c_ in ctxs:
+        c_.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        k = step(k)
+    for c_ in ctxs:
+        c_.sync()
+    dt = (time.perf_counter() - t0) / steps
+    out = np.empty((256, 256, 256), np.uint8)
+    for i in range(n):
+        ctx.d2h(out, d_o, offset=i * mc_amd.CHUNK_BYTES)
+        if xxh64_hex(out) != man[i]["decoded_xxh64"]:
+            bad.append(("per_chunk", i))
+    for c_ in ctxs[1:]:
+        c_.close()
+    d_b.free()
+    d_o.free()
+    return n * mc_amd.CHUNK_BYTES / 1e9 / dt
 
 
 def host_cpu():
@@ -466,9 +553,31 @@ def build_pyramid(L, ctx, path, nthreads):
     return a, blobs
 
 
-def dense_levels(ctx, blobs):
-    """Decoded levels (GPU decode of every installed chunk) as dense rasters."""
+def volume_manifest():
+    with open(os.path.join(ROOT, "tests", "golden", "volume_manifest.json")) as f:
+        m = json.load(f)["volume2048_q6"]
+    return {int(k): {tuple(r["coord"]): r for r in v} for k, v in m.items()}
+
+
+def check_pyramid_blobs(blobs, man):
+    """Chunks whose GPU-encoded blob differs from the reference's (or that the
+    reference has and the pyramid lacks, or the other way round)."""
+    bad = 0
+    for lod, lb in enumerate(blobs):
+        for c, r in man[lod].items():
+            b = lb.get(c)
+            if r["blob_len"] == 0:
+                bad += b is not None and len(b) > 0
+            elif b is None or len(b) != r["blob_len"] or xxh64_hex(np.frombuffer(b, np.uint8)) != r["blob_xxh64"]:
+                bad += 1
+    return bad
+
+
+def dense_levels(ctx, blobs, man=None):
+    """Decoded levels (GPU decode of every installed chunk) as dense rasters,
+    and the number of chunks whose decode differs from the manifest's."""
     out = []
+    bad = 0
     for lod, lb in enumerate(blobs):
         n = 2048 >> lod
         v = np.zeros((n, n, n), np.uint8)
@@ -477,9 +586,11 @@ def dense_levels(ctx, blobs):
             part = items[i:i + 32]
             dec = ctx.decode_chunks_host([b for _, b in part])
             for (cz, cy, cx), d in zip([c for c, _ in part], dec):
+                if man is not None and xxh64_hex(d) != man[lod][(cz, cy, cx)]["decoded_xxh64"]:
+                    bad += 1
                 v[256 * cz:256 * cz + 256, 256 * cy:256 * cy + 256, 256 * cx:256 * cx + 256] = d
         out.append(v)
-    return out
+    return out, bad
 
 
 def render_frame(k, w=1024, h=1024):
@@ -506,7 +617,7 @@ def render_frame(k, w=1024, h=1024):
     return grid, float(x0), float(x0), float(step), w, h, zoom
 
 
-def render_leg(a, blobs, ctx, nthreads, arena_bytes=4 << 30, lanes=8, boxes_per_batch=64, edge=128,
+def render_leg(a, levels, ctx, nthreads, arena_bytes=4 << 30, lanes=8, boxes_per_batch=64, edge=128,
                n_render=40, n_batch=80, lam_render=10.0, lam_batch=20.0):
     """Mixed render + box-batch serving over LODs 0-3 of the 2048^3 volume with
     a 4 GiB decoded-block arena: 1024^2 quad renders (trilinear, 9-step mean
@@ -540,9 +651,7 @@ def render_leg(a, blobs, ctx, nthreads, arena_bytes=4 << 30, lanes=8, boxes_per_
     log(f"[bench] render frames LOD {lods}: cold {cold} ms, resident {warm} ms")
     # parity and the CPU baseline: the oracle renders the same frames from
     # the GPU-decoded levels (one core)
-    t0 = time.perf_counter()
-    dense = dense_levels(ctx, blobs)
-    log(f"[bench] render parity: decoded levels in {time.perf_counter() - t0:.1f}s")
+    dense = levels
     exact, cpu_s = [], []
     for k in range(4):
         grid, x0, y0, step, w, h, _ = render_frame(k)
@@ -633,13 +742,48 @@ def render_leg(a, blobs, ctx, nthreads, arena_bytes=4 << 30, lanes=8, boxes_per_
             "mixed": phases}
 
 
-def serve_leg(rates, nthreads, lanes=8, arena_bytes=0, boxes_per_batch=64, edge=128, warm_batches=100,
+def _ref_region_lib(name):
+    """The reference build (oracle/_ref) with its region and block entry
+    points declared, or None when that build is absent."""
+    path = os.path.join(REF_DIR, name)
+    if not os.path.exists(path):
+        return None
+    vp = C.c_void_p
+    R = _ref_lib(name)
+    R.mc_archive_read_regions.argtypes = [vp, C.c_int, vp, C.c_int, C.c_long, C.c_long, C.c_long, vp, C.c_size_t,
+                                          C.c_int]
+    R.mc_archive_decode_block.argtypes = [vp, C.c_uint64, C.c_int, C.c_int, C.c_int, vp]
+    return R
+
+
+def _block_latency(L, a, picks, dst, reps=1):
+    """Per-call latency (ms) of mc_archive_decode_block over picks."""
+    lat = []
+    for _ in range(reps):
+        for co, bz, by, bx in picks:
+            t0 = time.perf_counter()
+            L.mc_archive_decode_block(a, co, bz, by, bx, dst.ctypes.data)
+            lat.append((time.perf_counter() - t0) * 1e3)
+    lat = np.array(lat)
+    return {"p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
+            "calls": int(lat.size)}
+
+
+def serve_leg(rates, nthreads, lanes=8, boxes_per_batch=64, edge=128, bounded_bytes=4 << 30, resident_batches=1000,
               with_render=True):
     """Serving of box batches from an HBM-resident 2048^3 archive
     (volume2048_q6): batches of 64 boxes of 128^3 (mc_archive_sample_boxes,
     seed 10000 + k, min material fraction 0.5) arrive as a Poisson process
     (exponential gaps from seed 99) and are served by mc_serve_* into device
-    memory; latency is submission to completion."""
+    memory; latency is submission to completion.  Each rate runs on a fresh
+    server whose arena (half of the free device memory) starts empty, so the
+    blocks are decoded by the requests ("cold"); a run through a 4 GiB arena
+    keeps missing ("bounded"); a run after a preload of every block serves
+    decoded blocks only ("resident").  The pyramid's GPU-encoded blobs and
+    its GPU decode are checked against the reference manifest, and served and
+    read_regions boxes against that decode.  mc_archive_read_regions into
+    host buffers and per-call mc_archive_decode_block are timed beside the
+    reference build's own (same archive file, host cores)."""
     import tempfile
     import threading
     L = mc_amd.api()
@@ -648,6 +792,13 @@ def serve_leg(rates, nthreads, lanes=8, arena_bytes=0, boxes_per_batch=64, edge=
     t0 = time.perf_counter()
     a, lv_blobs = build_pyramid(L, ctx, path, nthreads)
     log(f"[bench] serving volume (LOD 0-3) built in {time.perf_counter() - t0:.1f}s")
+    vman = volume_manifest()
+    bad_blobs = check_pyramid_blobs(lv_blobs, vman)
+    t0 = time.perf_counter()
+    levels, bad_dec = dense_levels(ctx, lv_blobs, vman)
+    log(f"[bench] pyramid blobs differing from the reference: {bad_blobs}; decoded chunks differing: {bad_dec} "
+        f"({time.perf_counter() - t0:.1f}s)")
+    vol = levels[0]
 
     def batch(k):
         arr = (mc_amd.McBox * boxes_per_batch)()
@@ -655,32 +806,45 @@ def serve_leg(rates, nthreads, lanes=8, arena_bytes=0, boxes_per_batch=64, edge=
                                         C.c_float(0.5), arr)
         return [(arr[i].z0, arr[i].y0, arr[i].x0) for i in range(got)]
 
+    def matches(got, boxes):
+        return all(np.array_equal(got[j], vol[z:z + edge, y:y + edge, x:x + edge]) for j, (z, y, x) in enumerate(boxes))
+
     box_bytes = edge ** 3
-    sv = mc_amd.Serve(a, lod=0, arena_bytes=arena_bytes, lanes=lanes)
     nbuf = 3 * lanes
     bufs = [mc_amd.DeviceBuffer(boxes_per_batch * box_bytes) for _ in range(nbuf)]
-    # bit-exactness: one batch against mc_archive_read_regions (host output)
-    b0 = batch(0)
-    t = sv.submit(b0, edge, edge, edge, bufs[0])
-    cold_ms = sv.wait(t)
-    got = np.empty((len(b0), edge, edge, edge), np.uint8)
-    ctx.d2h(got, bufs[0])
-    want = np.empty_like(got)
-    arr = (mc_amd.McBox * len(b0))(*[mc_amd.McBox(*o) for o in b0])
-    L.mc_archive_read_regions(a, 0, arr, len(b0), edge, edge, edge, want.ctypes.data_as(C.c_void_p),
-                              box_bytes, 0)
-    exact = bool(np.array_equal(got, want))
-    del got, want
+    # bit-exactness: two batches served cold and through read_regions (host
+    # output), against the manifest-checked decode
+    exact = True
+    rr_ms = []
+    sv = mc_amd.Serve(a, lod=0, arena_bytes=0, lanes=lanes)
+    for k in range(2):
+        bk = batch(k)
+        sv.wait(sv.submit(bk, edge, edge, edge, bufs[0]))
+        got = np.empty((len(bk), edge, edge, edge), np.uint8)
+        ctx.d2h(got, bufs[0])
+        exact &= matches(got, bk)
+        arr = (mc_amd.McBox * len(bk))(*[mc_amd.McBox(*o) for o in bk])
+        for rep in range(2):
+            t0 = time.perf_counter()
+            L.mc_archive_read_regions(a, 0, arr, len(bk), edge, edge, edge, got.ctypes.data_as(C.c_void_p),
+                                      box_bytes, 0)
+            rr_ms.append((time.perf_counter() - t0) * 1e3)
+        exact &= matches(got, bk)
+        del got
+    sv.close()
+    rr_batch = batch(0)
     out = []
-    kbase = 1
-    # The arena (half of the free device memory by default) starts empty: a
-    # first run at 200 batches/s fills it from requests alone ("cold").  The
-    # arena then preloads every present block (it holds the whole volume) and
-    # the Poisson runs are served from decoded blocks ("resident").
-    runs = [("cold", 200, warm_batches)] + [("resident", lam, nb) for lam, nb in rates]
+    kbase = 2
+    runs = [("cold", lam, nb, 0) for lam, nb in rates] + [("bounded", 100, rates[-1][1], bounded_bytes),
+                                                          ("resident", 100, resident_batches, 0)]
     preload = None
-    for phase, lam, nb in runs:
-        if phase == "resident" and preload is None:
+    slots = {}
+    for phase, lam, nb, arena in runs:
+        if not nb:
+            continue
+        sv = mc_amd.Serve(a, lod=0, arena_bytes=arena, lanes=lanes)
+        slots[phase] = sv.stats()["arena_slots"]
+        if phase == "resident":
             tp = time.perf_counter()
             nl = sv.preload()
             preload = {"blocks": nl, "ms": round(1e3 * (time.perf_counter() - tp), 1)}
@@ -715,32 +879,84 @@ def serve_leg(rates, nthreads, lanes=8, arena_bytes=0, boxes_per_batch=64, edge=
         th.join()
         wall = time.perf_counter() - start
         s1 = sv.stats()
+        # the last batch of the run, checked against the reference decode
+        last = np.empty((len(reqs[-1]), edge, edge, edge), np.uint8)
+        ctx.d2h(last, bufs[(nb - 1) % nbuf])
+        exact &= matches(last, reqs[-1])
+        del last
+        sv.close()
         lv = np.array(lat)
         vox = sum(len(r) for r in reqs) * box_bytes
         refs = s1["block_refs"] - s0["block_refs"]
-        out.append({"phase": phase, "lambda_per_s": lam, "batches": nb, "p50_ms": round(float(np.percentile(lv, 50)), 3),
+        out.append({"phase": phase, "lambda_per_s": lam, "batches": nb, "arena_slots": slots[phase],
+                    "p50_ms": round(float(np.percentile(lv, 50)), 3),
                     "p99_ms": round(float(np.percentile(lv, 99)), 3), "mean_ms": round(float(lv.mean()), 3),
                     "gbs": round(vox / wall / 1e9, 3),
                     "hit_rate": round((s1["hits"] - s0["hits"]) / max(1, refs), 4),
-                    "decoded_blocks": s1["misses"] - s0["misses"], "evictions": s1["evictions"] - s0["evictions"]})
+                    "decoded_blocks": s1["misses"] - s0["misses"], "evictions": s1["evictions"] - s0["evictions"],
+                    "deferred": s1["deferred"] - s0["deferred"]})
         log(f"[bench] serving {phase} lambda={lam}/s: p50 {out[-1]['p50_ms']} ms p99 {out[-1]['p99_ms']} ms "
-            f"{out[-1]['gbs']} GB/s hit rate {out[-1]['hit_rate']}")
-    st = sv.stats()
-    sv.close()
+            f"{out[-1]['gbs']} GB/s hit rate {out[-1]['hit_rate']} decoded {out[-1]['decoded_blocks']} "
+            f"evictions {out[-1]['evictions']}")
     for b_ in bufs:
         b_.free()
-    render = render_leg(a, lv_blobs, ctx, nthreads, lanes=lanes) if with_render else None
+    # per-call block decode latency (the path a block cache's miss takes)
+    rng = np.random.default_rng(5)
+    picks = []
+    coords = sorted(lv_blobs[0])
+    while len(picks) < 2000:
+        c = coords[int(rng.integers(0, len(coords)))]
+        co = L.mc_archive_chunk_offset(a, 0, *c)
+        bz, by, bx = (int(v) for v in rng.integers(0, 16, 3))
+        if L.mc_archive_block_present(a, 0, 16 * c[0] + bz, 16 * c[1] + by, 16 * c[2] + bx):
+            picks.append((co, bz, by, bx))
+    blk = np.empty(4096, np.uint8)
+    _block_latency(L, a, picks[:100], blk)
+    block_lat = _block_latency(L, a, picks, blk)
+    render = render_leg(a, levels, ctx, nthreads, lanes=lanes) if with_render else None
+    del levels
     L.mc_archive_close(a)
+    # the reference's own region read and block decode on the same archive
+    # file (its format is the reference's), on the host cores
+    ref = {}
+    arr = (mc_amd.McBox * len(rr_batch))(*[mc_amd.McBox(*o) for o in rr_batch])
+    got = np.empty((len(rr_batch), edge, edge, edge), np.uint8)
+    for tag, so in (("fast", "libmcref_fast.so"), ("strict", "libmcref.so")):
+        R = _ref_region_lib(so)
+        if R is None:
+            continue
+        ra = R.mc_archive_open_dims(path.encode(), 2048, 2048, 2048, C.c_float(6.0))
+        if not ra:
+            continue
+        t0 = time.perf_counter()
+        R.mc_archive_read_regions(ra, 0, arr, len(rr_batch), edge, edge, edge, got.ctypes.data_as(C.c_void_p),
+                                  box_bytes, nthreads)
+        ref[tag + "_read_regions_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        if tag == "strict":
+            ref["strict_read_regions_equal_ours"] = bool(matches(got, rr_batch))
+        ref[tag + "_decode_block"] = _block_latency(R, ra, picks[:1000], blk)
+        R.mc_archive_close(ra)
     try:
         os.remove(path)
     except OSError:
         pass
+    rr = np.array(rr_ms)
+    gb_batch = len(rr_batch) * box_bytes / 1e9
     return {"workload": "volume2048_q6", "volume": "2048^3 u8 (8^3 chunks), q=6, air 0.30-0.95 smooth field",
             "boxes_per_batch": boxes_per_batch, "box": f"{edge}^3", "arrivals": "Poisson, gaps from seed 99",
-            "arena_bytes": arena_bytes, "lanes": lanes, "cold_batch_ms": round(cold_ms, 3),
-            "bit_exact_vs_read_regions": exact, "rates": out, "arena_slots": st["arena_slots"],
-            "preload": preload,
-            "latency": "submission to completion, output in device memory", "render_serving": render}
+            "lanes": lanes, "pyramid_blobs_differing_from_reference": bad_blobs,
+            "pyramid_decoded_chunks_differing_from_reference": bad_dec,
+            "bit_exact_vs_reference_decode": bool(exact) and bad_dec == 0,
+            "rates": out, "preload": preload,
+            "latency": "submission to completion, output in device memory",
+            "read_regions_host_buffers": {"ms_per_batch": round(float(np.median(rr)), 2),
+                                          "gbs": round(gb_batch / (float(np.median(rr)) / 1e3), 3),
+                                          "note": "mc_archive_read_regions, 64 boxes of 128^3 into pageable memory"},
+            "decode_block_per_call": block_lat,
+            "reference_cpu": dict(ref, cores=nthreads,
+                                  note="the reference build on the same archive file: read_regions of one "
+                                       "64-box batch with nthreads=cores; decode_block per call"),
+            "render_serving": render}
 
 
 def main():
@@ -757,18 +973,24 @@ def main():
                     help="streams whose kernels carry HIP timing events during the timed steps")
     ap.add_argument("--serve-batches", default="1000,1000,1000",
                     help="batches served at 20, 50 and 100 batches/s (serving leg); 0 skips the leg")
+    ap.add_argument("--resident-batches", type=int, default=1000, help=argparse.SUPPRESS)
     ap.add_argument("--serve-only", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--mixed-only", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--dropin-only", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-quality batched leg")
     ap.add_argument("--no-dropin", action="store_true",
                     help="skip the drop-in calls with host buffers (their caller threads stop rocprofv3's tracer)")
     args = ap.parse_args()
     if args.serve_only:
         nbs = [int(x) for x in args.serve_batches.split(",")]
-        print(json.dumps(serve_leg(list(zip((20, 50, 100), nbs)), cpu_threads(), lanes=SERVE_LANES)))
+        print(json.dumps(serve_leg(list(zip((20, 50, 100), nbs)), cpu_threads(), lanes=SERVE_LANES,
+                                   resident_batches=args.resident_batches)))
         return
     if args.mixed_only:
         print(json.dumps(mixed_leg(cpu_threads())))
+        return
+    if args.dropin_only:
+        print(json.dumps(dropin_leg(cpu_threads())))
         return
     # The legs on other workloads run first, each in a child process, before
     # this process touches the GPU: each then has the device's hardware queue
@@ -782,14 +1004,19 @@ def main():
             raise RuntimeError(f"bench leg {flag} failed with status {r.returncode}")
         return json.loads(r.stdout.strip().splitlines()[-1])
 
-    mixed = serving = None
+    mixed = serving = dropin = None
+    if not args.no_dropin and args.gpus == 1:
+        log("[bench] drop-in calls with host buffers at the default queue count (child process)")
+        dropin = child("--dropin-only")
+        log(f"[bench] drop-in: {dropin}")
     if not args.no_mixed and args.gpus == 1:
         log("[bench] mixed-quality batched leg (child process)")
         mixed = child("--mixed-only")
     nbs = [int(x) for x in args.serve_batches.split(",")]
     if any(nbs) and args.gpus == 1:
         log("[bench] serving leg (child process)")
-        serving = child("--serve-only", "--serve-batches", args.serve_batches)
+        serving = child("--serve-only", "--serve-batches", args.serve_batches, "--resident-batches",
+                        str(args.resident_batches))
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); "
               "the codec shards by chunk with no cross-GPU exchange", file=sys.stderr)
@@ -858,8 +1085,6 @@ def main():
         ref_cpu = cpu_reference_baseline(chunks[:ns], blobs[:ns], man, nthreads)
     if ref_cpu:
         log("[bench] reference CPU build: " + ", ".join(f"{k} {v:.3f}" for k, v in ref_cpu.items()))
-    drop_chunks = chunks[:ns].copy()
-    del chunks
 
     offs = np.zeros(n, np.uint64)
     tot = 0
@@ -931,12 +1156,6 @@ def main():
     # caller threads create streams that share queues with them, but those
     # are gone or idle when the per-chunk leg runs.
     ctxs = [ctx] + [mc_amd.Ctx(0) for _ in range(args.streams - 1)]
-    dropin = None
-    if not args.no_dropin:
-        log("[bench] drop-in calls with host buffers")
-        dropin = dropin_leg(drop_chunks, blobs, nthreads)
-        log(f"[bench] drop-in: {dropin}")
-    del drop_chunks
     # per-chunk call arguments built once: each step issues exactly the C calls
     # mchip_decode_chunks(ctx, blobs, &off[i], &len[i], &q[i], 1, out_i, 16 MiB)
     Lib = mc_amd.load()
@@ -991,15 +1210,39 @@ def main():
     ms_per_step = dt * 1000.0 / args.steps
     value = n * mc_amd.CHUNK_BYTES / (dt / args.steps) / 1e9
 
-    # bit-exactness check of the last step's output
+    # bit-exactness check of the last step's output, and the air-voxel leak:
+    # source air voxels (value 0) that decode nonzero, as the reference's
+    # round-trip test reports it (tests/mc_roundtrip.c:50-60); air is exactly
+    # 0 only in blocks the bitmap marks absent
     out = np.empty((256, 256, 256), np.uint8)
     bad = 0
+    leak = dict(air=0, air_nonzero=0, air_sq=0, material=0, mat_sq=0, max_err=0)
     for i in range(n):
         ctx.d2h(out, d_out, offset=i * mc_amd.CHUNK_BYTES)
         if xxh64_hex(out) != man[i]["decoded_xxh64"]:
             bad += 1
+        src = chunks[i].reshape(-1)
+        dec = out.reshape(-1)
+        air = src == 0
+        da = dec[air].astype(np.int64)
+        e = np.abs(dec[~air].astype(np.int64) - src[~air].astype(np.int64))
+        leak["air"] += int(air.sum())
+        leak["air_nonzero"] += int(np.count_nonzero(da))
+        leak["air_sq"] += int((da * da).sum())
+        leak["material"] += int(e.size)
+        leak["mat_sq"] += int((e * e).sum())
+        leak["max_err"] = max(leak["max_err"], int(e.max()) if e.size else 0)
+    del chunks
     if bad:
         log(f"[bench] ERROR: {bad} decoded chunks differ from the reference")
+    mat_rmse = (leak["mat_sq"] / max(1, leak["material"])) ** 0.5
+    air_leak = {"air_voxels": leak["air"], "air_voxels_nonzero": leak["air_nonzero"],
+                "air_nonzero_frac": round(leak["air_nonzero"] / max(1, leak["air"]), 5),
+                "air_rmse": round((leak["air_sq"] / max(1, leak["air"])) ** 0.5, 4),
+                "material_rmse": round(mat_rmse, 4), "material_max_err": leak["max_err"],
+                "psnr_db": round(10 * np.log10(255.0 ** 2 / mat_rmse ** 2), 2) if mat_rmse > 0 else 99.0,
+                "note": "decoded vs generated voxels over all chunks (reference tests/mc_roundtrip.c:50-60)"}
+    log(f"[bench] air leak: {air_leak}")
 
 
     # roofline of the dominant kernel (entropy decode, one launch per chunk):
@@ -1040,6 +1283,7 @@ def main():
                               else f"{n} chunks per launch (automatic sequence size, 16-256 chunks)"),
                    "kernels_ms_per_chunk": {k: round(v[0] / (esteps * n), 4) for k, v in enc_k.items()}},
         "verify": verify,
+        "air_leak": air_leak,
         "dropin_host_buffers": dropin,
     }
     rec.update(extra)

@@ -43,10 +43,14 @@ const char *mchip_last_error(void);
  * (src/mc_codec.c:141-154, :533). */
 int         mchip_set_priors(const uint16_t *plo, const uint16_t *phi);
 
-/* Launch-shape tuning (process-wide): "decoder" selects the entropy decoder
- * variant (0 = one loop step per scan position, 1 = one bin per loop step
- * with branches, 2 = one bin per loop step, branch-free, payload in an LDS
- * window; the default).  Results are identical; only speed differs. */
+/* Launch-shape tuning (process-wide, read as one snapshot per call):
+ * "decoder" selects the entropy decoder variant (0 = one loop step per scan
+ * position, 2 = one bin per loop step, branch-free, payload in an LDS
+ * window; the default); "persistent", "persist_min" and "persist_wg" shape
+ * the persistent entropy launch of batched decodes; "enc_batch" sets the
+ * chunks per encode launch sequence (0 = sized from free device memory);
+ * "merge" (default 1) merges concurrent one-chunk host decodes.  Results
+ * are identical; only speed differs. */
 int         mchip_set_tuning(const char *key, int value);
 
 /* A context owns one HIP stream and a growable device workspace. */
@@ -94,6 +98,19 @@ int mchip_decode_chunks_static(const uint8_t *d_blobs, const uint64_t *blob_off,
  * out + i*out_stride, synchronise. */
 int mchip_decode_chunks_host(mchip_ctx *ctx, const uint8_t *const *blobs, const uint64_t *blob_len,
                              int n, uint8_t *out, uint64_t out_stride);
+
+/* One chunk blob in host memory decoded into the 256^3 raster at out (host),
+ * synchronously, as mchip_decode_chunks_host with n = 1, but merged with the
+ * calls other threads make at the same time: each call uploads its blob on
+ * ctx's stream into a slot of a shared device arena, one worker issues all
+ * queued calls as one batched decode (one batch in flight; calls arriving
+ * meanwhile form the next batch), and each call copies its chunk back on
+ * ctx's stream.  Blobs over 4 MiB, or tuning "merge" = 0, take the caller's
+ * own decode.  Replaces the per-call worker pool of mc_archive_decode_chunk
+ * (src/mc_archive.c:783-820) for concurrent callers. */
+int mchip_decode_chunk_merged(mchip_ctx *ctx, const uint8_t *blob, uint64_t blob_len, uint8_t *out);
+/* Batches issued and chunks decoded by the merged path so far. */
+int mchip_merge_stats(uint64_t *batches, uint64_t *chunks);
 
 /* Decode n independent block payloads (host memory) at quality q[i] into
  * out + 4096*i (16^3 z,y,x).  A NULL payload or blob-absent block writes

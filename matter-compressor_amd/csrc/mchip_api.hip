@@ -10,7 +10,10 @@
 #include <cstring>
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
+#include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -85,6 +88,7 @@ struct Tuning {
     int persistent = 1;       // batched decode (>= persist_min chunks): persistent entropy launch
     int persist_min = 2;
     int persist_wg = 0;       // workgroups of the persistent launch (0: occupancy x CUs)
+    int merge = 1;            // concurrent one-chunk host calls join one batched decode
 };
 std::mutex g_tune_mu;
 Tuning g_tune;
@@ -409,9 +413,11 @@ int persistent_grid() {
 
 // Descriptors, workspace and launch arguments of a decode of n resident
 // chunk blobs (everything but the launches).
+// out_at (optional) gives each chunk's byte offset in d_out; without it
+// chunk i is written at i * out_stride.
 static int prepare_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *blob_off, const uint64_t *blob_len,
                           const float *q, int n, uint8_t *d_out, uint64_t out_stride, const Tuning &tn,
-                          DecodeArgs &a) {
+                          DecodeArgs &a, const uint64_t *out_at = nullptr) {
     if (!c) return fail("null context");
     if (n <= 0) return 0;
     if (!d_blobs || !blob_off || !blob_len || !q || !d_out) return fail("null argument");
@@ -433,7 +439,7 @@ static int prepare_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
         if (blob_len[i] > 0xFFFFFFF0ull) return fail("blob %d too long", i);
         cd[i].blob_off = blob_off[i];
         cd[i].blob_len = blob_len[i];
-        cd[i].out_off = (uint64_t)i * out_stride;
+        cd[i].out_off = out_at ? out_at[i] : (uint64_t)i * out_stride;
         cd[i].scratch_off = sw;
         cd[i].qtab = slot;
         cd[i].pad = 0;
@@ -540,6 +546,7 @@ int mchip_set_tuning(const char *key, int value) {
         g_tune.decoder = value;
         return 0;
     }
+    if (!strcmp(key, "merge")) { g_tune.merge = value != 0; return 0; }
     return fail("unknown tuning key '%s'", key);
 }
 
@@ -810,6 +817,190 @@ int mchip_autotune_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
     if (report && cap) {
         snprintf(report, cap, "%s", rep.c_str());
     }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Merged decode of concurrent one-chunk host calls (mchip_decode_chunk_merged).
+//
+// Each caller uploads its blob into a slot of a shared device blob arena on
+// its own stream and queues a request; one worker thread issues every queued
+// request as one batched decode (one index, entropy and reconstruction
+// launch sequence over all of them) on its own stream, after the callers'
+// uploads, and waits for those kernels before it takes the next batch, so
+// the requests that arrive while a batch runs form the next one.  Each
+// caller then copies its chunk out of the shared output arena on its own
+// stream.  Without this, concurrent callers' kernels run one per hardware
+// queue at a time (HIP's default is 4 queues, one held by the null stream).
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kMergeSlots = 32;
+constexpr size_t kMergeBlobCap = 4ull << 20;       // larger blobs take the caller's own decode
+constexpr uint64_t kChunkBytes = 16777216ull;
+
+struct MergeBatch {                                // decode event of one batch, shared by its callers
+    hipEvent_t ev = nullptr;
+    ~MergeBatch() { if (ev) (void)hipEventDestroy(ev); }
+};
+
+struct MergeReq {
+    int slot = -1;
+    uint64_t len = 0;
+    float q = 8.0f;
+    hipEvent_t ev_in = nullptr;                    // the caller's upload
+    std::shared_ptr<MergeBatch> batch;             // set when the batch is issued
+    bool issued = false;
+    int rc = 0;
+    std::string err;
+};
+
+struct Merger {
+    std::mutex mu;
+    std::condition_variable cv_work, cv_issued, cv_slot;
+    std::vector<int> free_slots;
+    std::deque<MergeReq *> pending;
+    uint8_t *d_blob = nullptr, *d_out = nullptr;   // kMergeSlots slots each
+    mchip_ctx *ctx = nullptr;
+    int state = 0;                                 // 0 not started, 1 running, -1 unavailable
+    uint64_t batches = 0, chunks = 0;
+
+    // Issue queued requests as one batched decode, one batch in flight.
+    void run() {
+        (void)hipSetDevice(G().device);
+        std::vector<MergeReq *> b;
+        std::vector<uint64_t> off, len, oat;
+        std::vector<float> q;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] { return !pending.empty(); });
+                b.assign(pending.begin(), pending.end());
+                pending.clear();
+            }
+            const int n = (int)b.size();
+            off.resize(n); len.resize(n); oat.resize(n); q.resize(n);
+            for (int i = 0; i < n; ++i) {
+                off[i] = (uint64_t)b[i]->slot * kMergeBlobCap;
+                len[i] = b[i]->len;
+                oat[i] = (uint64_t)b[i]->slot * kChunkBytes;
+                q[i] = b[i]->q;
+            }
+            auto batch = std::make_shared<MergeBatch>();
+            int rc = hipEventCreateWithFlags(&batch->ev, hipEventDisableTiming) == hipSuccess ? 0
+                         : fail("merged decode: event creation failed");
+            for (int i = 0; i < n && !rc; ++i)
+                if (hipStreamWaitEvent(ctx->stream, b[i]->ev_in, 0) != hipSuccess)
+                    rc = fail("merged decode: stream wait failed");
+            DecodeArgs a;
+            if (!rc) rc = prepare_decode(ctx, d_blob, off.data(), len.data(), q.data(), n, d_out, kChunkBytes,
+                                         tuning(), a, oat.data());
+            if (!rc) {
+                launch_decode(ctx, a);
+                if (hipGetLastError() != hipSuccess || hipEventRecord(batch->ev, ctx->stream) != hipSuccess)
+                    rc = fail("merged decode: launch failed");
+            }
+            const std::string err = rc ? mchip_last_error() : std::string();
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                for (MergeReq *r : b) {
+                    r->batch = batch;
+                    r->rc = rc;
+                    r->err = err;
+                    r->issued = true;
+                }
+                batches++;
+                chunks += (uint64_t)n;
+            }
+            cv_issued.notify_all();
+            if (!rc) (void)hipEventSynchronize(batch->ev);
+        }
+    }
+};
+
+// The process-wide merger, started on first use (its worker lives until the
+// process ends).  Null when the device buffers cannot be had.
+Merger *merger() {
+    static Merger *m = new Merger;
+    std::lock_guard<std::mutex> lk(m->mu);
+    if (m->state == 0) {
+        m->state = -1;
+        if (hipMalloc((void **)&m->d_blob, kMergeSlots * kMergeBlobCap) != hipSuccess ||
+            hipMalloc((void **)&m->d_out, kMergeSlots * kChunkBytes) != hipSuccess ||
+            !(m->ctx = mchip_ctx_create())) {
+            if (m->d_blob) (void)hipFree(m->d_blob);
+            if (m->d_out) (void)hipFree(m->d_out);
+            m->d_blob = m->d_out = nullptr;
+            return nullptr;
+        }
+        for (int i = kMergeSlots - 1; i >= 0; --i) m->free_slots.push_back(i);
+        std::thread([] { m->run(); }).detach();
+        m->state = 1;
+    }
+    return m->state == 1 ? m : nullptr;
+}
+
+}  // namespace
+
+int mchip_decode_chunk_merged(mchip_ctx *c, const uint8_t *blob, uint64_t blob_len, uint8_t *out) {
+    if (!c) return fail("null context");
+    if (!blob || !out) return fail("null argument");
+    Merger *m = (tuning().merge && blob_len <= kMergeBlobCap) ? merger() : nullptr;
+    if (!m) return mchip_decode_chunks_host(c, &blob, &blob_len, 1, out, kChunkBytes);
+    MergeReq r;
+    r.len = blob_len;
+    if (blob_len >= 4) memcpy(&r.q, blob, 4);
+    {
+        std::unique_lock<std::mutex> lk(m->mu);
+        m->cv_slot.wait(lk, [&] { return !m->free_slots.empty(); });
+        r.slot = m->free_slots.back();
+        m->free_slots.pop_back();
+    }
+    auto release = [&] {
+        {
+            std::lock_guard<std::mutex> lk(m->mu);
+            m->free_slots.push_back(r.slot);
+        }
+        m->cv_slot.notify_one();
+    };
+    r.ev_in = get_event(c);
+    if (!r.ev_in ||
+        (blob_len && hipMemcpyAsync(m->d_blob + (size_t)r.slot * kMergeBlobCap, blob, blob_len,
+                                    hipMemcpyHostToDevice, c->stream) != hipSuccess) ||
+        hipEventRecord(r.ev_in, c->stream) != hipSuccess) {
+        if (r.ev_in) c->free_events.push_back(r.ev_in);
+        release();
+        return fail("merged decode: upload failed");
+    }
+    {
+        std::unique_lock<std::mutex> lk(m->mu);
+        m->pending.push_back(&r);
+        m->cv_work.notify_one();
+        m->cv_issued.wait(lk, [&] { return r.issued; });
+    }
+    c->free_events.push_back(r.ev_in);
+    int rc = r.rc;
+    if (!rc) {
+        if (hipStreamWaitEvent(c->stream, r.batch->ev, 0) != hipSuccess ||
+            hipMemcpyAsync(out, m->d_out + (size_t)r.slot * kChunkBytes, kChunkBytes, hipMemcpyDeviceToHost,
+                           c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = fail("merged decode: copy-out failed");
+    } else {
+        (void)hipStreamSynchronize(c->stream);
+        fail("%s", r.err.c_str());
+    }
+    r.batch.reset();
+    release();
+    return rc;
+}
+
+int mchip_merge_stats(uint64_t *batches, uint64_t *chunks) {
+    Merger *m = merger();
+    if (!m) return fail("merged decode unavailable");
+    std::lock_guard<std::mutex> lk(m->mu);
+    if (batches) *batches = m->batches;
+    if (chunks) *chunks = m->chunks;
     return 0;
 }
 

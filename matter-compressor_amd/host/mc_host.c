@@ -643,7 +643,8 @@ void mc_archive_decode_chunk(mc_archive *a, uint64_t chunk_off, mc_u8 *out, int 
     else if (!len || len > end - chunk_off) len = end - chunk_off;   /* the payload bound is the live cursor */
     mchip_ctx *d = mch_dev();
     const uint8_t *b = a->base + chunk_off;
-    if (!d || mchip_decode_chunks_host(d, &b, &len, 1, out, MCH_CHUNK_BYTES) != 0) {
+    /* concurrent callers join one batched decode on the device */
+    if (!d || mchip_decode_chunk_merged(d, b, len, out) != 0) {
         mch_dev_error("mc_archive_decode_chunk");
         memset(out, 0, (size_t)MCH_CHUNK_BYTES);
     }

@@ -68,6 +68,8 @@ def load():
         "mchip_decode_chunks": (i, [vp, u8p, vp, vp, vp, i, u8p, C.c_uint64]),
         "mchip_decode_chunks_host": (i, [vp, vp, vp, i, u8p, C.c_uint64]),
         "mchip_decode_chunks_static": (i, [u8p, vp, vp, vp, i, u8p, C.c_uint64, vp]),
+        "mchip_decode_chunk_merged": (i, [vp, vp, C.c_uint64, vp]),
+        "mchip_merge_stats": (i, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "mchip_deblock": (i, [vp, vp, i, i, i, C.c_float]),
         "mchip_deblock_host": (i, [vp, vp, i, i, i, C.c_float]),
         "mchip_decode_blocks_host": (i, [vp, vp, vp, vp, i, u8p]),
