@@ -49,7 +49,8 @@ int         mchip_set_priors(const uint16_t *plo, const uint16_t *phi);
  * window; the default); "persistent", "persist_min" and "persist_wg" shape
  * the persistent entropy launch of batched decodes; "enc_batch" sets the
  * chunks per encode launch sequence (0 = sized from free device memory);
- * "merge" (default 1) merges concurrent one-chunk host decodes.  Results
+ * "merge" (default 1), "merge_inflight" and "merge_min" shape the merging of
+ * concurrent one-chunk host decodes.  Results
  * are identical; only speed differs. */
 int         mchip_set_tuning(const char *key, int value);
 
@@ -102,11 +103,12 @@ int mchip_decode_chunks_host(mchip_ctx *ctx, const uint8_t *const *blobs, const 
 /* One chunk blob in host memory decoded into the 256^3 raster at out (host),
  * synchronously, as mchip_decode_chunks_host with n = 1, but merged with the
  * calls other threads make at the same time: each call uploads its blob on
- * ctx's stream into a slot of a shared device arena, one worker issues all
- * queued calls as one batched decode (one batch in flight; calls arriving
- * meanwhile form the next batch), and each call copies its chunk back on
- * ctx's stream.  Blobs over 4 MiB, or tuning "merge" = 0, take the caller's
- * own decode.  Replaces the per-call worker pool of mc_archive_decode_chunk
+ * ctx's stream into a slot of a shared device arena, one worker issues the
+ * queued calls as one batched decode when no batch is in flight, or when
+ * "merge_min" calls (default 8) wait and fewer than "merge_inflight" batches
+ * (default 2) are in flight, and each call copies its chunk back on ctx's
+ * stream.  Blobs over 4 MiB, or tuning "merge" = 0, take the caller's own
+ * decode.  Replaces the per-call worker pool of mc_archive_decode_chunk
  * (src/mc_archive.c:783-820) for concurrent callers. */
 int mchip_decode_chunk_merged(mchip_ctx *ctx, const uint8_t *blob, uint64_t blob_len, uint8_t *out);
 /* Batches issued and chunks decoded by the merged path so far. */

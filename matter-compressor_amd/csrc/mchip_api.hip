@@ -10,6 +10,7 @@
 #include <cstring>
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -89,6 +90,8 @@ struct Tuning {
     int persist_min = 2;
     int persist_wg = 0;       // workgroups of the persistent launch (0: occupancy x CUs)
     int merge = 1;            // concurrent one-chunk host calls join one batched decode
+    int merge_inflight = 2;   // merged batches in flight at most (1..4)
+    int merge_min = 8;        // queued calls that start a batch while another is in flight
 };
 std::mutex g_tune_mu;
 Tuning g_tune;
@@ -547,6 +550,16 @@ int mchip_set_tuning(const char *key, int value) {
         return 0;
     }
     if (!strcmp(key, "merge")) { g_tune.merge = value != 0; return 0; }
+    if (!strcmp(key, "merge_inflight")) {
+        if (value < 1 || value > 4) return fail("merge_inflight must be in [1, 4]");
+        g_tune.merge_inflight = value;
+        return 0;
+    }
+    if (!strcmp(key, "merge_min")) {
+        if (value < 1 || value > 32) return fail("merge_min must be in [1, 32]");
+        g_tune.merge_min = value;
+        return 0;
+    }
     return fail("unknown tuning key '%s'", key);
 }
 
@@ -855,29 +868,54 @@ struct MergeReq {
     std::string err;
 };
 
+constexpr int kMergeLanes = 4;                     // batches in flight at most
+
 struct Merger {
     std::mutex mu;
     std::condition_variable cv_work, cv_issued, cv_slot;
     std::vector<int> free_slots;
     std::deque<MergeReq *> pending;
     uint8_t *d_blob = nullptr, *d_out = nullptr;   // kMergeSlots slots each
-    mchip_ctx *ctx = nullptr;
+    mchip_ctx *ctx[kMergeLanes] = {};              // one context (stream + workspace) per batch in flight
     int state = 0;                                 // 0 not started, 1 running, -1 unavailable
     uint64_t batches = 0, chunks = 0;
 
-    // Issue queued requests as one batched decode, one batch in flight.
+    // Issue queued requests as batched decodes.  A batch starts when no
+    // batch is in flight, or when at least merge_min requests wait and a
+    // context is free (at most merge_inflight batches in flight); otherwise
+    // arrivals wait for a batch to finish and join the next one.
     void run() {
         (void)hipSetDevice(G().device);
+        struct Flight { int lane; std::shared_ptr<MergeBatch> b; };
+        std::deque<Flight> flight;
+        bool busy[kMergeLanes] = {};
         std::vector<MergeReq *> b;
         std::vector<uint64_t> off, len, oat;
         std::vector<float> q;
         for (;;) {
+            while (!flight.empty() && hipEventQuery(flight.front().b->ev) != hipErrorNotReady) {
+                busy[flight.front().lane] = false;
+                flight.pop_front();
+            }
+            const Tuning tn = tuning();
+            const int maxf = std::max(1, std::min(kMergeLanes, tn.merge_inflight));
+            int lane = -1;
+            for (int l = 0; l < maxf && lane < 0; ++l)
+                if (!busy[l]) lane = l;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv_work.wait(lk, [&] { return !pending.empty(); });
+                if (flight.empty()) {
+                    cv_work.wait(lk, [&] { return !pending.empty(); });
+                } else if (pending.empty() || lane < 0 || (int)pending.size() < tn.merge_min) {
+                    // a batch is in flight: wait a little for arrivals or for it to finish
+                    cv_work.wait_for(lk, std::chrono::microseconds(50));
+                    continue;
+                }
                 b.assign(pending.begin(), pending.end());
                 pending.clear();
             }
+            if (lane < 0) lane = 0;
+            mchip_ctx *c = ctx[lane];
             const int n = (int)b.size();
             off.resize(n); len.resize(n); oat.resize(n); q.resize(n);
             for (int i = 0; i < n; ++i) {
@@ -890,14 +928,14 @@ struct Merger {
             int rc = hipEventCreateWithFlags(&batch->ev, hipEventDisableTiming) == hipSuccess ? 0
                          : fail("merged decode: event creation failed");
             for (int i = 0; i < n && !rc; ++i)
-                if (hipStreamWaitEvent(ctx->stream, b[i]->ev_in, 0) != hipSuccess)
+                if (hipStreamWaitEvent(c->stream, b[i]->ev_in, 0) != hipSuccess)
                     rc = fail("merged decode: stream wait failed");
             DecodeArgs a;
-            if (!rc) rc = prepare_decode(ctx, d_blob, off.data(), len.data(), q.data(), n, d_out, kChunkBytes,
-                                         tuning(), a, oat.data());
+            if (!rc) rc = prepare_decode(c, d_blob, off.data(), len.data(), q.data(), n, d_out, kChunkBytes, tn, a,
+                                         oat.data());
             if (!rc) {
-                launch_decode(ctx, a);
-                if (hipGetLastError() != hipSuccess || hipEventRecord(batch->ev, ctx->stream) != hipSuccess)
+                launch_decode(c, a);
+                if (hipGetLastError() != hipSuccess || hipEventRecord(batch->ev, c->stream) != hipSuccess)
                     rc = fail("merged decode: launch failed");
             }
             const std::string err = rc ? mchip_last_error() : std::string();
@@ -913,7 +951,10 @@ struct Merger {
                 chunks += (uint64_t)n;
             }
             cv_issued.notify_all();
-            if (!rc) (void)hipEventSynchronize(batch->ev);
+            if (!rc) {
+                busy[lane] = true;
+                flight.push_back(Flight{lane, batch});
+            }
         }
     }
 };
@@ -925,9 +966,10 @@ Merger *merger() {
     std::lock_guard<std::mutex> lk(m->mu);
     if (m->state == 0) {
         m->state = -1;
-        if (hipMalloc((void **)&m->d_blob, kMergeSlots * kMergeBlobCap) != hipSuccess ||
-            hipMalloc((void **)&m->d_out, kMergeSlots * kChunkBytes) != hipSuccess ||
-            !(m->ctx = mchip_ctx_create())) {
+        bool ok = hipMalloc((void **)&m->d_blob, kMergeSlots * kMergeBlobCap) == hipSuccess &&
+                  hipMalloc((void **)&m->d_out, kMergeSlots * kChunkBytes) == hipSuccess;
+        for (int l = 0; l < kMergeLanes && ok; ++l) ok = (m->ctx[l] = mchip_ctx_create()) != nullptr;
+        if (!ok) {
             if (m->d_blob) (void)hipFree(m->d_blob);
             if (m->d_out) (void)hipFree(m->d_out);
             m->d_blob = m->d_out = nullptr;
