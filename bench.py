@@ -433,18 +433,47 @@ def mixed_leg(nthreads):
     ecaps = np.full(n, cap, np.uint64)
     qv = [x["q"] for x in specs]
     tv = [x["tau"] for x in specs]
+    # the encode workspace of a 256-chunk sequence (about 165 MiB per chunk)
+    # stays with the context between the timed calls
+    mc_amd.set_tuning("enc_keep_mib", 1 << 20)
     ctx.encode_chunks(d_in, qv, tv, d_enc, eoffs, ecaps)          # workspace allocation
     ctx.sync()
     t0 = time.perf_counter()
     elens, est = ctx.encode_chunks(d_in, qv, tv, d_enc, eoffs, ecaps)
     enc_s = time.perf_counter() - t0
-    d_in.free()
-    ok_enc = bool((est == 0).all())
-    for i in range(n):
-        b = np.empty(int(elens[i]), np.uint8)
-        ctx.d2h(b, d_enc, offset=int(eoffs[i]))
-        ok_enc &= len(b) == man[i]["blob_len"] and xxh64_hex(b) == man[i]["blob_xxh64"]
+
+    def check_blobs(lens, st):
+        ok = bool((st == 0).all())
+        for i in range(n):
+            b = np.empty(int(lens[i]), np.uint8)
+            ctx.d2h(b, d_enc, offset=int(eoffs[i]))
+            ok &= len(b) == man[i]["blob_len"] and xxh64_hex(b) == man[i]["blob_xxh64"]
+        return ok
+
+    ok_enc = check_blobs(elens, est)
     log(f"[bench] mixed encode {n * mc_amd.CHUNK_BYTES / enc_s / 1e9:.2f} GB/s, byte-identical={ok_enc}")
+    # the same sequence captured as a HIP graph and replayed
+    enc_graph_s, ok_graph_enc = None, None
+    try:
+        eg, egn = ctx.encode_graph(d_in, qv, tv, d_enc, eoffs, ecaps)
+        ctx.graph_launch(eg)
+        ctx.sync()
+        tge = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            ctx.graph_launch(eg)
+            ctx.sync()
+            tge.append(time.perf_counter() - t0)
+        glens, gst = ctx.encode_graph_results(eg, egn)
+        ok_graph_enc = check_blobs(glens, gst)
+        enc_graph_s = min(tge)
+        ctx.graph_destroy(eg)
+        log(f"[bench] mixed encode graph {n * mc_amd.CHUNK_BYTES / enc_graph_s / 1e9:.2f} GB/s, "
+            f"byte-identical={ok_graph_enc}")
+    except mc_amd.MchipError as e:
+        log(f"[bench] mixed encode graph not captured: {e}")
+    mc_amd.set_tuning("enc_keep_mib", 2048)
+    d_in.free()
     qs = np.array([x["q"] for x in specs], np.float32)
     d_out = mc_amd.DeviceBuffer(n * mc_amd.CHUNK_BYTES)
 
@@ -480,6 +509,8 @@ def mixed_leg(nthreads):
     res = {"workload": "mixed256", "chunks": n, "decoded_bytes": n * mc_amd.CHUNK_BYTES,
            "blob_bytes": int(np.sum(elens)),
            "encode_gbs": round(n * mc_amd.CHUNK_BYTES / enc_s / 1e9, 3), "encode_byte_identical": ok_enc,
+           "encode_graph_gbs": (round(n * mc_amd.CHUNK_BYTES / enc_graph_s / 1e9, 3) if enc_graph_s else None),
+           "encode_graph_byte_identical": ok_graph_enc,
            "decode_gbs": round(n * mc_amd.CHUNK_BYTES / min(ts) / 1e9, 3),
            "decode_graph_gbs": round(n * mc_amd.CHUNK_BYTES / min(tg) / 1e9, 3),
            "bit_exact": bad_eager == 0 and bad_graph == 0, "autotune": tune, "chosen": best,
@@ -769,7 +800,7 @@ def _block_latency(L, a, picks, dst, reps=1):
             "calls": int(lat.size)}
 
 
-def serve_leg(rates, nthreads, lanes=8, boxes_per_batch=64, edge=128, bounded_bytes=4 << 30, resident_batches=1000,
+def serve_leg(rates, nthreads, lanes=8, boxes_per_batch=64, edge=128, bounded_bytes=1 << 30, resident_batches=1000,
               with_render=True):
     """Serving of box batches from an HBM-resident 2048^3 archive
     (volume2048_q6): batches of 64 boxes of 128^3 (mc_archive_sample_boxes,
@@ -777,8 +808,8 @@ def serve_leg(rates, nthreads, lanes=8, boxes_per_batch=64, edge=128, bounded_by
     (exponential gaps from seed 99) and are served by mc_serve_* into device
     memory; latency is submission to completion.  Each rate runs on a fresh
     server whose arena (half of the free device memory) starts empty, so the
-    blocks are decoded by the requests ("cold"); a run through a 4 GiB arena
-    keeps missing ("bounded"); a run after a preload of every block serves
+    blocks are decoded by the requests ("cold"); a run through a 1 GiB arena
+    (a quarter of the blocks those runs touch) keeps missing ("bounded"); a run after a preload of every block serves
     decoded blocks only ("resident").  The pyramid's GPU-encoded blobs and
     its GPU decode are checked against the reference manifest, and served and
     read_regions boxes against that decode.  mc_archive_read_regions into
@@ -1050,6 +1081,7 @@ def main():
     qv = [s_["q"] for s_ in specs]
     tv = [s_["tau"] for s_ in specs]
     log("[bench] GPU encode warmup")
+    mc_amd.set_tuning("enc_keep_mib", 1 << 20)     # keep the 64-chunk workspace between the timed calls
     ctx.encode_chunks(d_in, qv, tv, d_enc, eoffs, ecaps)
     ctx.enable_timing(True)
     ctx.reset_timing()
@@ -1061,6 +1093,7 @@ def main():
     enc_k = {k: ctx.kernel_time(k) for k in ("enc_transform", "enc_entropy", "enc_assemble", "enc_copy",
                                               "enc_hash")}
     ctx.enable_timing(False)
+    mc_amd.set_tuning("enc_keep_mib", 2048)
     assert (est == 0).all(), f"encode status {est}"
     blobs = []
     bad_enc = 0

@@ -47,8 +47,12 @@ int         mchip_set_priors(const uint16_t *plo, const uint16_t *phi);
  * "decoder" selects the entropy decoder variant (0 = one loop step per scan
  * position, 2 = one bin per loop step, branch-free, payload in an LDS
  * window; the default); "persistent", "persist_min" and "persist_wg" shape
- * the persistent entropy launch of batched decodes; "enc_batch" sets the
- * chunks per encode launch sequence (0 = sized from free device memory);
+ * the persistent entropy launch of batched decodes; "len_shift" and
+ * "group_shift" the payload-length buckets of the lane order and of the
+ * persistent group order; "recon_bpw" the reconstruction blocks per
+ * workgroup; "tuned_clear" drops the autotuned shapes; "enc_batch" sets the
+ * chunks per encode launch sequence (0 = sized from free device memory) and
+ * "enc_keep_mib" the encode workspace a context keeps after a call;
  * "merge" (default 1), "merge_inflight" and "merge_min" shape the merging of
  * concurrent one-chunk host decodes.  Results
  * are identical; only speed differs. */
@@ -241,12 +245,26 @@ mchip_graph *mchip_decode_graph_create(mchip_ctx *ctx, const uint8_t *d_blobs, c
                                        uint64_t out_stride);
 int          mchip_graph_launch(mchip_graph *g);
 void         mchip_graph_destroy(mchip_graph *g);
-/* Time the decode variants (entropy decoder, persistent launch and its
- * workgroup count) on this batch, keep the fastest as the process tuning and
- * write the timings as JSON to report (cap bytes). */
+/* Time the decode launch shapes on this batch by coordinate descent (entropy
+ * decoder, persistent launch, its workgroup count and length buckets, the
+ * k_index lane-order buckets, k_recon blocks per workgroup), keep the
+ * fastest for batches of the same signature (chunk count and mean blob
+ * length as bit lengths, mixed or single quality) -- later decodes of such
+ * batches use it -- and write the timings as JSON to report (cap bytes). */
 int          mchip_autotune_decode(mchip_ctx *ctx, const uint8_t *d_blobs, const uint64_t *blob_off,
                                    const uint64_t *blob_len, const float *q, int n, uint8_t *d_out,
                                    uint64_t out_stride, char *report, size_t cap);
+
+/* The five launches of one mchip_encode_chunks sequence (n chunks, at most
+ * one encode sequence) captured as a HIP graph over a private workspace;
+ * mchip_graph_launch replays it on ctx's stream, encoding the rasters at
+ * the captured addresses into the captured output, and
+ * mchip_encode_graph_results (after the replay) gives blob_len and status
+ * as mchip_encode_chunks does. */
+mchip_graph *mchip_encode_graph_create(mchip_ctx *ctx, const uint8_t *d_in, uint64_t in_stride, const float *q,
+                                       const int *tau, const float *q_store, int n, uint8_t *d_out,
+                                       const uint64_t *out_off, const uint64_t *out_cap);
+int          mchip_encode_graph_results(mchip_graph *g, uint64_t *blob_len, int *status);
 
 /* ---- Device-resident block store and serving loop ------------------------
  * Replaces the reference's decoded-block cache (mc_cache, S3-FIFO over host

@@ -89,6 +89,10 @@ struct Tuning {
     int persistent = 1;       // batched decode (>= persist_min chunks): persistent entropy launch
     int persist_min = 2;
     int persist_wg = 0;       // workgroups of the persistent launch (0: occupancy x CUs)
+    int len_shift = 3;        // k_index lane order: length buckets of 1 << len_shift bytes (2..5)
+    int group_shift = 6;      // k_group_order: first-block length buckets of 1 << group_shift bytes (4..8)
+    int recon_bpw = 1;        // k_recon blocks per workgroup (1, 2, 4)
+    int enc_keep_mib = 2048;  // encode workspace a context keeps after a call (MiB); more is freed
     int merge = 1;            // concurrent one-chunk host calls join one batched decode
     int merge_inflight = 2;   // merged batches in flight at most (1..4)
     int merge_min = 8;        // queued calls that start a batch while another is in flight
@@ -97,6 +101,41 @@ std::mutex g_tune_mu;
 Tuning g_tune;
 Tuning tuning() {
     std::lock_guard<std::mutex> lk(g_tune_mu);
+    return g_tune;
+}
+
+// Decode launch shapes chosen by mchip_autotune_decode, per workload
+// signature: chunk count and mean blob length as bit lengths, and whether
+// the qualities differ.  Guarded by g_tune_mu.
+std::map<uint32_t, Tuning> g_tuned;
+
+uint32_t workload_signature(int n, const uint64_t *len, const float *q) {
+    uint64_t tot = 0;
+    bool mixed = false;
+    for (int i = 0; i < n; ++i) {
+        tot += len[i];
+        mixed |= q[i] != q[0];
+    }
+    const uint64_t mean = n > 0 ? tot / (uint64_t)n : 0;
+    auto bits = [](uint64_t x) { uint32_t b = 0; while (x) { b++; x >>= 1; } return b; };
+    return bits((uint64_t)n) | (bits(mean >> 10) << 8) | ((mixed ? 1u : 0u) << 16);
+}
+
+// The tuning a decode of this batch uses: the autotuned one for its
+// signature, else the process tuning.
+Tuning tuning_for(int n, const uint64_t *len, const float *q) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (!g_tuned.empty()) {
+        auto it = g_tuned.find(workload_signature(n, len, q));
+        if (it != g_tuned.end()) {
+            Tuning t = it->second;
+            t.enc_batch = g_tune.enc_batch;     // only the decode launch shapes come from the cache
+            t.merge = g_tune.merge;
+            t.merge_inflight = g_tune.merge_inflight;
+            t.merge_min = g_tune.merge_min;
+            return t;
+        }
+    }
     return g_tune;
 }
 
@@ -478,6 +517,9 @@ static int prepare_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
     a.n = n;
     a.decoder = tn.decoder;
     a.persistent = tn.persistent && n >= tn.persist_min && a.decoder == 2;
+    a.len_shift = tn.len_shift;
+    a.group_shift = tn.group_shift;
+    a.recon_bpw = tn.recon_bpw;
     a.nwg = 0;
     a.order = nullptr;
     a.pctl = nullptr;
@@ -550,6 +592,27 @@ int mchip_set_tuning(const char *key, int value) {
         return 0;
     }
     if (!strcmp(key, "merge")) { g_tune.merge = value != 0; return 0; }
+    if (!strcmp(key, "len_shift")) {
+        if (value < 2 || value > 5) return fail("len_shift must be in [2, 5]");
+        g_tune.len_shift = value;
+        return 0;
+    }
+    if (!strcmp(key, "group_shift")) {
+        if (value < 4 || value > 8) return fail("group_shift must be in [4, 8]");
+        g_tune.group_shift = value;
+        return 0;
+    }
+    if (!strcmp(key, "recon_bpw")) {
+        if (value != 1 && value != 2 && value != 4) return fail("recon_bpw must be 1, 2 or 4");
+        g_tune.recon_bpw = value;
+        return 0;
+    }
+    if (!strcmp(key, "tuned_clear")) { g_tuned.clear(); return 0; }
+    if (!strcmp(key, "enc_keep_mib")) {
+        if (value < 0) return fail("enc_keep_mib must be >= 0");
+        g_tune.enc_keep_mib = value;
+        return 0;
+    }
     if (!strcmp(key, "merge_inflight")) {
         if (value < 1 || value > 4) return fail("merge_inflight must be in [1, 4]");
         g_tune.merge_inflight = value;
@@ -644,7 +707,8 @@ int mchip_decode_chunks(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *bl
     if (!c) return fail("null context");
     if (n <= 0) return 0;
     DecodeArgs a;
-    if (prepare_decode(c, d_blobs, blob_off, blob_len, q, n, d_out, out_stride, tuning(), a)) return -1;
+    if (prepare_decode(c, d_blobs, blob_off, blob_len, q, n, d_out, out_stride, tuning_for(n, blob_len, q), a))
+        return -1;
     launch_decode(c, a);
     HIPCHK(hipGetLastError());
     return 0;
@@ -687,7 +751,7 @@ int mchip_decode_chunks_static(const uint8_t *d_blobs, const uint64_t *blob_off,
     if (need > kStaticScratchBytes)
         return fail("mchip_decode_chunks_static: these blobs need %zu bytes of workspace; the fixed workspace "
                     "holds %zu", need, kStaticScratchBytes);
-    const Tuning tn = tuning();
+    const Tuning tn = tuning_for(n, blob_len, q);
     DecodeArgs a;
     a.blobs = d_blobs;
     a.cd = dref;
@@ -701,6 +765,9 @@ int mchip_decode_chunks_static(const uint8_t *d_blobs, const uint64_t *blob_off,
     a.n = n;
     a.decoder = tn.decoder;
     a.persistent = tn.persistent && n >= tn.persist_min && a.decoder == 2;
+    a.len_shift = tn.len_shift;
+    a.group_shift = tn.group_shift;
+    a.recon_bpw = tn.recon_bpw;
     a.order = (uint32_t *)(base + o_order);
     a.pctl = a.order + (size_t)n * 64;
     a.nwg = 0;
@@ -722,6 +789,7 @@ struct mchip_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t stream = nullptr;   // the stream replays are launched on
+    int n = 0;                      // chunks of an encode graph (0: a decode graph)
 };
 
 mchip_graph *mchip_decode_graph_create(mchip_ctx *caller, const uint8_t *d_blobs, const uint64_t *blob_off,
@@ -733,7 +801,8 @@ mchip_graph *mchip_decode_graph_create(mchip_ctx *caller, const uint8_t *d_blobs
     g->ctx = mchip_ctx_create();
     DecodeArgs a;
     bool ok = g->ctx &&
-              !prepare_decode(g->ctx, d_blobs, blob_off, blob_len, q, n, d_out, out_stride, tuning(), a) &&
+              !prepare_decode(g->ctx, d_blobs, blob_off, blob_len, q, n, d_out, out_stride,
+                              tuning_for(n, blob_len, q), a) &&
               hipStreamSynchronize(g->ctx->stream) == hipSuccess;
     if (ok) {
         // the launches only (no copies, no memsets): k_group_order resets the
@@ -773,27 +842,23 @@ void mchip_graph_destroy(mchip_graph *g) {
 
 int mchip_autotune_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *blob_off, const uint64_t *blob_len,
                           const float *q, int n, uint8_t *d_out, uint64_t out_stride, char *report, size_t cap) {
-    if (!c || n <= 0) return fail("mchip_autotune_decode: bad argument");
-    // the candidates run with private tunings; the process tuning changes
-    // only at the end, in one step, so concurrent decodes never see a mix
-    const Tuning base = tuning();
-    struct Cand { int decoder, persistent, wg_div; };
-    std::vector<Cand> cands = {{2, 1, 1}, {2, 1, 2}, {2, 0, 1}, {0, 0, 1}};
-    if (n < 2) cands = {{2, 0, 1}, {0, 0, 1}};
+    if (!c || n <= 0 || !blob_len || !q) return fail("mchip_autotune_decode: bad argument");
+    // Coordinate descent over the decode launch shapes, one knob at a time
+    // from the batch's current tuning, each candidate timed (best of two)
+    // with a private tuning; the winner is cached for this batch's signature
+    // (chunk count, mean blob length, mixed qualities), so concurrent
+    // decodes never see a partial change.
+    const int full = persistent_grid();
+    Tuning best = tuning_for(n, blob_len, q);
+    best.persist_min = 1;
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     std::string rep = "[";
-    float best = 1e30f;
-    Tuning pick = base;
-    for (size_t k = 0; k < cands.size(); ++k) {
-        Tuning tn = base;
-        tn.decoder = cands[k].decoder;
-        tn.persistent = cands[k].persistent;
-        tn.persist_min = 1;
-        tn.persist_wg = cands[k].persistent ? persistent_grid() / cands[k].wg_div : 0;
+    int nrep = 0;
+    auto time_of = [&](const Tuning &tn, float *out_ms) -> int {
         float t = 1e30f;
-        for (int rep_i = 0; rep_i < 2; ++rep_i) {                // best of two runs
+        for (int rep_i = 0; rep_i < 2; ++rep_i) {
             DecodeArgs a;
             HIPCHK(hipEventRecord(e0, c->stream));
             if (prepare_decode(c, d_blobs, blob_off, blob_len, q, n, d_out, out_stride, tn, a)) return -1;
@@ -805,14 +870,49 @@ int mchip_autotune_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
             HIPCHK(hipEventElapsedTime(&ms, e0, e1));
             if (ms < t) t = ms;
         }
-        char buf[160];
-        snprintf(buf, sizeof buf, "%s{\"decoder\": %d, \"persistent\": %d, \"persist_wg\": %d, \"ms\": %.3f}",
-                 k ? ", " : "", tn.decoder, tn.persistent, tn.persist_wg, t);
+        char buf[256];
+        snprintf(buf, sizeof buf,
+                 "%s{\"decoder\": %d, \"persistent\": %d, \"persist_wg\": %d, \"recon_bpw\": %d, "
+                 "\"len_shift\": %d, \"group_shift\": %d, \"ms\": %.3f}",
+                 nrep++ ? ", " : "", tn.decoder, tn.persistent, tn.persist_wg, tn.recon_bpw, tn.len_shift,
+                 tn.group_shift, t);
         rep += buf;
-        if (t < best) {
-            best = t;
-            pick = tn;
-            pick.persist_min = n < base.persist_min ? n : base.persist_min;
+        *out_ms = t;
+        return 0;
+    };
+    float best_ms;
+    if (time_of(best, &best_ms)) return -1;
+    enum Knob { kDecoder, kPersistent, kPersistWg, kReconBpw, kLenShift, kGroupShift };
+    struct Step { Knob k; std::vector<int> vals; };
+    std::vector<Step> steps = {
+        {kDecoder, {2, 0}},
+        {kPersistent, n >= 2 ? std::vector<int>{1, 0} : std::vector<int>{0}},
+        {kPersistWg, {full, full * 3 / 4, full / 2, full / 4}},
+        {kReconBpw, {1, 2, 4}},
+        {kLenShift, {2, 3, 4}},
+        {kGroupShift, {5, 6, 7}},
+    };
+    for (const Step &st : steps) {
+        for (int v : st.vals) {
+            Tuning tn = best;
+            switch (st.k) {
+            case kDecoder: if (tn.decoder == v) continue; tn.decoder = v; break;
+            case kPersistent: if (tn.persistent == v) continue; tn.persistent = v; break;
+            case kPersistWg:
+                if (!tn.persistent || tn.decoder != 2 || n < 2) continue;
+                if ((tn.persist_wg ? tn.persist_wg : full) == v) continue;
+                tn.persist_wg = v;
+                break;
+            case kReconBpw: if (tn.recon_bpw == v) continue; tn.recon_bpw = v; break;
+            case kLenShift: if (tn.len_shift == v) continue; tn.len_shift = v; break;
+            case kGroupShift:
+                if (!tn.persistent || tn.decoder != 2 || n < 2 || tn.group_shift == v) continue;
+                tn.group_shift = v;
+                break;
+            }
+            float ms;
+            if (time_of(tn, &ms)) return -1;
+            if (ms < best_ms * 0.99f) { best_ms = ms; best = tn; }
         }
     }
     rep += "]";
@@ -820,16 +920,9 @@ int mchip_autotune_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
     (void)hipEventDestroy(e1);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tune.decoder = pick.decoder;
-        g_tune.persistent = pick.persistent;
-        if (pick.persistent) {
-            g_tune.persist_min = pick.persist_min;
-            g_tune.persist_wg = pick.persist_wg;
-        }
+        g_tuned[workload_signature(n, blob_len, q)] = best;
     }
-    if (report && cap) {
-        snprintf(report, cap, "%s", rep.c_str());
-    }
+    if (report && cap) snprintf(report, cap, "%s", rep.c_str());
     return 0;
 }
 
@@ -931,8 +1024,8 @@ struct Merger {
                 if (hipStreamWaitEvent(c->stream, b[i]->ev_in, 0) != hipSuccess)
                     rc = fail("merged decode: stream wait failed");
             DecodeArgs a;
-            if (!rc) rc = prepare_decode(c, d_blob, off.data(), len.data(), q.data(), n, d_out, kChunkBytes, tn, a,
-                                         oat.data());
+            if (!rc) rc = prepare_decode(c, d_blob, off.data(), len.data(), q.data(), n, d_out, kChunkBytes,
+                                         tuning_for(n, len.data(), q.data()), a, oat.data());
             if (!rc) {
                 launch_decode(c, a);
                 if (hipGetLastError() != hipSuccess || hipEventRecord(batch->ev, c->stream) != hipSuccess)
@@ -1137,6 +1230,85 @@ static int enc_batch_for() {
     return (int)(eb < 16 ? 16 : (eb > 256 ? 256 : eb));
 }
 
+}  // extern "C"
+
+// Descriptors, workspace and launch arguments of an encode of m chunks at
+// d_in + i*in_stride (everything but the launches); q, tau, q_store,
+// out_off and out_cap are indexed from 0.
+static int prepare_encode(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride, const float *q, const int *tau,
+                          const float *q_store, int m, uint8_t *d_out, const uint64_t *out_off,
+                          const uint64_t *out_cap, EncodeArgs &a) {
+    void *h = nullptr;
+    if (desc_buffer(c, sizeof(EncChunkDesc) * (size_t)m, &h)) return -1;
+    EncChunkDesc *cd = (EncChunkDesc *)h;
+    for (int i = 0; i < m; ++i) {
+        uint32_t slot;
+        if (qtab_slot(q[i], &slot)) return -1;
+        cd[i].in_off = (uint64_t)i * in_stride;
+        cd[i].out_off = out_off[i];
+        cd[i].out_cap = out_cap[i];
+        cd[i].qtab = slot;
+        cd[i].tau = tau ? (tau[i] < 0 ? 0 : tau[i]) : 0;
+        cd[i].q_store = q_store ? q_store[i] : q[i];
+        cd[i].pad = 0;
+    }
+    const size_t nb = (size_t)m * kNVox;
+    if (ensure(c, &c->d_ecd, &c->cap_ecd, sizeof(EncChunkDesc) * m) ||
+        ensure(c, &c->d_einfo, &c->cap_einfo, sizeof(uint4) * nb) ||
+        ensure(c, &c->d_levels, &c->cap_levels, sizeof(int16_t) * nb * kNVox) ||
+        ensure(c, &c->d_corr, &c->cap_corr, sizeof(uint32_t) * nb * kNVox) ||
+        ensure(c, &c->d_slots, &c->cap_slots, (size_t)kSlotBytes * nb) ||
+        ensure(c, &c->d_plen, &c->cap_plen, sizeof(uint32_t) * nb) ||
+        ensure(c, &c->d_dst, &c->cap_dst, sizeof(uint32_t) * nb) ||
+        ensure(c, &c->d_res, &c->cap_res, sizeof(EncChunkResult) * m) ||
+        ensure(c, &c->d_fmap, &c->cap_fmap, (size_t)kFmapStride * m))
+        return -1;
+    HIPCHK(hipMemcpyAsync(c->d_ecd, cd, sizeof(EncChunkDesc) * m, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->ev_desc, c->stream));
+    c->desc_pending = true;
+    a.in = d_in;
+    a.cd = (const EncChunkDesc *)c->d_ecd;
+    a.qtabs = qtabs_now();
+    a.einfo = (uint4 *)c->d_einfo;
+    a.levels = (int16_t *)c->d_levels;
+    a.corr = (uint32_t *)c->d_corr;
+    a.slots = (uint32_t *)c->d_slots;
+    a.plen = (uint32_t *)c->d_plen;
+    a.dst = (uint32_t *)c->d_dst;
+    a.out = d_out;
+    a.res = (EncChunkResult *)c->d_res;
+    a.fmap = (uint32_t *)c->d_fmap;
+    a.n = m;
+    return 0;
+}
+
+static void launch_encode(mchip_ctx *c, const EncodeArgs &a) {
+    timed(c, "enc_transform", [&] { launch_enc_transform(a, c->stream); });
+    timed(c, "enc_entropy", [&] { launch_enc_entropy(a, c->stream); });
+    timed(c, "enc_assemble", [&] { launch_enc_assemble(a, c->stream); });
+    timed(c, "enc_copy", [&] { launch_enc_copy(a, c->stream); });
+    timed(c, "enc_hash", [&] { launch_enc_hash(a, c->stream); });
+}
+
+// Frees the context's encode workspace above the retention limit (tuning
+// "enc_keep_mib"), once the stream is idle.
+static int trim_encode_workspace(mchip_ctx *c) {
+    const size_t keep = (size_t)tuning().enc_keep_mib << 20;
+    const size_t held = c->cap_levels + c->cap_corr + c->cap_slots;
+    if (held <= keep) return 0;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    void **bufs[] = {&c->d_levels, &c->d_corr, &c->d_slots};
+    size_t *caps[] = {&c->cap_levels, &c->cap_corr, &c->cap_slots};
+    for (int i = 0; i < 3; ++i) {
+        if (*bufs[i]) HIPCHK(hipFree(*bufs[i]));
+        *bufs[i] = nullptr;
+        *caps[i] = 0;
+    }
+    return 0;
+}
+
+extern "C" {
+
 int mchip_encode_chunks(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride, const float *q, const int *tau,
                         const float *q_store, int n, uint8_t *d_out, const uint64_t *out_off,
                         const uint64_t *out_cap, uint64_t *blob_len, int *status) {
@@ -1147,60 +1319,17 @@ int mchip_encode_chunks(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride, c
     const int eb = enc_batch_for();
     for (int b0 = 0; b0 < n; b0 += eb) {
         const int m = n - b0 < eb ? n - b0 : eb;
-        void *h = nullptr;
-        if (desc_buffer(c, sizeof(EncChunkDesc) * (size_t)m, &h)) return -1;
-        EncChunkDesc *cd = (EncChunkDesc *)h;
-        for (int i = 0; i < m; ++i) {
-            const int k = b0 + i;
-            uint32_t slot;
-            if (qtab_slot(q[k], &slot)) return -1;
-            cd[i].in_off = (uint64_t)i * in_stride;     // relative to the sub-batch base
-            cd[i].out_off = out_off[k];
-            cd[i].out_cap = out_cap[k];
-            cd[i].qtab = slot;
-            cd[i].tau = tau ? (tau[k] < 0 ? 0 : tau[k]) : 0;
-            cd[i].q_store = q_store ? q_store[k] : q[k];
-            cd[i].pad = 0;
-        }
-        const size_t nb = (size_t)m * kNVox;
-        if (ensure(c, &c->d_ecd, &c->cap_ecd, sizeof(EncChunkDesc) * m) ||
-            ensure(c, &c->d_einfo, &c->cap_einfo, sizeof(uint4) * nb) ||
-            ensure(c, &c->d_levels, &c->cap_levels, sizeof(int16_t) * nb * kNVox) ||
-            ensure(c, &c->d_corr, &c->cap_corr, sizeof(uint32_t) * nb * kNVox) ||
-            ensure(c, &c->d_slots, &c->cap_slots, (size_t)kSlotBytes * nb) ||
-            ensure(c, &c->d_plen, &c->cap_plen, sizeof(uint32_t) * nb) ||
-            ensure(c, &c->d_dst, &c->cap_dst, sizeof(uint32_t) * nb) ||
-            ensure(c, &c->d_res, &c->cap_res, sizeof(EncChunkResult) * m) ||
-            ensure(c, &c->d_fmap, &c->cap_fmap, (size_t)kFmapStride * m))
-            return -1;
         if ((size_t)m > c->cap_h_res) {
             if (c->h_res) HIPCHK(hipHostFree(c->h_res));
             c->h_res = nullptr;
             HIPCHK(hipHostMalloc(&c->h_res, sizeof(EncChunkResult) * m, hipHostMallocDefault));
             c->cap_h_res = m;
         }
-        HIPCHK(hipMemcpyAsync(c->d_ecd, cd, sizeof(EncChunkDesc) * m, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipEventRecord(c->ev_desc, c->stream));
-        c->desc_pending = true;
         EncodeArgs a;
-        a.in = d_in + (size_t)b0 * in_stride;
-        a.cd = (const EncChunkDesc *)c->d_ecd;
-        a.qtabs = qtabs_now();
-        a.einfo = (uint4 *)c->d_einfo;
-        a.levels = (int16_t *)c->d_levels;
-        a.corr = (uint32_t *)c->d_corr;
-        a.slots = (uint32_t *)c->d_slots;
-        a.plen = (uint32_t *)c->d_plen;
-        a.dst = (uint32_t *)c->d_dst;
-        a.out = d_out;
-        a.res = (EncChunkResult *)c->d_res;
-        a.fmap = (uint32_t *)c->d_fmap;
-        a.n = m;
-        timed(c, "enc_transform", [&] { launch_enc_transform(a, c->stream); });
-        timed(c, "enc_entropy", [&] { launch_enc_entropy(a, c->stream); });
-        timed(c, "enc_assemble", [&] { launch_enc_assemble(a, c->stream); });
-        timed(c, "enc_copy", [&] { launch_enc_copy(a, c->stream); });
-        timed(c, "enc_hash", [&] { launch_enc_hash(a, c->stream); });
+        if (prepare_encode(c, d_in + (size_t)b0 * in_stride, in_stride, q + b0, tau ? tau + b0 : nullptr,
+                           q_store ? q_store + b0 : nullptr, m, d_out, out_off + b0, out_cap + b0, a))
+            return -1;
+        launch_encode(c, a);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(c->h_res, c->d_res, sizeof(EncChunkResult) * m, hipMemcpyDeviceToHost, c->stream));
         if (mchip_ctx_sync(c)) return -1;
@@ -1209,6 +1338,61 @@ int mchip_encode_chunks(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride, c
             blob_len[b0 + i] = r[i].blob_len;
             if (status) status[b0 + i] = (int)r[i].status;
         }
+    }
+    return trim_encode_workspace(c);
+}
+
+mchip_graph *mchip_encode_graph_create(mchip_ctx *caller, const uint8_t *d_in, uint64_t in_stride, const float *q,
+                                       const int *tau, const float *q_store, int n, uint8_t *d_out,
+                                       const uint64_t *out_off, const uint64_t *out_cap) {
+    if (!caller || n <= 0 || !d_in || !q || !d_out || !out_off || !out_cap || in_stride % 16 ||
+        ((uintptr_t)d_in) % 16) {
+        fail("mchip_encode_graph_create: bad argument");
+        return nullptr;
+    }
+    if (n > enc_batch_for()) {
+        fail("mchip_encode_graph_create: %d chunks exceed one encode sequence (%d)", n, enc_batch_for());
+        return nullptr;
+    }
+    auto *g = new mchip_graph;
+    g->stream = caller->stream;
+    g->ctx = mchip_ctx_create();
+    g->n = n;
+    EncodeArgs a;
+    bool ok = g->ctx && !prepare_encode(g->ctx, d_in, in_stride, q, tau, q_store, n, d_out, out_off, out_cap, a) &&
+              hipStreamSynchronize(g->ctx->stream) == hipSuccess;
+    if (ok) {
+        // the five launches only: descriptors stay in the graph's workspace
+        // and every kernel rewrites what it reads later, so replays need no
+        // copies or memsets
+        const bool timing = g->ctx->timing;
+        g->ctx->timing = false;
+        ok = hipStreamBeginCapture(g->ctx->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
+            launch_encode(g->ctx, a);
+            ok = hipStreamEndCapture(g->ctx->stream, &g->graph) == hipSuccess && g->graph &&
+                 hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) == hipSuccess;
+        }
+        g->ctx->timing = timing;
+    }
+    if (!ok) {
+        fail("mchip_encode_graph_create: capture failed");
+        mchip_graph_destroy(g);
+        return nullptr;
+    }
+    return g;
+}
+
+int mchip_encode_graph_results(mchip_graph *g, uint64_t *blob_len, int *status) {
+    if (!g || !g->exec || g->n <= 0 || !blob_len) return fail("mchip_encode_graph_results: bad argument");
+    mchip_ctx *c = g->ctx;
+    std::vector<EncChunkResult> r(g->n);
+    HIPCHK(hipStreamSynchronize(g->stream));
+    HIPCHK(hipMemcpyAsync(r.data(), c->d_res, sizeof(EncChunkResult) * g->n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < g->n; ++i) {
+        blob_len[i] = r[i].blob_len;
+        if (status) status[i] = (int)r[i].status;
     }
     return 0;
 }

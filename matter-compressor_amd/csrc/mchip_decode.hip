@@ -57,15 +57,15 @@ __device__ __forceinline__ uint32_t wg_exclusive_scan(uint32_t v, uint32_t *lds4
 // length in 8-byte buckets, so each wave holds blocks of similar length and
 // the longest chains start first.
 constexpr uint32_t kLenBuckets = 1024;
-__device__ __forceinline__ uint32_t len_bucket(uint32_t len) {
-    return kLenBuckets - 1u - min(len >> 3, kLenBuckets - 1u);
+__device__ __forceinline__ uint32_t len_bucket(uint32_t len, uint32_t shift) {
+    return kLenBuckets - 1u - min(len >> shift, kLenBuckets - 1u);
 }
 
 __global__ __launch_bounds__(256) void k_index(const uint8_t *__restrict__ blobs,
                                                const DescRef cd,
                                                BlkRec *__restrict__ rec,
                                                uint16_t *__restrict__ plist,
-                                               ChunkInfo *__restrict__ cinfo) {
+                                               ChunkInfo *__restrict__ cinfo, uint32_t len_shift) {
     __shared__ uint32_t lds4[4];
     __shared__ uint32_t hist[kLenBuckets];
     const int c = blockIdx.x, t = threadIdx.x;
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void k_index(const uint8_t *__restrict__ blobs
             uint64_t a = lens + 2ull * s;
             uint32_t len = b[a] | ((uint32_t)b[a + 1] << 8);
             if (off + len <= L) { r.off = (uint32_t)off; r.len = len; r.slot = (uint32_t)s; }
-            const uint32_t k = len_bucket(len);
+            const uint32_t k = len_bucket(len, len_shift);
             place[j] = (k << 16) | atomicAdd(&hist[k], 1u);
             off += len;
             s++;
@@ -204,8 +204,9 @@ __global__ __launch_bounds__(64) void k_entropy(const uint8_t *__restrict__ blob
 __global__ __launch_bounds__(1024) void k_group_order(const BlkRec *__restrict__ rec,
                                                       const uint16_t *__restrict__ plist,
                                                       const ChunkInfo *__restrict__ cinfo, int n,
-                                                      uint32_t *__restrict__ order, uint32_t *__restrict__ pctl) {
-    constexpr int kBuckets = 256;             // 64-byte payload-length buckets, longest first
+                                                      uint32_t *__restrict__ order, uint32_t *__restrict__ pctl,
+                                                      uint32_t group_shift) {
+    constexpr int kBuckets = 256;             // (1 << group_shift)-byte payload-length buckets, longest first
     __shared__ uint32_t hist[kBuckets], base[kBuckets];
     const int t = threadIdx.x;
     for (int i = t; i < kBuckets; i += 1024) hist[i] = 0;
@@ -217,7 +218,7 @@ __global__ __launch_bounds__(1024) void k_group_order(const BlkRec *__restrict__
         valid = (uint32_t)k * 64 < np;
         if (!valid) return 0;
         const uint32_t len = rec[(size_t)c * kNVox + plist[(size_t)c * kNVox + k * 64]].len;
-        const uint32_t b = len >> 6;
+        const uint32_t b = len >> group_shift;
         return kBuckets - 1 - (b < kBuckets - 1 ? b : kBuckets - 1);
     };
     for (int g = t; g < ng; g += 1024) {
@@ -453,6 +454,9 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t i, uint32_t n) {
     return (i % 8u) * (n / 8u) + i / 8u;
 }
 
+// kB blocks per workgroup (a tuning knob): workgroup g reconstructs blocks
+// kB*g .. kB*g + kB-1 of the XCD-remapped order one after another.
+template <int kB>
 __global__ __launch_bounds__(256) void k_recon(const uint8_t *__restrict__ blobs,
                                                const DescRef cd,
                                                const QTab *__restrict__ qtabs,
@@ -463,31 +467,29 @@ __global__ __launch_bounds__(256) void k_recon(const uint8_t *__restrict__ blobs
                                                uint8_t *__restrict__ out, int nchunks) {
     __shared__ float tile[256 * kPad];
     __shared__ __attribute__((aligned(16))) uint8_t u8t[kNVox];
-    const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
-    const int c = (int)(g >> 12), bi = (int)(g & 4095);
+    const uint32_t g0 = xcd_remap(blockIdx.x, gridDim.x) * kB;
     const int t = threadIdx.x;
-    if (c >= nchunks) return;
-    const size_t cb = (size_t)c * kNVox;
-    const BlkRec r = rec[cb + bi];
-    uint8_t *ob = out + cd[c].out_off + ((size_t)(bi >> 8) * 16 * 65536) +
-                  ((size_t)((bi >> 4) & 15) * 16 * 256) + (size_t)(bi & 15) * 16;
-    uint4 *orow = (uint4 *)(ob + (size_t)(t >> 4) * 65536 + (size_t)(t & 15) * 256);
-    if (r.off == 0) { *orow = make_uint4(0, 0, 0, 0); return; }
-
-    const uint2 inf = binfo[cb + bi];
-    const QTab *qt = qtabs + cd[c].qtab;
-    const uint32_t status = inf.x >> 16;
-
-    if (status == kStatusOk) {
-        const uint32_t *w = scratch + cd[c].scratch_off + 8ull * (r.off - cinfo[c].pay0) + 32ull * r.slot;
-        recon_from_list(inf, w, qt, tile, u8t);
+    for (int k = 0; k < kB; ++k) {
+        const uint32_t g = g0 + k;
+        const int c = (int)(g >> 12), bi = (int)(g & 4095);
+        if (c >= nchunks) return;
+        const size_t cb = (size_t)c * kNVox;
+        const BlkRec r = rec[cb + bi];
+        uint8_t *ob = out + cd[c].out_off + ((size_t)(bi >> 8) * 16 * 65536) +
+                      ((size_t)((bi >> 4) & 15) * 16 * 256) + (size_t)(bi & 15) * 16;
+        uint4 *orow = (uint4 *)(ob + (size_t)(t >> 4) * 65536 + (size_t)(t & 15) * 256);
+        if (r.off == 0) { *orow = make_uint4(0, 0, 0, 0); continue; }
+        const uint2 inf = binfo[cb + bi];
+        const QTab *qt = qtabs + cd[c].qtab;
+        if ((inf.x >> 16) == kStatusOk) {
+            const uint32_t *w = scratch + cd[c].scratch_off + 8ull * (r.off - cinfo[c].pay0) + 32ull * r.slot;
+            recon_from_list(inf, w, qt, tile, u8t);
+        } else {
+            // single-lane fallback: thread 0 decodes the stream straight into LDS
+            recon_single_lane(blobs + cd[c].blob_off + r.off, r.len, qt, tile, u8t);
+        }
         *orow = ((const uint4 *)u8t)[t];
-        return;
     }
-
-    // Single-lane fallback: thread 0 decodes the stream straight into LDS.
-    recon_single_lane(blobs + cd[c].blob_off + r.off, r.len, qt, tile, u8t);
-    *orow = ((const uint4 *)u8t)[t];
 }
 
 // ---------------------------------------------------------------------------
@@ -502,7 +504,8 @@ hipError_t upload_constants(const float *cmf, const uint16_t *band_start, const 
 }
 
 void launch_index(const DecodeArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_index, dim3(a.n), dim3(256), 0, s, a.blobs, a.cd, a.rec, a.plist, a.cinfo);
+    hipLaunchKernelGGL(k_index, dim3(a.n), dim3(256), 0, s, a.blobs, a.cd, a.rec, a.plist, a.cinfo,
+                       (uint32_t)a.len_shift);
 }
 hipError_t occupancy_persist(int *per_cu) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_entropy_persist, 64, 0);
@@ -510,7 +513,8 @@ hipError_t occupancy_persist(int *per_cu) {
 
 void launch_entropy(const DecodeArgs &a, hipStream_t s) {
     if (a.persistent && a.decoder == kDecLane) {
-        hipLaunchKernelGGL(k_group_order, dim3(1), dim3(1024), 0, s, a.rec, a.plist, a.cinfo, a.n, a.order, a.pctl);
+        hipLaunchKernelGGL(k_group_order, dim3(1), dim3(1024), 0, s, a.rec, a.plist, a.cinfo, a.n, a.order, a.pctl,
+                           (uint32_t)a.group_shift);
         hipLaunchKernelGGL(k_entropy_persist, dim3(a.nwg), dim3(64), 0, s, a.blobs, a.cd, a.qtabs, a.rec, a.plist,
                            a.cinfo, a.scratch, a.binfo, a.order, a.pctl);
         return;
@@ -523,8 +527,16 @@ void launch_entropy(const DecodeArgs &a, hipStream_t s) {
                            a.rec, a.plist, a.cinfo, a.scratch, a.binfo);
 }
 void launch_recon(const DecodeArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_recon, dim3((unsigned)a.n * kNVox), dim3(256), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
-                       a.cinfo, a.scratch, a.binfo, a.out, a.n);
+    const unsigned nb = (unsigned)a.n * kNVox;
+    if (a.recon_bpw == 4)
+        hipLaunchKernelGGL(k_recon<4>, dim3(nb / 4), dim3(256), 0, s, a.blobs, a.cd, a.qtabs, a.rec, a.cinfo,
+                           a.scratch, a.binfo, a.out, a.n);
+    else if (a.recon_bpw == 2)
+        hipLaunchKernelGGL(k_recon<2>, dim3(nb / 2), dim3(256), 0, s, a.blobs, a.cd, a.qtabs, a.rec, a.cinfo,
+                           a.scratch, a.binfo, a.out, a.n);
+    else
+        hipLaunchKernelGGL(k_recon<1>, dim3(nb), dim3(256), 0, s, a.blobs, a.cd, a.qtabs, a.rec, a.cinfo,
+                           a.scratch, a.binfo, a.out, a.n);
 }
 
 // ---------------------------------------------------------------------------

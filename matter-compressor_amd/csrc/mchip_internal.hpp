@@ -58,6 +58,9 @@ struct DecodeArgs {
     // the counter at pctl[0]; pctl[1] holds the number of groups
     int persistent;
     int nwg;
+    int len_shift;          // k_index lane order: payload-length buckets of 1 << len_shift bytes
+    int group_shift;        // k_group_order: first-block length buckets of 1 << group_shift bytes
+    int recon_bpw;          // k_recon blocks per workgroup: 1, 2 or 4
     uint32_t *order;        // [n*64]
     uint32_t *pctl;         // [2]
 };

@@ -53,6 +53,8 @@ def load():
         "mchip_d2d": (i, [vp, vp, vp, C.c_size_t]),
         "mchip_decode_graph_create": (vp, [vp, vp, vp, vp, vp, i, vp, C.c_uint64]),
         "mchip_graph_launch": (i, [vp]),
+        "mchip_encode_graph_create": (vp, [vp, vp, C.c_uint64, vp, vp, vp, i, vp, vp, vp]),
+        "mchip_encode_graph_results": (i, [vp, vp, vp]),
         "mchip_graph_destroy": (None, [vp]),
         "mchip_autotune_decode": (i, [vp, vp, vp, vp, vp, i, vp, C.c_uint64, C.c_char_p, C.c_size_t]),
         "mchip_store_create": (vp, [vp, C.c_uint64, vp, vp, vp, i, i, i]),
@@ -271,9 +273,9 @@ class Ctx:
         offs = np.ascontiguousarray(offs, np.uint64)
         lens = np.ascontiguousarray(lens, np.uint64)
         qs = np.ascontiguousarray(qs, np.float32)
-        rep = C.create_string_buffer(4096)
+        rep = C.create_string_buffer(16384)
         check(load().mchip_autotune_decode(self.h, C.c_void_p(d_blobs.ptr), _ptr(offs), _ptr(lens), _ptr(qs),
-                                           len(offs), C.c_void_p(d_out.ptr), out_stride, rep, 4096))
+                                           len(offs), C.c_void_p(d_out.ptr), out_stride, rep, 16384))
         return _json.loads(rep.value.decode())
 
     def decode_chunks_host(self, blobs):
@@ -294,6 +296,28 @@ class Ctx:
         check(load().mchip_decode_blocks_host(self.h, _ptr_array(arrs), _ptr(lens), _ptr(qs), n, _ptr(out)))
         return out
 
+
+    def encode_graph(self, d_in, qs, taus, d_out, out_offs, out_caps, q_store=None, in_stride=CHUNK_BYTES):
+        """HIP graph of one encode sequence (replay with graph_launch, then
+        encode_graph_results)."""
+        qs = np.ascontiguousarray(qs, np.float32)
+        taus = np.ascontiguousarray(taus, np.int32)
+        offs = np.ascontiguousarray(out_offs, np.uint64)
+        caps = np.ascontiguousarray(out_caps, np.uint64)
+        qst = None if q_store is None else np.ascontiguousarray(q_store, np.float32)
+        g = load().mchip_encode_graph_create(self.h, C.c_void_p(d_in.ptr), in_stride, _ptr(qs), _ptr(taus),
+                                             None if qst is None else _ptr(qst), len(qs), C.c_void_p(d_out.ptr),
+                                             _ptr(offs), _ptr(caps))
+        if not g:
+            raise MchipError(last_error())
+        return g, len(qs)
+
+    @staticmethod
+    def encode_graph_results(g, n):
+        lens = np.zeros(n, np.uint64)
+        st = np.zeros(n, np.int32)
+        check(load().mchip_encode_graph_results(g, _ptr(lens), _ptr(st)))
+        return lens, st
 
     # ---- encode ----------------------------------------------------------
     def encode_chunks(self, d_in, qs, taus, d_out, out_offs, out_caps, q_store=None, in_stride=CHUNK_BYTES):

@@ -178,26 +178,47 @@ def test_persistent_batch_and_graph_replays_equal_eager(gpu_ctx):
     d_tmp.free()
 
 
-def test_autotune_keeps_results_exact(gpu_ctx):
+def test_every_launch_shape_and_the_autotuner_keep_results_exact(gpu_ctx):
+    """Every value of every decode launch-shape knob the autotuner searches
+    decodes a mixed-quality batch exactly as the oracle; so do the autotuned
+    shapes cached for the batch's signature."""
     import mc_amd
-    specs = [dict(seed=s, air=a) for s, a in [(31, 0.4), (32, 0.6)]]
+    specs = [dict(seed=s, air=a) for s, a in [(31, 0.4), (32, 0.6), (33, 0.8)]]
     chunks = _chunks(specs)
-    blobs = [ob.encode_chunk(c, q, 0) for c, q in zip(chunks, [1.0, 16.0])]
+    blobs = [ob.encode_chunk(c, q, t) for c, (q, t) in zip(chunks, [(1.0, 4), (16.0, 0), (6.0, 16)])]
+    want = [ob.decode_chunk(b) for b in blobs]
     d_in, offs, lens, qs = _resident(gpu_ctx, blobs)
     d_out = mc_amd.DeviceBuffer(len(blobs) * mc_amd.CHUNK_BYTES)
-    try:
-        print("autotune", flush=True)
-        rep = gpu_ctx.autotune_decode(d_in, offs, lens, qs, d_out)
-        assert len(rep) >= 2 and all(r["ms"] > 0 for r in rep)
+    got = np.empty((len(blobs), 256, 256, 256), np.uint8)
+    defaults = [("decoder", 2), ("persistent", 1), ("persist_min", 2), ("persist_wg", 0), ("recon_bpw", 1),
+                ("len_shift", 3), ("group_shift", 6)]
+
+    def check(what):
         gpu_ctx.decode_chunks(d_in, offs, lens, qs, d_out)
         gpu_ctx.sync()
-        got = np.empty((len(blobs), 256, 256, 256), np.uint8)
         gpu_ctx.d2h(got, d_out)
         for k in range(len(blobs)):
-            assert np.array_equal(got[k], ob.decode_chunk(blobs[k]))
+            assert np.array_equal(got[k], want[k]), f"{what}: chunk {k}"
+
+    try:
+        knobs = [("decoder", [0, 2]), ("persistent", [0, 1]), ("persist_wg", [0, 64, 300]),
+                 ("recon_bpw", [1, 2, 4]), ("len_shift", [2, 3, 4, 5]), ("group_shift", [4, 5, 6, 7, 8])]
+        for key, vals in knobs:
+            for v in vals:
+                mc_amd.set_tuning(key, v)
+                print(f"{key}={v}", flush=True)
+                check(f"{key}={v}")
+            mc_amd.set_tuning(key, dict(defaults)[key])
+        print("autotune", flush=True)
+        rep = gpu_ctx.autotune_decode(d_in, offs, lens, qs, d_out)
+        assert len(rep) >= 6 and all(r["ms"] > 0 for r in rep)
+        check("autotuned")
+        mc_amd.set_tuning("tuned_clear", 1)
+        check("after clearing the autotuned shapes")
     finally:
-        for k, v in (("decoder", 2), ("persistent", 1), ("persist_min", 2), ("persist_wg", 0)):
+        for k, v in defaults:
             mc_amd.set_tuning(k, v)
+        mc_amd.set_tuning("tuned_clear", 1)
         d_in.free()
         d_out.free()
 

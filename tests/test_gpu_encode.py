@@ -50,3 +50,44 @@ def test_encode_decode_roundtrip_tau_bound(gpu_ctx):
     for k, tau in enumerate([8, 32]):
         err = np.abs(out[k].astype(np.int16) - chunks[k].astype(np.int16))
         assert err.max() <= tau
+
+
+def test_encode_graph_replays_equal_reference(gpu_ctx):
+    """One encode sequence captured as a HIP graph: every replay (with eager
+    encodes and a scrambled output buffer between replays) writes the
+    reference's blobs (manifest), and the graph's results give their
+    lengths."""
+    import mc_amd
+    man = json.load(open(os.path.join(ROOT, "tests", "golden", "chunk_manifest.json")))
+    specs = [man["plain64_q6"][2]] + [man["mixed256"][k] for k in (3, 9, 17)]
+    chunks = mc_amd.synth_chunks([s["seed"] for s in specs], [s["air"] for s in specs])
+    n = len(specs)
+    cap = 20 << 20
+    d_in = mc_amd.DeviceBuffer(n * mc_amd.CHUNK_BYTES)
+    d_out = mc_amd.DeviceBuffer(n * cap)
+    d_tmp = mc_amd.DeviceBuffer(cap)
+    gpu_ctx.h2d(d_in, chunks)
+    offs = np.arange(n, dtype=np.uint64) * cap
+    caps = np.full(n, cap, np.uint64)
+    qs = [s["q"] for s in specs]
+    taus = [s["tau"] for s in specs]
+    print("encode graph", flush=True)
+    g, gn = gpu_ctx.encode_graph(d_in, qs, taus, d_out, offs, caps)
+    try:
+        for rep in range(3):
+            gpu_ctx.encode_chunks(d_in, qs[rep:rep + 1], taus[rep:rep + 1], d_tmp, [0], [cap])
+            gpu_ctx.h2d(d_out, np.full(n * cap, 0x33, np.uint8))
+            gpu_ctx.graph_launch(g)
+            gpu_ctx.sync()
+            lens, st = gpu_ctx.encode_graph_results(g, gn)
+            assert (st == 0).all()
+            for k, s in enumerate(specs):
+                assert int(lens[k]) == s["blob_len"], f"replay {rep} chunk {k} length"
+                b = np.empty(int(lens[k]), np.uint8)
+                gpu_ctx.d2h(b, d_out, offset=int(offs[k]))
+                assert f"{ob.xxh64(b):016x}" == s["blob_xxh64"], f"replay {rep} chunk {k}"
+    finally:
+        gpu_ctx.graph_destroy(g)
+        d_in.free()
+        d_out.free()
+        d_tmp.free()
