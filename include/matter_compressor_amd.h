@@ -164,8 +164,13 @@ int        mc_reader_decode_blocks(mc_reader *r, int n, const uint64_t *chunk_of
  * is a batch of n boxes of dz x dy x dx voxels written to device memory at
  * d_out + i*batch_stride; its voxels equal mc_archive_read_regions' for the
  * same boxes.  Requests submitted while others run are admitted as lanes and
- * arena slots free up and overlap with them on the device.  Chunks appended
- * after mc_serve_open are not seen. */
+ * arena slots free up and overlap with them on the device.  The arena grows
+ * on demand within arena_bytes while the device keeps memory free, and can
+ * be resized live.  Appends are seen: each submit compares the archive's
+ * generation with the last one seen and, after a change, uploads the chunks
+ * whose offsets changed (the reference's mc_cache re-resolution and
+ * invalidation, src/mc_cache.c:690-697, :722-736); chunks outside the grid
+ * a level had at open are not seen. */
 typedef struct mc_serve mc_serve;
 mc_serve *mc_serve_open(mc_archive *a, int lod, uint64_t arena_bytes, int lanes);
 /* LODs lod0 .. lod0+nlods-1 in one store sharing one arena: box requests read
@@ -189,6 +194,17 @@ void      mc_serve_close(mc_serve *s);
 /* decode every present block of the LOD into the arena while it has room
  * (before or between requests); loaded receives the number of blocks */
 int       mc_serve_preload(mc_serve *s, uint64_t *loaded);
+/* re-read the chunks whose offsets changed since the archive generation last
+ * seen (submits do this by themselves); returns the number of chunks the
+ * store's grids do not cover, or -1 */
+int       mc_serve_refresh(mc_serve *s);
+/* re-read one chunk of level lod from the archive now, dropping its decoded
+ * blocks (mc_cache_invalidate_chunk, src/matter_compressor.h:460-463); 0, 1
+ * when the chunk lies outside the store's grid, -1 on failure */
+int       mc_serve_invalidate_chunk(mc_serve *s, int lod, int cz, int cy, int cx);
+/* change the arena budget while requests run (mc_cache_resize,
+ * src/matter_compressor.h:522) */
+int       mc_serve_resize(mc_serve *s, uint64_t arena_bytes);
 /* the device serving loop behind s (for mchip_server_get_stats, include/mchip.h) */
 struct mchip_server *mc_serve_device(mc_serve *s);
 

@@ -67,6 +67,8 @@ int         mchip_ctx_sync(mchip_ctx *ctx);
 /* Device memory helpers (so callers need no HIP headers). */
 void       *mchip_malloc(size_t bytes);
 void        mchip_free(void *dptr);
+/* Free and total device memory in bytes. */
+int         mchip_mem_info(size_t *free_bytes, size_t *total_bytes);
 int         mchip_h2d(mchip_ctx *ctx, void *dst, const void *src, size_t bytes);
 int         mchip_d2h(mchip_ctx *ctx, void *dst, const void *src, size_t bytes);
 int         mchip_memset(mchip_ctx *ctx, void *dst, int value, size_t bytes);
@@ -302,13 +304,23 @@ typedef struct {
     uint64_t misses;          /* decoded into the arena */
     uint64_t evictions;
     uint64_t deferred;        /* admissions postponed for lanes or arena slots */
-    uint64_t arena_slots;
+    uint64_t arena_slots;     /* usable decoded-block slots now */
     uint64_t renders;         /* render requests submitted */
+    uint64_t arena_grows;     /* arena segments mapped on demand */
+    uint64_t arena_mapped_bytes;  /* device memory backing the arena */
+    uint64_t arena_budget_bytes;  /* the most the arena may grow to */
+    uint64_t updated_blocks;  /* block payloads replaced by mchip_server_update_blocks */
 } mchip_server_stats;
 
-/* A serving loop over st with an arena of arena_bytes / 4096 decoded-block
- * slots (0: half of the device memory free at creation) and up to lanes
- * requests in flight, each on its own stream.  A worker thread admits the
+/* A serving loop over st with an arena of at most arena_bytes / 4096
+ * decoded-block slots (0: half of the device memory free at creation) and up
+ * to lanes requests in flight, each on its own stream.  The arena is a
+ * reserved device address range backed on demand (HIP virtual memory): it
+ * starts with one 256 MiB segment and grows by segments when a request needs
+ * slots, as long as its budget allows and the device keeps 2 GiB free;
+ * otherwise resident blocks are replaced (CLOCK).  A request that finds no
+ * slots or memory waits for the requests in flight; it fails only when none
+ * is in flight.  A worker thread admits the
  * submitted requests in arrival order as lanes and arena slots free up
  * (CLOCK replacement; blocks of requests in flight are pinned), so requests
  * that arrive while others run overlap with them on the device.  A request
@@ -347,6 +359,21 @@ int          mchip_server_wait(mchip_server *sv, int64_t ticket, double *latency
  * whole store, every later request is served from decoded blocks only. */
 int          mchip_server_preload(mchip_server *sv, uint64_t *loaded);
 int          mchip_server_get_stats(mchip_server *sv, mchip_server_stats *out);
+/* Change the arena budget while the server runs.  Growing takes effect as
+ * requests need slots; shrinking drops the blocks held in slots beyond the
+ * new size, waits until no request in flight holds one of them and returns
+ * their memory to the device.  (The reference's mc_cache_resize,
+ * src/mc_cache.c:200.) */
+int          mchip_server_resize(mchip_server *sv, uint64_t arena_bytes);
+/* Replace the payloads of n blocks of the store behind sv (after a chunk is
+ * appended again, src/mc_cache.c:690-697): block ids[i] (store index) gets
+ * bytes payload[off[i] .. off[i]+len[i]) coded at quality q[i] (len 0: the
+ * block is now absent).  The bytes are uploaded; decoded copies of these
+ * blocks leave the arena, so requests admitted afterwards decode the new
+ * payloads, while requests already admitted finish with the old ones. */
+int          mchip_server_update_blocks(mchip_server *sv, const uint8_t *payload, uint64_t nbytes, int n,
+                                        const uint64_t *ids, const uint64_t *off, const uint32_t *len,
+                                        const float *q);
 /* Waits for the requests in flight, then frees the lanes and the arena. */
 void         mchip_server_destroy(mchip_server *sv);
 

@@ -682,6 +682,15 @@ void *mchip_malloc(size_t bytes) {
 }
 void mchip_free(void *p) { if (p) (void)hipFree(p); }
 
+int mchip_mem_info(size_t *free_bytes, size_t *total_bytes) {
+    if (mchip_init(-1)) return -1;
+    size_t f = 0, t = 0;
+    HIPCHK(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return 0;
+}
+
 int mchip_h2d(mchip_ctx *c, void *dst, const void *src, size_t n) {
     HIPCHK(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));

@@ -60,8 +60,9 @@ struct StoreLevel {
 
 struct mchip_store {
     int device = 0;
-    uint8_t *d_pay = nullptr;
-    uint64_t nbytes = 0;
+    uint8_t *d_pay = nullptr;          // payload heap: bytes [0, nbytes) in use, cap allocated
+    uint64_t nbytes = 0, cap = 0;
+    std::vector<uint8_t *> old_pay;    // replaced heaps (decodes launched earlier still read them)
     int gz = 0, gy = 0, gx = 0;           // level 0 (box requests)
     std::vector<StoreLevel> lv;
     uint64_t nblocks = 0;
@@ -139,13 +140,103 @@ int grow_pinned(void **p, size_t *cap, size_t need, Graveyard &g) {
     return 0;
 }
 
+// The decoded-block arena: a device address range reserved once and backed
+// by physical segments mapped on demand (HIP virtual memory), so that it
+// grows while free device memory allows and shrinks on request without
+// moving; kernels in flight keep valid addresses.  Without virtual-memory
+// support it is one fixed allocation.
+constexpr size_t kArenaSeg = 256ull << 20;       // 65536 slots per segment
+
+struct Arena {
+    uint8_t *base = nullptr;
+    size_t reserved = 0;                         // address range
+    size_t mapped = 0;                           // backed bytes
+    bool vmm = false;
+    int device = 0;
+    std::vector<hipMemGenericAllocationHandle_t> seg;
+
+    // budget: the most the arena may hold; initial: bytes backed now.
+    int init(int dev, size_t budget, size_t initial) {
+        device = dev;
+        int vm = 0;
+        if (hipDeviceGetAttribute(&vm, hipDeviceAttributeVirtualMemoryManagementSupported, dev) == hipSuccess && vm) {
+            size_t fr = 0, tot = 0;
+            if (hipMemGetInfo(&fr, &tot) != hipSuccess) tot = 0;
+            // the range covers the whole device, so that a resize can raise the budget
+            const size_t span = (std::max(budget, tot) + kArenaSeg - 1) / kArenaSeg * kArenaSeg;
+            void *p = nullptr;
+            if (hipMemAddressReserve(&p, span, 0, nullptr, 0) == hipSuccess) {
+                base = (uint8_t *)p;
+                reserved = span;
+                vmm = true;
+                return grow(initial);
+            }
+        }
+        MCHIP_RT_CHECK(hipMalloc((void **)&base, budget));
+        reserved = mapped = budget;
+        return 0;
+    }
+    // Back the arena up to want bytes (a multiple of kArenaSeg).
+    int grow(size_t want) {
+        if (!vmm) return want <= mapped ? 0 : rt_fail("arena: a fixed allocation of %zu bytes", mapped);
+        if (want > reserved) return rt_fail("arena: %zu bytes exceed the reserved range", want);
+        hipMemAllocationProp prop = {};
+        prop.type = hipMemAllocationTypePinned;
+        prop.location.type = hipMemLocationTypeDevice;
+        prop.location.id = device;
+        hipMemAccessDesc acc = {};
+        acc.location = prop.location;
+        acc.flags = hipMemAccessFlagsProtReadWrite;
+        while (mapped < want) {
+            hipMemGenericAllocationHandle_t h;
+            MCHIP_RT_CHECK(hipMemCreate(&h, kArenaSeg, &prop, 0));
+            if (hipMemMap(base + mapped, kArenaSeg, 0, h, 0) != hipSuccess) {
+                (void)hipMemRelease(h);
+                return rt_fail("arena: mapping a segment failed");
+            }
+            if (hipMemSetAccess(base + mapped, kArenaSeg, &acc, 1) != hipSuccess) {
+                (void)hipMemUnmap(base + mapped, kArenaSeg);
+                (void)hipMemRelease(h);
+                return rt_fail("arena: enabling access to a segment failed");
+            }
+            seg.push_back(h);
+            mapped += kArenaSeg;
+        }
+        return 0;
+    }
+    // Release the segments beyond want bytes; no work may use them.
+    int shrink(size_t want) {
+        while (vmm && mapped > want && !seg.empty()) {
+            mapped -= kArenaSeg;
+            MCHIP_RT_CHECK(hipMemUnmap(base + mapped, kArenaSeg));
+            MCHIP_RT_CHECK(hipMemRelease(seg.back()));
+            seg.pop_back();
+        }
+        return 0;
+    }
+    void release() {
+        if (vmm) {
+            (void)shrink(0);
+            if (base) (void)hipMemAddressFree(base, reserved);
+        } else if (base) {
+            (void)hipFree(base);
+        }
+        base = nullptr;
+    }
+};
+
 }  // namespace
 
 struct mchip_server {
     mchip_store *st = nullptr;
-    uint8_t *d_arena = nullptr;
+    Arena arena;
+    uint8_t *d_arena = nullptr;      // arena.base
+    size_t budget = 0;               // bytes the arena may grow to
+    size_t headroom = 2ull << 30;    // device memory the arena leaves free when it grows
     int32_t *d_table = nullptr;      // per block: arena slot, written by the decoding request
-    uint32_t nslots = 0;
+    uint32_t nslots = 0;             // usable slots (the backed part of the arena)
+    std::vector<int32_t> freelist;   // free slots below nslots
+    hipStream_t upd_stream = nullptr;    // uploads of updated payloads
     std::vector<int32_t> slot_of;    // per block: arena slot or -1
     std::vector<int32_t> blk_of;     // per slot: block or -1
     std::vector<int32_t> pins;       // per slot: requests in flight that reference it
@@ -164,41 +255,94 @@ struct mchip_server {
     mchip_server_stats stats{};
     Graveyard grave;
 
+    // More usable slots: up to the budget, from the backed part of the
+    // arena, mapping one more segment when the budget and the free device
+    // memory (beyond the headroom) allow; the new slots join the free list.
+    bool grow_slots() {
+        const uint64_t cap_slots = std::min<uint64_t>(budget / kNVox, 0x7FFFFFFFull);
+        if (nslots >= cap_slots) return false;
+        if ((uint64_t)nslots * kNVox >= arena.mapped) {
+            if (!arena.vmm) return false;
+            size_t fr = 0, tot = 0;
+            if (hipMemGetInfo(&fr, &tot) != hipSuccess || fr < kArenaSeg + headroom) return false;
+            if (arena.grow(arena.mapped + kArenaSeg)) return false;
+            stats.arena_grows++;
+        }
+        const uint32_t n0 = nslots, n1 = (uint32_t)std::min<uint64_t>(cap_slots, arena.mapped / kNVox);
+        blk_of.resize(n1, -1);
+        pins.resize(n1, 0);
+        refbit.resize(n1, 0);
+        producer.resize(n1, -1);
+        for (uint32_t x = n1; x-- > n0;) freelist.push_back((int32_t)x);
+        nslots = n1;
+        stats.arena_slots = nslots;
+        stats.arena_mapped_bytes = arena.mapped;
+        return true;
+    }
+
+    // A free slot: from the free list, from a new segment, or by CLOCK
+    // replacement among unpinned slots.  -1: every slot is pinned.
     int evict_one() {
+        if (freelist.empty()) grow_slots();
+        if (!freelist.empty()) {
+            const int32_t x = freelist.back();
+            freelist.pop_back();
+            return x;
+        }
         for (uint32_t k = 0; k < 2 * nslots + 1; ++k) {
-            const uint32_t s = hand;
-            hand = hand + 1 == nslots ? 0 : hand + 1;
-            if (pins[s] > 0) continue;
-            if (blk_of[s] < 0) return (int)s;
-            if (refbit[s]) { refbit[s] = 0; continue; }
-            slot_of[blk_of[s]] = -1;
-            blk_of[s] = -1;
+            const uint32_t x = hand;
+            hand = hand + 1 >= nslots ? 0 : hand + 1;
+            if (pins[x] > 0 || blk_of[x] < 0) continue;
+            if (refbit[x]) { refbit[x] = 0; continue; }
+            slot_of[blk_of[x]] = -1;
+            blk_of[x] = -1;
             stats.evictions++;
-            return (int)s;
+            return (int)x;
         }
         return -1;
     }
 
+    // Drop one pin of slot x; a slot whose block was detached (updated or
+    // retired while in use) becomes free when its last pin goes.
+    void unpin_slot(int32_t x) {
+        pins[x]--;
+        if (pins[x] == 0 && blk_of[x] < 0 && (uint32_t)x < nslots) freelist.push_back(x);
+    }
+
     // Release the request's pins; its decodes are complete.
     void unpin(Req *r) {
-        for (int32_t s : r->pinned) {
-            pins[s]--;
-            if (producer[s] == r->ticket) producer[s] = -1;
+        for (int32_t x : r->pinned) {
+            if (producer[x] == r->ticket) producer[x] = -1;
+            unpin_slot(x);
         }
         r->pinned.clear();
     }
     // Release the request's pins and free the slots it was to decode (not
     // admitted, or its device work failed).
     void rollback(Req *r) {
-        for (int32_t s : r->pinned) {
-            pins[s]--;
-            if (producer[s] == r->ticket) {
-                producer[s] = -1;
-                slot_of[blk_of[s]] = -1;
-                blk_of[s] = -1;
+        for (int32_t x : r->pinned) {
+            if (producer[x] == r->ticket) {
+                producer[x] = -1;
+                if (blk_of[x] >= 0) {
+                    slot_of[blk_of[x]] = -1;
+                    blk_of[x] = -1;
+                }
             }
+            unpin_slot(x);
         }
         r->pinned.clear();
+    }
+
+    // Detach block id from its slot: later requests decode it again; the
+    // slot is free now, or when the requests holding it finish.
+    void detach(uint64_t id) {
+        const int32_t x = slot_of[id];
+        if (x < 0) return;
+        slot_of[id] = -1;
+        if (blk_of[x] == (int32_t)id) {
+            blk_of[x] = -1;
+            if (pins[x] == 0 && (uint32_t)x < nslots) freelist.push_back(x);
+        }
     }
 
     // The blocks of one admission attempt.
@@ -281,9 +425,11 @@ struct mchip_server {
         return 0;
     }
 
-    // 1 admitted and issued, 0 not enough free slots now, -1 error (r->err).
+    // 1 admitted and issued, 0 not enough free slots or memory now (r->err
+    // says which), -1 error (r->err).
     int try_admit(Req *r, int li) {
         Lane &L = lanes[li];
+        r->err.clear();
         const mchip_store &S = *st;
         if (r->kind == 1 && !r->touched_ok && touch(r, L)) {
             r->err = std::string("render touch pass failed: ") + mchip_last_error();
@@ -328,6 +474,11 @@ struct mchip_server {
         }
         if (!ok) {                         // not enough unpinned slots now
             rollback(r);
+            char buf[200];
+            snprintf(buf, sizeof buf,
+                     "serving request: the arena's %u slots cannot hold the request (arena budget %zu bytes, "
+                     "device memory left free %zu bytes)", nslots, budget, headroom);
+            r->err = buf;
             return 0;
         }
         // staging: [grid (render)] [jobs] [table entries] [origins | block map (boxes)]
@@ -344,8 +495,13 @@ struct mchip_server {
         };
         if (grow_pinned(&L.h_stage, &L.cap_h, total, grave) || grow_device(&L.d_stage, &L.cap_d, total, grave) ||
             grow_device(&L.d_scratch, &L.cap_scratch, sizeof(uint32_t) * (ad.sw + 64), grave) ||
-            grow_device(&L.d_info, &L.cap_info, sizeof(uint2) * (ad.jobs.size() + 1), grave))
-            return fail_req();
+            grow_device(&L.d_info, &L.cap_info, sizeof(uint2) * (ad.jobs.size() + 1), grave)) {
+            // memory is short: wait for requests in flight to finish (the
+            // caller fails the request only when none is in flight)
+            rollback(r);
+            r->err = std::string("serving request: device or pinned memory is short: ") + mchip_last_error();
+            return 0;
+        }
         uint8_t *h = (uint8_t *)L.h_stage;
         if (gb) memcpy(h, r->grid.data(), gb);
         if (jb) memcpy(h + o_job, ad.jobs.data(), jb);
@@ -426,7 +582,8 @@ struct mchip_server {
                     if (!r->deferred) { r->deferred = true; stats.deferred++; }
                     break;                      // wait for requests in flight to release slots
                 } else {
-                    if (rc == 0) r->err = "serving request: the arena holds fewer slots than one request needs";
+                    if (rc == 0 && r->err.empty())
+                        r->err = "serving request: the arena holds fewer slots than one request needs";
                     r->state = 3;
                     r->t_done = now_ms();
                     pending.pop_front();
@@ -499,8 +656,9 @@ mchip_store *mchip_store_create_levels(const uint8_t *payload, uint64_t nbytes, 
         }
         st->qslot[i] = it->second;
     }
+    const uint64_t cap = nbytes + nbytes / 4 + (64ull << 20);       // room for updated chunks
     if (hipGetDevice(&st->device) != hipSuccess ||
-        hipMalloc((void **)&st->d_pay, nbytes + 256) != hipSuccess ||
+        hipMalloc((void **)&st->d_pay, cap) != hipSuccess ||
         (nbytes && hipMemcpy(st->d_pay, payload, nbytes, hipMemcpyHostToDevice) != hipSuccess)) {
         rt_fail("mchip_store_create: device allocation or upload of %llu bytes failed",
                 (unsigned long long)nbytes);
@@ -508,7 +666,8 @@ mchip_store *mchip_store_create_levels(const uint8_t *payload, uint64_t nbytes, 
         delete st;
         return nullptr;
     }
-    st->nbytes = nbytes;
+    st->nbytes = (nbytes + 255) & ~255ull;
+    st->cap = cap;
     return st;
 }
 
@@ -525,6 +684,7 @@ mchip_store *mchip_store_create(const uint8_t *payload, uint64_t nbytes, const u
 void mchip_store_destroy(mchip_store *st) {
     if (!st) return;
     if (st->d_pay) (void)hipFree(st->d_pay);
+    for (uint8_t *p : st->old_pay) (void)hipFree(p);
     delete st;
 }
 
@@ -542,28 +702,30 @@ mchip_server *mchip_server_create(mchip_store *st, uint64_t arena_bytes, int lan
     if (ns < 1 || ns > 0x7FFFFFFFull) { rt_fail("mchip_server_create: arena of %llu bytes", (unsigned long long)arena_bytes); return nullptr; }
     auto *sv = new mchip_server;
     sv->st = st;
-    sv->nslots = (uint32_t)ns;
-    if (hipMalloc((void **)&sv->d_arena, ns * kNVox) != hipSuccess) {
-        rt_fail("mchip_server_create: arena allocation of %llu bytes failed", (unsigned long long)(ns * kNVox));
+    sv->budget = ns * kNVox;
+    // back one segment now; the arena grows by segments as requests need
+    // slots, while the budget and the free device memory allow
+    const size_t initial = kArenaSeg;
+    if (sv->arena.init(st->device, sv->budget, initial)) {
+        rt_fail("mchip_server_create: arena of %llu bytes: %s", (unsigned long long)sv->budget, mchip_last_error());
         delete sv;
         return nullptr;
     }
+    sv->d_arena = sv->arena.base;
     const size_t nb = (size_t)st->nblocks;
     if (hipMalloc((void **)&sv->d_table, 4 * nb + 4) != hipSuccess ||
-        hipMemset(sv->d_table, 0xFF, 4 * nb + 4) != hipSuccess) {
+        hipMemset(sv->d_table, 0xFF, 4 * nb + 4) != hipSuccess ||
+        hipStreamCreateWithFlags(&sv->upd_stream, hipStreamNonBlocking) != hipSuccess) {
         rt_fail("mchip_server_create: slot table allocation failed");
-        (void)hipFree(sv->d_arena);
+        sv->arena.release();
         if (sv->d_table) (void)hipFree(sv->d_table);
         delete sv;
         return nullptr;
     }
     sv->slot_of.assign(nb, -1);
     sv->mark.assign(nb, 0u);
-    sv->blk_of.assign(ns, -1);
-    sv->pins.assign(ns, 0);
-    sv->refbit.assign(ns, 0);
-    sv->producer.assign(ns, -1);
-    sv->stats.arena_slots = ns;
+    sv->grow_slots();
+    sv->stats.arena_budget_bytes = sv->budget;
     sv->lanes.resize(lanes);
     for (Lane &L : sv->lanes) {
         // initial lane buffers for a batch of 64 boxes of 128^3 (64 * 9^3
@@ -679,7 +841,7 @@ int mchip_server_preload(mchip_server *sv, uint64_t *loaded) {
     std::vector<int32_t> pinned;
     uint64_t n = 0;
     auto unpin_all = [&]() {
-        for (int32_t s : pinned) sv->pins[s]--;
+        for (int32_t s : pinned) sv->unpin_slot(s);
     };
     auto drop_jobs = [&]() {                      // jobs not decoded: their slots hold nothing
         for (const BlockJob &j : jobs) {
@@ -721,7 +883,6 @@ int mchip_server_preload(mchip_server *sv, uint64_t *loaded) {
         if (!S.len[id] || sv->slot_of[id] >= 0) continue;
         const int s = sv->evict_one();
         if (s < 0) break;                         // the arena is full of pinned blocks
-        if (sv->blk_of[s] >= 0) { sv->slot_of[sv->blk_of[s]] = -1; }
         sv->blk_of[s] = (int32_t)id;
         sv->slot_of[id] = s;
         sv->pins[s]++;                            // pinned until the preload ends
@@ -742,6 +903,101 @@ int mchip_server_get_stats(mchip_server *sv, mchip_server_stats *out) {
     if (!sv || !out) return rt_fail("null argument");
     std::lock_guard<std::mutex> lk(sv->mu);
     *out = sv->stats;
+    return 0;
+}
+
+int mchip_server_resize(mchip_server *sv, uint64_t arena_bytes) {
+    if (!sv) return rt_fail("null server");
+    if (arena_bytes < (uint64_t)kNVox || arena_bytes / kNVox > 0x7FFFFFFFull)
+        return rt_fail("mchip_server_resize: arena of %llu bytes", (unsigned long long)arena_bytes);
+    std::unique_lock<std::mutex> lk(sv->mu);
+    if (!sv->arena.vmm && arena_bytes > sv->arena.mapped)
+        return rt_fail("mchip_server_resize: the arena is one fixed allocation of %zu bytes (no virtual memory "
+                       "support)", sv->arena.mapped);
+    const uint32_t ns = (uint32_t)(arena_bytes / kNVox);
+    sv->budget = (size_t)ns * kNVox;
+    sv->stats.arena_budget_bytes = sv->budget;
+    if (ns >= sv->nslots) return 0;                // grows as requests need slots
+    // retire the slots from ns on: no new request gets one, their blocks
+    // are decoded again elsewhere when needed
+    const uint32_t old = sv->nslots;
+    sv->nslots = ns;
+    if (sv->hand >= ns) sv->hand = 0;
+    sv->freelist.erase(std::remove_if(sv->freelist.begin(), sv->freelist.end(),
+                                      [ns](int32_t x) { return (uint32_t)x >= ns; }),
+                       sv->freelist.end());
+    for (uint32_t x = ns; x < old; ++x)
+        if (sv->blk_of[x] >= 0) {
+            sv->slot_of[sv->blk_of[x]] = -1;
+            sv->blk_of[x] = -1;
+        }
+    sv->cv_done.wait(lk, [sv, ns, old] {
+        for (uint32_t x = ns; x < old; ++x)
+            if (sv->pins[x] > 0) return false;
+        return true;
+    });
+    sv->blk_of.resize(ns);
+    sv->pins.resize(ns);
+    sv->refbit.resize(ns);
+    sv->producer.resize(ns);
+    sv->stats.arena_slots = ns;
+    // segments wholly beyond the new size go back to the device
+    if (sv->arena.vmm) {
+        const size_t keep = ((size_t)ns * kNVox + kArenaSeg - 1) / kArenaSeg * kArenaSeg;
+        if (keep < sv->arena.mapped) {
+            MCHIP_RT_CHECK(hipDeviceSynchronize());
+            if (sv->arena.shrink(keep)) return -1;
+        }
+        sv->stats.arena_mapped_bytes = sv->arena.mapped;
+    }
+    return 0;
+}
+
+int mchip_server_update_blocks(mchip_server *sv, const uint8_t *payload, uint64_t nbytes, int n, const uint64_t *ids,
+                               const uint64_t *off, const uint32_t *len, const float *q) {
+    if (!sv) return rt_fail("null server");
+    if (n < 0 || (n && (!ids || !off || !len || !q)) || (nbytes && !payload))
+        return rt_fail("mchip_server_update_blocks: bad argument");
+    mchip_store &S = *sv->st;
+    for (int i = 0; i < n; ++i) {
+        if (ids[i] >= S.nblocks) return rt_fail("mchip_server_update_blocks: block %llu of %llu",
+                                                (unsigned long long)ids[i], (unsigned long long)S.nblocks);
+        if (len[i] && (off[i] > nbytes || len[i] > nbytes - off[i]))
+            return rt_fail("mchip_server_update_blocks: block %d lies outside the payload bytes", i);
+    }
+    std::vector<uint32_t> qs(n, 0);
+    for (int i = 0; i < n; ++i)
+        if (len[i] && rt_qtab_slot(q[i], &qs[i])) return -1;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    const uint64_t need = (nbytes + 255) & ~255ull;
+    if (S.nbytes + need > S.cap) {
+        // a larger heap; decodes launched earlier keep reading the old one
+        const uint64_t cap = std::max<uint64_t>(2 * S.cap, S.nbytes + need + (64ull << 20));
+        uint8_t *p = nullptr;
+        MCHIP_RT_CHECK(hipMalloc((void **)&p, cap));
+        if (hipMemcpyAsync(p, S.d_pay, S.nbytes, hipMemcpyDeviceToDevice, sv->upd_stream) != hipSuccess ||
+            hipStreamSynchronize(sv->upd_stream) != hipSuccess) {
+            (void)hipFree(p);
+            return rt_fail("mchip_server_update_blocks: moving the payload heap failed");
+        }
+        S.old_pay.push_back(S.d_pay);
+        S.d_pay = p;
+        S.cap = cap;
+    }
+    const uint64_t at = S.nbytes;
+    if (nbytes) {
+        MCHIP_RT_CHECK(hipMemcpyAsync(S.d_pay + at, payload, nbytes, hipMemcpyHostToDevice, sv->upd_stream));
+        MCHIP_RT_CHECK(hipStreamSynchronize(sv->upd_stream));
+    }
+    S.nbytes += need;
+    for (int i = 0; i < n; ++i) {
+        const uint64_t id = ids[i];
+        S.off[id] = len[i] ? at + off[i] : 0;
+        S.len[id] = len[i];
+        S.qslot[id] = qs[i];
+        sv->detach(id);
+    }
+    sv->stats.updated_blocks += (uint64_t)n;
     return 0;
 }
 
@@ -770,8 +1026,9 @@ void mchip_server_destroy(mchip_server *sv) {
     for (auto &kv : sv->reqs) delete kv.second;
     for (void *p : sv->grave.dev) (void)hipFree(p);
     for (void *p : sv->grave.host) (void)hipHostFree(p);
-    if (sv->d_arena) (void)hipFree(sv->d_arena);
+    sv->arena.release();
     if (sv->d_table) (void)hipFree(sv->d_table);
+    if (sv->upd_stream) (void)hipStreamDestroy(sv->upd_stream);
     delete sv;
 }
 

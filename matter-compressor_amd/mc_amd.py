@@ -46,6 +46,7 @@ def load():
         "mchip_ctx_stream": (vp, [vp]),
         "mchip_ctx_sync": (i, [vp]),
         "mchip_malloc": (vp, [C.c_size_t]),
+        "mchip_mem_info": (i, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
         "mchip_free": (None, [vp]),
         "mchip_h2d": (i, [vp, vp, vp, C.c_size_t]),
         "mchip_d2h": (i, [vp, vp, vp, C.c_size_t]),
@@ -66,6 +67,8 @@ def load():
         "mchip_server_wait": (i, [vp, C.c_int64, C.POINTER(C.c_double)]),
         "mchip_server_get_stats": (i, [vp, vp]),
         "mchip_server_preload": (i, [vp, C.POINTER(C.c_uint64)]),
+        "mchip_server_resize": (i, [vp, C.c_uint64]),
+        "mchip_server_update_blocks": (i, [vp, vp, C.c_uint64, i, vp, vp, vp, vp]),
         "mchip_server_destroy": (None, [vp]),
         "mchip_decode_chunks": (i, [vp, u8p, vp, vp, vp, i, u8p, C.c_uint64]),
         "mchip_decode_chunks_host": (i, [vp, vp, vp, i, u8p, C.c_uint64]),
@@ -98,6 +101,13 @@ def load():
         f.argtypes = args
     _lib = L
     return L
+
+
+def mem_info():
+    """(free, total) device memory in bytes."""
+    f, t = C.c_size_t(), C.c_size_t()
+    check(load().mchip_mem_info(C.byref(f), C.byref(t)))
+    return f.value, t.value
 
 
 def set_tuning(key, value):
@@ -462,6 +472,9 @@ def api():
         "mc_serve_close": (None, [vp]),
         "mc_serve_device": (vp, [vp]),
         "mc_serve_preload": (i, [vp, C.POINTER(C.c_uint64)]),
+        "mc_serve_refresh": (i, [vp]),
+        "mc_serve_invalidate_chunk": (i, [vp, i, i, i, i]),
+        "mc_serve_resize": (i, [vp, u64]),
         "mc_archive_block_present": (i, [vp, i, i, i, i]),
         "mc_archive_block_fraction": (f, [vp, i, i, i, i]),
         "mc_archive_sample_boxes": (i, [vp, i, u64, i, lng, lng, lng, f, vp]),
@@ -479,7 +492,8 @@ def api():
 
 
 SERVER_STATS = ["requests", "completed", "boxes", "block_refs", "hits", "misses", "evictions", "deferred",
-                "arena_slots", "renders"]
+                "arena_slots", "renders", "arena_grows", "arena_mapped_bytes", "arena_budget_bytes",
+                "updated_blocks"]
 
 
 def server_stats(server_handle):
@@ -541,6 +555,19 @@ class Serve:
 
     def stats(self):
         return server_stats(api().mc_serve_device(self.h))
+
+    def refresh(self):
+        n = api().mc_serve_refresh(self.h)
+        if n < 0:
+            raise MchipError(f"mc_serve_refresh failed: {last_error()}")
+        return n
+
+    def invalidate_chunk(self, lod, cz, cy, cx):
+        return api().mc_serve_invalidate_chunk(self.h, lod, cz, cy, cx)
+
+    def resize(self, arena_bytes):
+        if api().mc_serve_resize(self.h, arena_bytes) != 0:
+            raise MchipError(f"mc_serve_resize failed: {last_error()}")
 
     def close(self):
         if self.h:
