@@ -9,7 +9,11 @@
  * engine (include/mchip.h); the archive file format (v7 .mca) and its
  * index, install and metadata logic are plain C on the host.  A program
  * built against the reference's codec/archive section links against
- * libmc_amd.so instead, unchanged.
+ * libmc_amd.so instead, unchanged, unless it calls mc_build or
+ * mc_build_to_file (src/matter_compressor.h:151-154, the whole-volume
+ * builders over a voxel callback): those are not exported, since the builder is outside
+ * this library; such a program appends its chunks with
+ * mc_archive_append_chunk_par instead.
  */
 #ifndef MATTER_COMPRESSOR_AMD_H
 #define MATTER_COMPRESSOR_AMD_H
