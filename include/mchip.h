@@ -45,8 +45,10 @@ int         mchip_set_priors(const uint16_t *plo, const uint16_t *phi);
 
 /* Launch-shape tuning (process-wide, read as one snapshot per call):
  * "decoder" selects the entropy decoder variant (0 = one loop step per scan
- * position, 2 = one bin per loop step, branch-free, payload in an LDS
- * window; the default); "persistent", "persist_min" and "persist_wg" shape
+ * position; 2 = one bin per loop step, branch-free, payload in an LDS
+ * window; 3 = the same bins with the loop in assembly and lanes that meet a
+ * rare event paused and served together, the default); "persistent",
+ * "persist_min" and "persist_wg" shape
  * the persistent entropy launch of batched decodes; "len_shift" and
  * "group_shift" the payload-length buckets of the lane order and of the
  * persistent group order; "recon_bpw" the reconstruction blocks per

@@ -82,7 +82,7 @@ constexpr uint32_t kMaxQTabs = 1u << 20;     // 504 MiB of tables
 // Launch-shape tuning knobs (mchip_set_tuning); process-wide, read as one
 // snapshot per call (tuning()).
 struct Tuning {
-    int decoder = 2;          // entropy decoder: 0 per scan position, 2 branch-free lane decoder
+    int decoder = 3;          // entropy decoder: 0 per scan position, 2 branch-free lane decoder, 3 its assembly loop
                               // over an LDS payload window
     int enc_batch = 0;        // chunks per encode launch sequence (device workspace ~165 MiB each);
                               // 0: sized from the free device memory (enc_batch_for)
@@ -438,16 +438,17 @@ int collect_timing(mchip_ctx *c) {
 
 // Workgroups of the persistent entropy launch: as many as the device keeps
 // resident at once (occupancy per CU x CUs).
-int persistent_grid() {
-    static std::atomic<int> cached{0};
-    const int have = cached.load(std::memory_order_relaxed);
+int persistent_grid(int decoder) {
+    static std::atomic<int> cached[2];                 // lane decoder, assembly lane decoder
+    std::atomic<int> &slot = cached[decoder == 3 ? 1 : 0];
+    const int have = slot.load(std::memory_order_relaxed);
     if (have) return have;
     int dev = 0, cus = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 1024;
-    if (occupancy_persist(&per) != hipSuccess || per <= 0) per = 4;
-    cached.store(cus * per, std::memory_order_relaxed);
+    if (occupancy_persist(decoder, &per) != hipSuccess || per <= 0) per = 4;
+    slot.store(cus * per, std::memory_order_relaxed);
     return cus * per;
 }
 
@@ -516,7 +517,7 @@ static int prepare_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
     a.out = d_out;
     a.n = n;
     a.decoder = tn.decoder;
-    a.persistent = tn.persistent && n >= tn.persist_min && a.decoder == 2;
+    a.persistent = tn.persistent && n >= tn.persist_min && a.decoder >= 2;
     a.len_shift = tn.len_shift;
     a.group_shift = tn.group_shift;
     a.recon_bpw = tn.recon_bpw;
@@ -527,7 +528,7 @@ static int prepare_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
         if (ensure(c, &c->d_order, &c->cap_order, sizeof(uint32_t) * ((size_t)n * 64 + 64))) return -1;
         a.order = (uint32_t *)c->d_order;
         a.pctl = a.order + (size_t)n * 64;
-        a.nwg = tn.persist_wg > 0 ? tn.persist_wg : persistent_grid();
+        a.nwg = tn.persist_wg > 0 ? tn.persist_wg : persistent_grid(tn.decoder);
         if (a.nwg > n * 64) a.nwg = n * 64;
     }
     return 0;
@@ -587,7 +588,7 @@ int mchip_set_tuning(const char *key, int value) {
         return 0;
     }
     if (!strcmp(key, "decoder")) {
-        if (value != 0 && value != 2) return fail("decoder must be 0 or 2");
+        if (value != 0 && value != 2 && value != 3) return fail("decoder must be 0, 2 or 3");
         g_tune.decoder = value;
         return 0;
     }
@@ -773,7 +774,7 @@ int mchip_decode_chunks_static(const uint8_t *d_blobs, const uint64_t *blob_off,
     a.out = d_out;
     a.n = n;
     a.decoder = tn.decoder;
-    a.persistent = tn.persistent && n >= tn.persist_min && a.decoder == 2;
+    a.persistent = tn.persistent && n >= tn.persist_min && a.decoder >= 2;
     a.len_shift = tn.len_shift;
     a.group_shift = tn.group_shift;
     a.recon_bpw = tn.recon_bpw;
@@ -781,7 +782,7 @@ int mchip_decode_chunks_static(const uint8_t *d_blobs, const uint64_t *blob_off,
     a.pctl = a.order + (size_t)n * 64;
     a.nwg = 0;
     if (a.persistent) {
-        a.nwg = tn.persist_wg > 0 ? tn.persist_wg : persistent_grid();
+        a.nwg = tn.persist_wg > 0 ? tn.persist_wg : persistent_grid(tn.decoder);
         if (a.nwg > n * 64) a.nwg = n * 64;
     }
     hipStream_t s = (hipStream_t)stream;
@@ -857,9 +858,9 @@ int mchip_autotune_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
     // with a private tuning; the winner is cached for this batch's signature
     // (chunk count, mean blob length, mixed qualities), so concurrent
     // decodes never see a partial change.
-    const int full = persistent_grid();
     Tuning best = tuning_for(n, blob_len, q);
     best.persist_min = 1;
+    const int full = persistent_grid(best.decoder);
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -894,7 +895,7 @@ int mchip_autotune_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
     enum Knob { kDecoder, kPersistent, kPersistWg, kReconBpw, kLenShift, kGroupShift };
     struct Step { Knob k; std::vector<int> vals; };
     std::vector<Step> steps = {
-        {kDecoder, {2, 0}},
+        {kDecoder, {3, 2, 0}},
         {kPersistent, n >= 2 ? std::vector<int>{1, 0} : std::vector<int>{0}},
         {kPersistWg, {full, full * 3 / 4, full / 2, full / 4}},
         {kReconBpw, {1, 2, 4}},
@@ -908,14 +909,14 @@ int mchip_autotune_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
             case kDecoder: if (tn.decoder == v) continue; tn.decoder = v; break;
             case kPersistent: if (tn.persistent == v) continue; tn.persistent = v; break;
             case kPersistWg:
-                if (!tn.persistent || tn.decoder != 2 || n < 2) continue;
+                if (!tn.persistent || tn.decoder < 2 || n < 2) continue;
                 if ((tn.persist_wg ? tn.persist_wg : full) == v) continue;
                 tn.persist_wg = v;
                 break;
             case kReconBpw: if (tn.recon_bpw == v) continue; tn.recon_bpw = v; break;
             case kLenShift: if (tn.len_shift == v) continue; tn.len_shift = v; break;
             case kGroupShift:
-                if (!tn.persistent || tn.decoder != 2 || n < 2 || tn.group_shift == v) continue;
+                if (!tn.persistent || tn.decoder < 2 || n < 2 || tn.group_shift == v) continue;
                 tn.group_shift = v;
                 break;
             }

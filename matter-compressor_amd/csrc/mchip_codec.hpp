@@ -125,11 +125,11 @@ __device__ __forceinline__ void decode_header(RcDec &d, const QTab *__restrict__
 // Same bins, same order, same results as decode_coefs.
 //
 // The payload is read from a per-lane window of kWin words (LDS on the GPU,
-// word j at win[(kWinAlloc - 1 - j) * kLaneStride], descending so that one
-// paired read yields words j + 1 and j as the low and high halves) holding the payload's aligned words
-// byte-swapped to MSB-first, zero past the payload end (the reference reads
-// zeros there, src/mc_codec.c:197).  rd_bits is the bit position of the next
-// unread byte inside the window.
+// word j at win[j * kLaneStride]; one paired read with offsets j + 1 and j
+// yields the two words as the low and high halves of a 64-bit value) holding
+// the payload's aligned words byte-swapped to MSB-first, zero past the
+// payload end (the reference reads zeros there, src/mc_codec.c:197).  rd_bits
+// is the bit position of the next unread byte inside the window.
 // ---------------------------------------------------------------------------
 #ifndef MCHIP_LANE_WIN
 #define MCHIP_LANE_WIN 64
@@ -152,7 +152,7 @@ __device__ __noinline__ void window_fill(const uint8_t *p, uint32_t len, int32_t
     for (int32_t j0 = 0; j0 < kWin; j0 += 16) {
         if (first + j0 >= nwords) {                            // rest is past the payload
             #pragma unroll 1
-            for (int32_t j = j0; j < kWin; ++j) win[(kWinAlloc - 1 - j) * kLaneStride] = 0u;
+            for (int32_t j = j0; j < kWin; ++j) win[j * kLaneStride] = 0u;
             break;
         }
         uint32_t a[16];
@@ -166,7 +166,7 @@ __device__ __noinline__ void window_fill(const uint8_t *p, uint32_t len, int32_t
             // aligned word i holds payload bytes 4i - lead .. 4i - lead + 3 (little-endian)
             const int64_t room = (int64_t)len + lead - 4ll * (first + j0 + u);
             const uint32_t m = room >= 4 ? 0xFFFFFFFFu : (room <= 0 ? 0u : 0xFFFFFFFFu >> (8 * (4 - (int32_t)room)));
-            win[(kWinAlloc - 1 - (j0 + u)) * kLaneStride] = __builtin_bswap32(a[u] & m);
+            win[(j0 + u) * kLaneStride] = __builtin_bswap32(a[u] & m);
         }
     }
 }
@@ -183,9 +183,9 @@ struct LaneCoder {
     // Issue the window reads for the current position; they are consumed by
     // the next renormalisation, so their latency overlaps a whole bin.
     __device__ __forceinline__ void fetch() {
-        const uint32_t *q = win + (kWinAlloc - 2 - (rd_bits >> 5)) * kLaneStride;
-        w_lo = q[0];                      // word j + 1
-        w_hi = q[kLaneStride];            // word j
+        const uint32_t *q = win + (rd_bits >> 5) * kLaneStride;
+        w_hi = q[0];                      // word j
+        w_lo = q[kLaneStride];            // word j + 1
     }
     __device__ __forceinline__ uint32_t next32() const {    // 32 bits (MSB first) at rd_bits
         return (uint32_t)(((((uint64_t)w_hi) << 32) | w_lo) << (rd_bits & 24) >> 32);
@@ -366,10 +366,14 @@ __device__ __forceinline__ uint32_t decode_coefs_lane(LaneCoder &d, const QTab *
 }
 
 // Lane decoder entry: header, coefficients and corrections into the list at
-// w (cap words).  Returns the block record like entropy_to_list.
-__device__ __forceinline__ uint2 entropy_to_list_lane(const uint8_t *payload, uint32_t len, const QTab *__restrict__ qt,
-                                                      const uint16_t *band_start, uint32_t *w, uint32_t cap,
-                                                      uint32_t *ctx, uint32_t *win) {
+// w (cap words).  Returns the block record like entropy_to_list.  Coefs is the
+// coefficient decoder (decode_coefs_lane, or the assembly loop of
+// mchip_lane_asm.hpp on the GPU); both read the same bins in the same order.
+using LaneCoefFn = uint32_t (*)(LaneCoder &, const QTab *__restrict__, const uint16_t *, LaneSink &, uint32_t *);
+template <LaneCoefFn Coefs>
+__device__ __forceinline__ uint2 entropy_to_list_lane_with(const uint8_t *payload, uint32_t len,
+                                                           const QTab *__restrict__ qt, const uint16_t *band_start,
+                                                           uint32_t *w, uint32_t cap, uint32_t *ctx, uint32_t *win) {
     LaneCoder d;
     d.init(payload, len, win);
     uint32_t f = 0, dcv = 0;
@@ -378,7 +382,7 @@ __device__ __forceinline__ uint2 entropy_to_list_lane(const uint8_t *payload, ui
     for (int b = 0; b < 8; ++b) dcv = (dcv << 1) | d.bit(qt->dc[b]);
     LaneSink sink;
     sink.init(w, cap);
-    const uint32_t nsig = decode_coefs_lane(d, qt, band_start, sink, ctx);
+    const uint32_t nsig = Coefs(d, qt, band_start, sink, ctx);
     uint32_t nc = 0;
     if (f & 2) {
         uint32_t n = d.expgolomb() + 1, pos = 0;
@@ -396,6 +400,12 @@ __device__ __forceinline__ uint2 entropy_to_list_lane(const uint8_t *payload, ui
     const uint32_t total = sink.count();
     const uint32_t status = total > cap ? kStatusFallback : kStatusOk;
     return make_uint2(dcv | (f << 8) | (status << 16), nsig | (nc << 16));
+}
+
+__device__ __forceinline__ uint2 entropy_to_list_lane(const uint8_t *payload, uint32_t len, const QTab *__restrict__ qt,
+                                                      const uint16_t *band_start, uint32_t *w, uint32_t cap,
+                                                      uint32_t *ctx, uint32_t *win) {
+    return entropy_to_list_lane_with<decode_coefs_lane>(payload, len, qt, band_start, w, cap, ctx, win);
 }
 
 // Entropy-decode one payload into a (raster index, level) list at w (cap

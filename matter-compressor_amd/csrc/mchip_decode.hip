@@ -19,6 +19,7 @@
 // corrupt payloads) are decoded inside k_recon by a single lane straight into
 // the LDS tile, so every payload decodes exactly like the reference.
 #include "mchip_codec.hpp"
+#include "mchip_lane_asm.hpp"
 #include "mchip_device.hpp"
 #include "mchip_internal.hpp"
 
@@ -134,21 +135,26 @@ __global__ __launch_bounds__(256) void k_index(const uint8_t *__restrict__ blobs
 // ---------------------------------------------------------------------------
 // k_entropy: one lane per present block of one chunk (blockIdx.y = chunk).
 // ---------------------------------------------------------------------------
-// Decoder variants (a tuning knob; both give identical results):
+// Decoder variants (a tuning knob; all give identical results):
 //   kDecPosition  one loop iteration per scan position (mchip_codec.hpp
 //                 decode_coefs), payload read through a prefetched bit buffer;
 //   kDecLane      one bin per iteration, branch-free, payload staged in a
-//                 per-lane LDS window (LaneDec).  In SIMT every branch body
+//                 per-lane LDS window (LaneCoder).  In SIMT every branch body
 //                 taken by any lane costs the whole wave, so the branch-free
 //                 loop pays one bin body per bin instead of the union of the
-//                 lanes' ladders.
-constexpr int kDecPosition = 0, kDecLane = 2;
+//                 lanes' ladders;
+//   kDecAsm       the lane decoder with its bin loop in assembly and event
+//                 lanes paused and served together (mchip_lane_asm.hpp); the
+//                 default.
+constexpr int kDecPosition = 0, kDecLane = 2, kDecAsm = 3;
 
 template <int kMode>
 __device__ __forceinline__ uint2 entropy_lane(const uint8_t *pay, uint32_t len, const QTab *qt, const uint16_t *band,
                                               uint32_t *w, uint32_t cap, uint32_t *stage, uint32_t *ctx,
                                               uint32_t *win) {
-    if constexpr (kMode == kDecLane)
+    if constexpr (kMode == kDecAsm)
+        return entropy_to_list_asm(pay, len, qt, band, w, cap, ctx, win);
+    else if constexpr (kMode == kDecLane)
         return entropy_to_list_lane(pay, len, qt, band, w, cap, ctx, win);
     else
         return entropy_to_list(pay, len, qt, band, w, cap, stage, 64);
@@ -163,9 +169,10 @@ __global__ __launch_bounds__(64) void k_entropy(const uint8_t *__restrict__ blob
                                                 const ChunkInfo *__restrict__ cinfo,
                                                 uint32_t *__restrict__ scratch,
                                                 uint2 *__restrict__ binfo) {
-    __shared__ uint32_t stage[kMode == kDecLane ? 1 : kStage * 64];
-    __shared__ uint32_t sctx[kMode == kDecLane ? 17 * 64 : 1];
-    __shared__ uint32_t pwin[kMode == kDecLane ? kWinAlloc * 64 : 1];
+    constexpr bool kLaneMode = kMode == kDecLane || kMode == kDecAsm;
+    __shared__ uint32_t stage[kLaneMode ? 1 : kStage * 64];
+    __shared__ uint32_t sctx[kLaneMode ? kCtxSlots * 64 : 1];
+    __shared__ uint32_t pwin[kLaneMode ? kWinAlloc * 64 : 1];
     __shared__ QTab sq;
     __shared__ uint16_t sband[9];
     const int c = blockIdx.y;
@@ -241,6 +248,7 @@ __global__ __launch_bounds__(1024) void k_group_order(const BlkRec *__restrict__
     }
 }
 
+template <int kMode>
 __global__ __launch_bounds__(64) void k_entropy_persist(const uint8_t *__restrict__ blobs, const DescRef cd,
                                                         const QTab *__restrict__ qtabs,
                                                         const BlkRec *__restrict__ rec,
@@ -249,7 +257,8 @@ __global__ __launch_bounds__(64) void k_entropy_persist(const uint8_t *__restric
                                                         uint32_t *__restrict__ scratch, uint2 *__restrict__ binfo,
                                                         const uint32_t *__restrict__ order,
                                                         uint32_t *__restrict__ pctl) {
-    __shared__ uint32_t sctx[17 * 64];
+    static_assert(kMode == kDecLane || kMode == kDecAsm, "persistent launch takes a lane decoder");
+    __shared__ uint32_t sctx[kCtxSlots * 64];
     __shared__ uint32_t pwin[kWinAlloc * 64];
     __shared__ QTab sq;
     __shared__ uint16_t sband[9];
@@ -281,8 +290,8 @@ __global__ __launch_bounds__(64) void k_entropy_persist(const uint8_t *__restric
             } else {
                 const uint8_t *pay = blobs + cd[c].blob_off + r.off;
                 const uint64_t wofs = cd[c].scratch_off + 8ull * (r.off - cinfo[c].pay0) + 32ull * r.slot;
-                binfo[cb + bi] = entropy_to_list_lane(pay, r.len, &sq, sband, scratch + wofs, 8u * r.len + 32u,
-                                                      sctx + threadIdx.x, pwin + threadIdx.x);
+                binfo[cb + bi] = entropy_lane<kMode>(pay, r.len, &sq, sband, scratch + wofs, 8u * r.len + 32u,
+                                                     nullptr, sctx + threadIdx.x, pwin + threadIdx.x);
             }
         }
     }
@@ -507,19 +516,28 @@ void launch_index(const DecodeArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_index, dim3(a.n), dim3(256), 0, s, a.blobs, a.cd, a.rec, a.plist, a.cinfo,
                        (uint32_t)a.len_shift);
 }
-hipError_t occupancy_persist(int *per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_entropy_persist, 64, 0);
+hipError_t occupancy_persist(int decoder, int *per_cu) {
+    return decoder == kDecLane
+               ? hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_entropy_persist<kDecLane>, 64, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_entropy_persist<kDecAsm>, 64, 0);
 }
 
 void launch_entropy(const DecodeArgs &a, hipStream_t s) {
-    if (a.persistent && a.decoder == kDecLane) {
+    if (a.persistent && (a.decoder == kDecLane || a.decoder == kDecAsm)) {
         hipLaunchKernelGGL(k_group_order, dim3(1), dim3(1024), 0, s, a.rec, a.plist, a.cinfo, a.n, a.order, a.pctl,
                            (uint32_t)a.group_shift);
-        hipLaunchKernelGGL(k_entropy_persist, dim3(a.nwg), dim3(64), 0, s, a.blobs, a.cd, a.qtabs, a.rec, a.plist,
-                           a.cinfo, a.scratch, a.binfo, a.order, a.pctl);
+        if (a.decoder == kDecAsm)
+            hipLaunchKernelGGL(k_entropy_persist<kDecAsm>, dim3(a.nwg), dim3(64), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
+                               a.plist, a.cinfo, a.scratch, a.binfo, a.order, a.pctl);
+        else
+            hipLaunchKernelGGL(k_entropy_persist<kDecLane>, dim3(a.nwg), dim3(64), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
+                               a.plist, a.cinfo, a.scratch, a.binfo, a.order, a.pctl);
         return;
     }
-    if (a.decoder == kDecLane)
+    if (a.decoder == kDecAsm)
+        hipLaunchKernelGGL(k_entropy<kDecAsm>, dim3(kNVox / 64, a.n), dim3(64), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
+                           a.plist, a.cinfo, a.scratch, a.binfo);
+    else if (a.decoder == kDecLane)
         hipLaunchKernelGGL(k_entropy<kDecLane>, dim3(kNVox / 64, a.n), dim3(64), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
                            a.plist, a.cinfo, a.scratch, a.binfo);
     else
@@ -548,9 +566,10 @@ __global__ __launch_bounds__(64) void k_entropy_jobs(const uint8_t *__restrict__
                                                      const BlockJob *__restrict__ jobs, int n,
                                                      const QTab *__restrict__ qtabs,
                                                      uint32_t *__restrict__ scratch, uint2 *__restrict__ binfo) {
-    __shared__ uint32_t stage[kMode == kDecLane ? 1 : kStage * 64];
-    __shared__ uint32_t sctx[kMode == kDecLane ? 17 * 64 : 1];
-    __shared__ uint32_t pwin[kMode == kDecLane ? kWinAlloc * 64 : 1];
+    constexpr bool kLaneMode = kMode == kDecLane || kMode == kDecAsm;
+    __shared__ uint32_t stage[kLaneMode ? 1 : kStage * 64];
+    __shared__ uint32_t sctx[kLaneMode ? kCtxSlots * 64 : 1];
+    __shared__ uint32_t pwin[kLaneMode ? kWinAlloc * 64 : 1];
     const int i = blockIdx.x * 64 + threadIdx.x;
     if (i >= n) return;
     const BlockJob j = jobs[i];
@@ -577,7 +596,10 @@ __global__ __launch_bounds__(256) void k_recon_jobs(const uint8_t *__restrict__ 
 
 void launch_jobs(const uint8_t *base, const BlockJob *jobs, int n, const QTab *qtabs, uint32_t *scratch,
                  uint2 *binfo, uint8_t *arena, int decoder, hipStream_t s) {
-    if (decoder == kDecLane)
+    if (decoder == kDecAsm)
+        hipLaunchKernelGGL(k_entropy_jobs<kDecAsm>, dim3((n + 63) / 64), dim3(64), 0, s, base, jobs, n, qtabs,
+                           scratch, binfo);
+    else if (decoder == kDecLane)
         hipLaunchKernelGGL(k_entropy_jobs<kDecLane>, dim3((n + 63) / 64), dim3(64), 0, s, base, jobs, n, qtabs,
                            scratch, binfo);
     else
