@@ -52,7 +52,9 @@ int         mchip_set_priors(const uint16_t *plo, const uint16_t *phi);
  * the persistent entropy launch of batched decodes; "len_shift" and
  * "group_shift" the payload-length buckets of the lane order and of the
  * persistent group order; "recon_bpw" the reconstruction blocks per
- * workgroup; "tuned_clear" drops the autotuned shapes; "enc_batch" sets the
+ * workgroup; "slices" the chunk slices of a batched persistent decode, whose
+ * reconstruction overlaps the next slice's entropy pass on a second stream
+ * (0 = two from 16 chunks up, else one); "tuned_clear" drops the autotuned shapes; "enc_batch" sets the
  * chunks per encode launch sequence (0 = sized from free device memory) and
  * "enc_keep_mib" the encode workspace a context keeps after a call;
  * "merge" (default 1), "merge_inflight" and "merge_min" shape the merging of

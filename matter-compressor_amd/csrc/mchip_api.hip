@@ -92,6 +92,8 @@ struct Tuning {
     int len_shift = 3;        // k_index lane order: length buckets of 1 << len_shift bytes (2..5)
     int group_shift = 6;      // k_group_order: first-block length buckets of 1 << group_shift bytes (4..8)
     int recon_bpw = 1;        // k_recon blocks per workgroup (1, 2, 4)
+    int slices = 0;           // batched persistent decode: chunk slices whose reconstruction overlaps the
+                              // next slice's entropy pass on a second stream (0: 2 from 16 chunks, else 1)
     int enc_keep_mib = 2048;  // encode workspace a context keeps after a call (MiB); more is freed
     int merge = 1;            // concurrent one-chunk host calls join one batched decode
     int merge_inflight = 2;   // merged batches in flight at most (1..4)
@@ -337,6 +339,9 @@ struct mchip_ctx {
     void *d_stage_out = nullptr; size_t cap_stage_out = 0;
     void *d_aux = nullptr; size_t cap_aux = 0;
     void *d_order = nullptr; size_t cap_order = 0;      // persistent decode: group order + counters
+    // second stream of sliced batched decodes, created on first use
+    hipStream_t stream_b = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // encode workspace
     void *d_ecd = nullptr; size_t cap_ecd = 0;
     void *d_einfo = nullptr; size_t cap_einfo = 0;
@@ -524,20 +529,73 @@ static int prepare_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
     a.nwg = 0;
     a.order = nullptr;
     a.pctl = nullptr;
+    a.slices = 1;
     if (a.persistent) {
-        if (ensure(c, &c->d_order, &c->cap_order, sizeof(uint32_t) * ((size_t)n * 64 + 64))) return -1;
+        if (ensure(c, &c->d_order, &c->cap_order, sizeof(uint32_t) * ((size_t)n * 64 + 2 * kMaxSlices))) return -1;
         a.order = (uint32_t *)c->d_order;
         a.pctl = a.order + (size_t)n * 64;
         a.nwg = tn.persist_wg > 0 ? tn.persist_wg : persistent_grid(tn.decoder);
         if (a.nwg > n * 64) a.nwg = n * 64;
+        // slices need descriptors in device memory (a slice addresses its own)
+        // (automatic: two slices from 16 chunks; more slices lengthen the
+        // persistent launches' tails more than the overlap gains)
+        const int want = tn.slices > 0 ? tn.slices : (n >= 16 ? 2 : 1);
+        const int ns = std::max(1, std::min(want, std::min(kMaxSlices, n)));
+        if (!inl && ns > 1) {
+            if (!c->stream_b && (hipStreamCreateWithFlags(&c->stream_b, hipStreamNonBlocking) != hipSuccess ||
+                                 hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+                                 hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess))
+                return fail("second stream creation failed");
+            a.slices = ns;
+        }
     }
     return 0;
 }
 
+// Chunks [n*i/ns, n*(i+1)/ns) of a as a decode of their own: per-chunk
+// records, descriptors and the group order are addressed from the slice's
+// first chunk; workspace and output offsets in the descriptors are absolute.
+static DecodeArgs slice_of(const DecodeArgs &a, int i, int ns) {
+    const int c0 = (int)((int64_t)a.n * i / ns), c1 = (int)((int64_t)a.n * (i + 1) / ns);
+    DecodeArgs s = a;
+    s.n = c1 - c0;
+    s.cd.ptr = a.cd.ptr + c0;
+    s.rec = a.rec + (size_t)c0 * kNVox;
+    s.plist = a.plist + (size_t)c0 * kNVox;
+    s.cinfo = a.cinfo + c0;
+    s.binfo = a.binfo + (size_t)c0 * kNVox;
+    s.order = a.order + (size_t)c0 * 64;
+    s.pctl = a.order + (size_t)a.n * 64 + 2 * i;
+    if (s.nwg > s.n * 64) s.nwg = s.n * 64;
+    s.slices = 1;
+    return s;
+}
+
+// Index all chunks, then per chunk slice the entropy pass and the
+// reconstruction.  Slices alternate between the context's stream and its
+// second stream, so a slice's reconstruction (and the tail of its persistent
+// entropy launch) overlaps the next slice's entropy pass; the context's
+// stream waits for the second at the end, so completion on it means the
+// whole decode is done (also inside a captured graph: fork and join events).
 static void launch_decode(mchip_ctx *c, const DecodeArgs &a) {
     timed(c, "index", [&] { launch_index(a, c->stream); });
-    timed(c, "entropy", [&] { launch_entropy(a, c->stream); });
-    timed(c, "recon", [&] { launch_recon(a, c->stream); });
+    if (a.slices <= 1) {
+        timed(c, "entropy", [&] { launch_entropy(a, c->stream); });
+        timed(c, "recon", [&] { launch_recon(a, c->stream); });
+        return;
+    }
+    timed(c, "entropy_recon", [&] {
+        (void)hipEventRecord(c->ev_fork, c->stream);
+        (void)hipStreamWaitEvent(c->stream_b, c->ev_fork, 0);
+        for (int i = 0; i < a.slices; ++i) {
+            const DecodeArgs s = slice_of(a, i, a.slices);
+            hipStream_t st = (i & 1) ? c->stream_b : c->stream;
+            launch_entropy(s, st);
+            launch_recon(s, st);
+        }
+        (void)hipEventRecord(c->ev_join, c->stream_b);
+        (void)hipStreamWaitEvent(c->stream, c->ev_join, 0);
+    });
 }
 
 
@@ -608,6 +666,11 @@ int mchip_set_tuning(const char *key, int value) {
         g_tune.recon_bpw = value;
         return 0;
     }
+    if (!strcmp(key, "slices")) {
+        if (value < 0 || value > kMaxSlices) return fail("slices must be 0..%d", kMaxSlices);
+        g_tune.slices = value;
+        return 0;
+    }
     if (!strcmp(key, "tuned_clear")) { g_tuned.clear(); return 0; }
     if (!strcmp(key, "enc_keep_mib")) {
         if (value < 0) return fail("enc_keep_mib must be >= 0");
@@ -663,6 +726,12 @@ void mchip_ctx_destroy(mchip_ctx *c) {
     }
     for (hipEvent_t e : c->free_events) (void)hipEventDestroy(e);
     (void)hipEventDestroy(c->ev_desc);
+    if (c->stream_b) {
+        (void)hipStreamSynchronize(c->stream_b);
+        (void)hipStreamDestroy(c->stream_b);
+        (void)hipEventDestroy(c->ev_fork);
+        (void)hipEventDestroy(c->ev_join);
+    }
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -883,16 +952,16 @@ int mchip_autotune_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
         char buf[256];
         snprintf(buf, sizeof buf,
                  "%s{\"decoder\": %d, \"persistent\": %d, \"persist_wg\": %d, \"recon_bpw\": %d, "
-                 "\"len_shift\": %d, \"group_shift\": %d, \"ms\": %.3f}",
+                 "\"len_shift\": %d, \"group_shift\": %d, \"slices\": %d, \"ms\": %.3f}",
                  nrep++ ? ", " : "", tn.decoder, tn.persistent, tn.persist_wg, tn.recon_bpw, tn.len_shift,
-                 tn.group_shift, t);
+                 tn.group_shift, tn.slices, t);
         rep += buf;
         *out_ms = t;
         return 0;
     };
     float best_ms;
     if (time_of(best, &best_ms)) return -1;
-    enum Knob { kDecoder, kPersistent, kPersistWg, kReconBpw, kLenShift, kGroupShift };
+    enum Knob { kDecoder, kPersistent, kPersistWg, kReconBpw, kLenShift, kGroupShift, kSlices };
     struct Step { Knob k; std::vector<int> vals; };
     std::vector<Step> steps = {
         {kDecoder, {3, 2, 0}},
@@ -901,6 +970,7 @@ int mchip_autotune_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
         {kReconBpw, {1, 2, 4}},
         {kLenShift, {2, 3, 4}},
         {kGroupShift, {5, 6, 7}},
+        {kSlices, {1, 2, 4, 8}},
     };
     for (const Step &st : steps) {
         for (int v : st.vals) {
@@ -918,6 +988,10 @@ int mchip_autotune_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
             case kGroupShift:
                 if (!tn.persistent || tn.decoder < 2 || n < 2 || tn.group_shift == v) continue;
                 tn.group_shift = v;
+                break;
+            case kSlices:
+                if (!tn.persistent || tn.decoder < 2 || n <= kInlineDesc || v > n || tn.slices == v) continue;
+                tn.slices = v;
                 break;
             }
             float ms;

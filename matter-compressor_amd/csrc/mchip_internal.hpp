@@ -63,7 +63,9 @@ struct DecodeArgs {
     int recon_bpw;          // k_recon blocks per workgroup: 1, 2 or 4
     uint32_t *order;        // [n*64]
     uint32_t *pctl;         // [2]
+    int slices;             // chunk slices of a batched persistent decode (1: none)
 };
+constexpr int kMaxSlices = 32;
 
 // ---- encode ----------------------------------------------------------------
 constexpr int kSlotBytes = 17408;          // per-block payload cap (src/mc_codec.c:442, :732)

@@ -178,6 +178,54 @@ def test_persistent_batch_and_graph_replays_equal_eager(gpu_ctx):
     d_tmp.free()
 
 
+def test_sliced_batch_on_two_streams_equals_oracle(gpu_ctx):
+    """A batched decode split into chunk slices (each slice's reconstruction
+    on one of two streams, overlapping the next slice's entropy pass) equals
+    the oracle for every slice count, eagerly and as HIP graph replays with
+    eager work and a scrambled output between the replays."""
+    import mc_amd
+    specs = [dict(seed=40 + k, air=a) for k, a in enumerate([0.3, 0.5, 0.7, 0.9, 0.4, 0.6, 0.8])]
+    chunks = _chunks(specs)
+    qt = [(6.0, 0), (0.5, 1), (16.0, 32), (2.5, 8), (6.0, 16), (64.0, 128), (1.0, 4)]
+    blobs = [ob.encode_chunk(c, q, t) for c, (q, t) in zip(chunks, qt)]
+    want = [ob.decode_chunk(b) for b in blobs]
+    d_in, offs, lens, qs = _resident(gpu_ctx, blobs)
+    d_out = mc_amd.DeviceBuffer(len(blobs) * mc_amd.CHUNK_BYTES)
+    d_tmp = mc_amd.DeviceBuffer(mc_amd.CHUNK_BYTES)
+    got = np.empty((len(blobs), 256, 256, 256), np.uint8)
+
+    def check(what):
+        gpu_ctx.d2h(got, d_out)
+        for k in range(len(blobs)):
+            assert np.array_equal(got[k], want[k]), f"{what}: chunk {k}"
+
+    try:
+        for ns in (0, 1, 2, 3, 7):
+            mc_amd.set_tuning("slices", ns)
+            print(f"slices={ns}", flush=True)
+            gpu_ctx.h2d(d_out, np.full(len(blobs) * mc_amd.CHUNK_BYTES, 3, np.uint8))
+            gpu_ctx.decode_chunks(d_in, offs, lens, qs, d_out)
+            gpu_ctx.sync()
+            check(f"slices={ns}")
+        mc_amd.set_tuning("slices", 3)
+        print("graph, 3 slices", flush=True)
+        g = gpu_ctx.decode_graph(d_in, offs, lens, qs, d_out)
+        try:
+            for rep in range(3):
+                gpu_ctx.decode_chunks(d_in, offs[2:3], lens[2:3], qs[2:3], d_tmp)
+                gpu_ctx.h2d(d_out, np.full(len(blobs) * mc_amd.CHUNK_BYTES, 7, np.uint8))
+                gpu_ctx.graph_launch(g)
+                gpu_ctx.sync()
+                check(f"replay {rep}")
+        finally:
+            gpu_ctx.graph_destroy(g)
+    finally:
+        mc_amd.set_tuning("slices", 0)
+        d_in.free()
+        d_out.free()
+        d_tmp.free()
+
+
 def test_every_launch_shape_and_the_autotuner_keep_results_exact(gpu_ctx):
     """Every value of every decode launch-shape knob the autotuner searches
     decodes a mixed-quality batch exactly as the oracle; so do the autotuned
