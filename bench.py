@@ -56,17 +56,24 @@ def xxh64_hex(a):
     return f"{xxhash.xxh64(memoryview(np.ascontiguousarray(a)).cast('B'), seed=0).intdigest():016x}"
 
 
-def measured_traffic(workload, kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), or None."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.txt")
+def profile_value(workload, kernel, kind, key):
+    """A figure from a committed rocprofv3 summary
+    profiles/{kernel}_{workload}_{kind}.txt (a line `key: value`), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_{kind}.txt")
     try:
         for line in open(path):
-            if line.startswith("hbm_bytes_per_launch:"):
-                return int(float(line.split(":")[1]))
+            if line.startswith(key + ":"):
+                return float(line.split(":")[1])
     except OSError:
         pass
     return None
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), or None."""
+    v = profile_value(workload, kernel, "traffic", "hbm_bytes_per_launch")
+    return None if v is None else int(v)
 
 
 def cpu_threads():
@@ -1149,14 +1156,18 @@ def main():
     extra = {}
     if not args.no_batched:
         # all chunks in one launch sequence, measured before the per-chunk
-        # leg creates its streams (idle hardware queues slow single-stream work)
-        ctx.decode_chunks(d_blobs, offs, lens, qs, d_out)
-        ctx.sync()
+        # leg creates its streams (idle hardware queues slow single-stream
+        # work); on a context of its own, whose second stream (a sliced
+        # batch's) is gone again before the per-chunk streams are created
+        bctx = mc_amd.Ctx(0)
+        bctx.decode_chunks(d_blobs, offs, lens, qs, d_out)
+        bctx.sync()
         t0 = time.perf_counter()
         for _ in range(max(2, args.steps // 2)):
-            ctx.decode_chunks(d_blobs, offs, lens, qs, d_out)
-        ctx.sync()
+            bctx.decode_chunks(d_blobs, offs, lens, qs, d_out)
+        bctx.sync()
         dtb = (time.perf_counter() - t0) / max(2, args.steps // 2)
+        bctx.close()
         extra["batched_gbs"] = round(n * mc_amd.CHUNK_BYTES / dtb / 1e9, 3)
     # ---- verify: blob hashes recomputed and checked against the stored ones,
     # for the resident blobs (one wave per blob) and for host blobs (gathered
@@ -1290,6 +1301,21 @@ def main():
     achieved = alg_bytes / (kt[dom] / 1e3) / 1e9
     path_gbs = alg_bytes / (sum(kt.values()) / 1e3) / 1e9
     traffic = measured_traffic(args.workload, dom)
+    # the decode path's HBM traffic per chunk (every decode kernel's
+    # calibrated FETCH_SIZE + WRITE_SIZE) against its algorithmic bytes
+    path_traffic = profile_value(args.workload, dom, "traffic", "decode_path_hbm_bytes_per_chunk")
+    # issue roofline of the dominant kernel: VALU wave-instructions per launch
+    # (SQ_INSTS_VALU summed over its waves) over its mean duration, against
+    # one wave64 VALU instruction per SIMD per 4 cycles on 256 CUs x 4 SIMDs
+    valu_per_launch = profile_value(args.workload, dom, "valu", "valu_insts_per_launch")
+    valu_peak = 256 * 4 * 2.4e9 / 4.0               # 2.4 GHz: MI355X peak engine clock
+    valu_roof = None
+    if valu_per_launch:
+        ach = valu_per_launch / (kt[dom] / 1e3)
+        valu_roof = {"bound": "valu_issue", "achieved": round(ach / 1e9, 2), "peak": round(valu_peak / 1e9, 1),
+                     "unit": "G wave64 VALU instructions/s", "frac": round(ach / valu_peak, 4),
+                     "valu_insts_per_launch": int(valu_per_launch),
+                     "note": "per launch alone; up to 21 chunks decode at once in the timed steps"}
     rec = {
         "metric": "decode GB/s of decoded voxels on 1xMI355X, bit-exact (64 x 256^3 u8 chunks, q=6, one chunk per launch)",
         "value": round(value, 3),
@@ -1311,7 +1337,9 @@ def main():
         "kernels_ms_avg": {k: round(v, 4) for k, v in kt.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "alg_bytes_per_launch": round(alg_bytes), "decode_path_gbs": round(path_gbs, 2)},
+                     "alg_bytes_per_launch": round(alg_bytes), "decode_path_gbs": round(path_gbs, 2),
+                     "decode_path_traffic_ratio": (round(path_traffic / alg_bytes, 3) if path_traffic else None),
+                     "valu": valu_roof},
         "cpu_baseline": cpu_record(ref_cpu, cpu_dec, cpu_enc, ns, n, nthreads),
         "encode": {"value": round(enc_gbs, 3), "unit": "GB/s", "byte_identical": bad_enc == 0,
                    "launch": (f"{min(n, enc_batch)} chunks per launch" if enc_batch
