@@ -12,7 +12,7 @@
 // count and form: VOP2/VOPC forms that use VCC issue in about 3 SIMD cycles,
 // VOP3 forms with an SGPR mask or three operands in about 4.5, and a lone
 // wave waits about 9 cycles on a dependent result.  The compiled loop spent
-// about 95 instructions per bin; this one spends 65, with the context read
+// about 95 instructions per bin; this one spends 67, with the context read
 // of the next bin issued right after the decision so its LDS latency is
 // covered by the renormalisation, the list store and the state update.
 //
@@ -41,7 +41,7 @@ namespace mchip {
 
 constexpr uint32_t kHoldBins = MCHIP_HOLD_BINS;        // bins a paused lane waits at most
 constexpr uint32_t kCtxSlots = 18;                      // context slots per lane
-static_assert(kWin == 64 && kLaneStride == 64, "the loop's window threshold and slot stride assume 64/64");
+static_assert(kLaneStride == 64 && kWin % 16 == 0 && kWin >= 16, "slot stride of 256 bytes; whole window fills");
 static_assert(kHoldBins >= 1 && kHoldBins <= 64, "hold window");
 
 // LDS byte offset of a pointer into a __shared__ array.
@@ -84,7 +84,7 @@ __device__ __forceinline__ uint32_t decode_coefs_asm(LaneCoder &d, const QTab *_
     uint32_t done = 0;
     do {
         uint32_t t0, t1, t2, t3, t4, t5, t6, t7, t8, r0, bi, sh, age;
-        uint64_t sg, ss, si, sj, sr, la, held;
+        uint64_t sg, ss, si, sj, sr, la, held, sx;
         // Registers: v101 holds code during the loop, v100 the next 32
         // payload bits beside it (a 64-bit shift renormalises code);
         // v[102:103] the window pair shifted to the read position; v104 the
@@ -152,7 +152,12 @@ __device__ __forceinline__ uint32_t decode_coefs_asm(LaneCoder &d, const QTab *_
             "v_lshl_or_b32 %[t6], %[t6], 16, %[p]\n"
             "v_min_u32 v104, %[capm1], %[n]\n"
             "v_lshl_add_u64 %[la], v[104:105], 2, %[lst]\n"
+            // only the sign bins' lanes store: a 64-lane store of scattered
+            // words on every bin occupied the memory pipeline once several
+            // waves share a CU (batched decode 140 -> 206 GB/s)
+            "s_and_saveexec_b64 %[sx], %[sg]\n"
             "global_store_dword %[la], %[t6], off\n"
+            "s_mov_b64 exec, %[sx]\n"
             "v_cndmask_b32_e64 %[mag], %[st], %[mag], %[sg]\n"
             "v_addc_co_u32_e64 %[n], %[sj], 0, %[n], %[sg]\n"
             "v_addc_co_u32_e64 %[p], %[sj], 0, %[p], %[si]\n"
@@ -179,7 +184,7 @@ __device__ __forceinline__ uint32_t decode_coefs_asm(LaneCoder &d, const QTab *_
               [t0] "=&v"(t0), [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3), [t4] "=&v"(t4), [t5] "=&v"(t5),
               [t6] "=&v"(t6), [t7] "=&v"(t7), [t8] "=&v"(t8), [r0] "=&v"(r0), [bi] "=&v"(bi), [sh] "=&v"(sh),
               [la] "=&v"(la), [sg] "=&s"(sg), [ss] "=&s"(ss), [si] "=&s"(si), [sj] "=&s"(sj), [sr] "=&s"(sr),
-              [held] "=&s"(held), [age] "=&s"(age)
+              [held] "=&s"(held), [age] "=&s"(age), [sx] "=&s"(sx)
             : [ctxb] "v"(ctxb), [winb] "v"(winb), [evp] "v"(evp), [capm1] "v"(capm1), [lst] "v"(lst),
               [hold] "n"(kHoldBins - 1), [thr] "n"(32 * (kWin - 1) - 1)
             : "vcc", "v100", "v101", "v102", "v103", "v104", "v105", "memory");
