@@ -55,7 +55,10 @@ int         mchip_set_priors(const uint16_t *plo, const uint16_t *phi);
  * workgroup; "slices" the chunk slices of a batched persistent decode, whose
  * reconstruction overlaps the next slice's entropy pass on a second stream
  * (0 = two from 16 chunks up, else one); "tuned_clear" drops the autotuned shapes; "enc_batch" sets the
- * chunks per encode launch sequence (0 = sized from free device memory) and
+ * chunks per encode launch sequence (0 = sized from free device memory),
+ * "enc_slices" the chunk slices of one encode sequence, run alternately on
+ * two streams (0 = one slice; more slices measured slower on the mixed
+ * workload, as each slice's entropy pass then ends in its own tail), and
  * "enc_keep_mib" the encode workspace a context keeps after a call;
  * "merge" (default 1), "merge_inflight" and "merge_min" shape the merging of
  * concurrent one-chunk host decodes.  Results

@@ -92,6 +92,7 @@ struct Tuning {
     int len_shift = 3;        // k_index lane order: length buckets of 1 << len_shift bytes (2..5)
     int group_shift = 6;      // k_group_order: first-block length buckets of 1 << group_shift bytes (4..8)
     int recon_bpw = 1;        // k_recon blocks per workgroup (1, 2, 4)
+    int enc_slices = 0;       // encode sequence: chunk slices on two streams (0: one slice)
     int slices = 0;           // batched persistent decode: chunk slices whose reconstruction overlaps the
                               // next slice's entropy pass on a second stream (0: 2 from 16 chunks, else 1)
     int enc_keep_mib = 2048;  // encode workspace a context keeps after a call (MiB); more is freed
@@ -664,6 +665,11 @@ int mchip_set_tuning(const char *key, int value) {
     if (!strcmp(key, "recon_bpw")) {
         if (value != 1 && value != 2 && value != 4) return fail("recon_bpw must be 1, 2 or 4");
         g_tune.recon_bpw = value;
+        return 0;
+    }
+    if (!strcmp(key, "enc_slices")) {
+        if (value < 0 || value > kMaxSlices) return fail("enc_slices must be 0..%d", kMaxSlices);
+        g_tune.enc_slices = value;
         return 0;
     }
     if (!strcmp(key, "slices")) {
@@ -1323,8 +1329,34 @@ static int prepare_encode(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride,
                           const float *q_store, int m, uint8_t *d_out, const uint64_t *out_off,
                           const uint64_t *out_cap, EncodeArgs &a) {
     void *h = nullptr;
-    if (desc_buffer(c, sizeof(EncChunkDesc) * (size_t)m, &h)) return -1;
+    const size_t order_at = sizeof(EncChunkDesc) * (size_t)m;
+    if (desc_buffer(c, order_at + sizeof(uint32_t) * (size_t)m, &h)) return -1;
     EncChunkDesc *cd = (EncChunkDesc *)h;
+    // Chunk slices of the sequence (run on two streams, see launch_encode)
+    const Tuning tn = tuning();
+    const int want = tn.enc_slices > 0 ? tn.enc_slices : 1;
+    const int ns = std::max(1, std::min(want, std::min(kMaxSlices, m)));
+    if (ns > 1 && !c->stream_b &&
+        (hipStreamCreateWithFlags(&c->stream_b, hipStreamNonBlocking) != hipSuccess ||
+         hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+         hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess))
+        return fail("second stream creation failed");
+    // Within a slice the entropy pass takes the chunks heaviest first (lowest
+    // q, then the larger max-error bound with its corrections), so that the
+    // longest waves of a mixed-quality sequence start in the first dispatch
+    // round; ord[] holds each slice's order relative to its first chunk.
+    uint32_t *ord = (uint32_t *)((uint8_t *)h + order_at);
+    auto qkey = [&](uint32_t i) { return q[i] > 0.0f && q[i] <= 1024.0f ? q[i] : 1024.0f; };   // NaN-free
+    for (int si = 0; si < ns; ++si) {
+        const int c0 = (int)((int64_t)m * si / ns), c1 = (int)((int64_t)m * (si + 1) / ns);
+        for (int i = c0; i < c1; ++i) ord[i] = (uint32_t)i;
+        std::stable_sort(ord + c0, ord + c1, [&](uint32_t x, uint32_t y) {
+            const float kx = qkey(x), ky = qkey(y);
+            const int tx = tau ? tau[x] : 0, ty = tau ? tau[y] : 0;
+            return kx < ky || (kx == ky && tx > ty);
+        });
+        for (int i = c0; i < c1; ++i) ord[i] -= (uint32_t)c0;
+    }
     for (int i = 0; i < m; ++i) {
         uint32_t slot;
         if (qtab_slot(q[i], &slot)) return -1;
@@ -1337,7 +1369,7 @@ static int prepare_encode(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride,
         cd[i].pad = 0;
     }
     const size_t nb = (size_t)m * kNVox;
-    if (ensure(c, &c->d_ecd, &c->cap_ecd, sizeof(EncChunkDesc) * m) ||
+    if (ensure(c, &c->d_ecd, &c->cap_ecd, order_at + sizeof(uint32_t) * (size_t)m) ||
         ensure(c, &c->d_einfo, &c->cap_einfo, sizeof(uint4) * nb) ||
         ensure(c, &c->d_levels, &c->cap_levels, sizeof(int16_t) * nb * kNVox) ||
         ensure(c, &c->d_corr, &c->cap_corr, sizeof(uint32_t) * nb * kNVox) ||
@@ -1347,11 +1379,12 @@ static int prepare_encode(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride,
         ensure(c, &c->d_res, &c->cap_res, sizeof(EncChunkResult) * m) ||
         ensure(c, &c->d_fmap, &c->cap_fmap, (size_t)kFmapStride * m))
         return -1;
-    HIPCHK(hipMemcpyAsync(c->d_ecd, cd, sizeof(EncChunkDesc) * m, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_ecd, cd, order_at + sizeof(uint32_t) * (size_t)m, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipEventRecord(c->ev_desc, c->stream));
     c->desc_pending = true;
     a.in = d_in;
     a.cd = (const EncChunkDesc *)c->d_ecd;
+    a.order = (const uint32_t *)((const uint8_t *)c->d_ecd + order_at);
     a.qtabs = qtabs_now();
     a.einfo = (uint4 *)c->d_einfo;
     a.levels = (int16_t *)c->d_levels;
@@ -1363,15 +1396,61 @@ static int prepare_encode(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride,
     a.res = (EncChunkResult *)c->d_res;
     a.fmap = (uint32_t *)c->d_fmap;
     a.n = m;
+    a.slices = ns;
     return 0;
 }
 
+// Chunks [n*i/ns, n*(i+1)/ns) of an encode sequence as a sequence of their
+// own (per-chunk and per-block arrays addressed from the slice's first chunk;
+// input and output offsets in the descriptors are absolute).
+static EncodeArgs enc_slice_of(const EncodeArgs &a, int i, int ns) {
+    const int c0 = (int)((int64_t)a.n * i / ns), c1 = (int)((int64_t)a.n * (i + 1) / ns);
+    const size_t b0 = (size_t)c0 * kNVox;
+    EncodeArgs s = a;
+    s.n = c1 - c0;
+    s.cd = a.cd + c0;
+    s.einfo = a.einfo + b0;
+    s.levels = a.levels + b0 * kNVox;
+    s.corr = a.corr + b0 * kNVox;
+    s.slots = a.slots + b0 * (kSlotBytes / 4);
+    s.plen = a.plen + b0;
+    s.dst = a.dst + b0;
+    s.fmap = a.fmap + (size_t)c0 * (kFmapStride / 4);
+    s.res = a.res + c0;
+    s.order = a.order + c0;
+    s.slices = 1;
+    return s;
+}
+
+// The five launches of a sequence; with slices, slice i runs on the
+// context's stream (even i) or its second stream (odd i), so that one slice's
+// latency-bound passes (the blob hash is one wave per chunk) run beside the
+// other slice's transform and entropy passes.  The context's stream waits for
+// the second at the end (fork and join events, also inside a captured graph).
 static void launch_encode(mchip_ctx *c, const EncodeArgs &a) {
-    timed(c, "enc_transform", [&] { launch_enc_transform(a, c->stream); });
-    timed(c, "enc_entropy", [&] { launch_enc_entropy(a, c->stream); });
-    timed(c, "enc_assemble", [&] { launch_enc_assemble(a, c->stream); });
-    timed(c, "enc_copy", [&] { launch_enc_copy(a, c->stream); });
-    timed(c, "enc_hash", [&] { launch_enc_hash(a, c->stream); });
+    if (a.slices <= 1) {
+        timed(c, "enc_transform", [&] { launch_enc_transform(a, c->stream); });
+        timed(c, "enc_entropy", [&] { launch_enc_entropy(a, c->stream); });
+        timed(c, "enc_assemble", [&] { launch_enc_assemble(a, c->stream); });
+        timed(c, "enc_copy", [&] { launch_enc_copy(a, c->stream); });
+        timed(c, "enc_hash", [&] { launch_enc_hash(a, c->stream); });
+        return;
+    }
+    timed(c, "enc_sliced", [&] {
+        (void)hipEventRecord(c->ev_fork, c->stream);
+        (void)hipStreamWaitEvent(c->stream_b, c->ev_fork, 0);
+        for (int i = 0; i < a.slices; ++i) {
+            const EncodeArgs s = enc_slice_of(a, i, a.slices);
+            hipStream_t st = (i & 1) ? c->stream_b : c->stream;
+            launch_enc_transform(s, st);
+            launch_enc_entropy(s, st);
+            launch_enc_assemble(s, st);
+            launch_enc_copy(s, st);
+            launch_enc_hash(s, st);
+        }
+        (void)hipEventRecord(c->ev_join, c->stream_b);
+        (void)hipStreamWaitEvent(c->stream, c->ev_join, 0);
+    });
 }
 
 // Frees the context's encode workspace above the retention limit (tuning

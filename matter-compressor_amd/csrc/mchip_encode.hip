@@ -240,7 +240,7 @@ __global__ __launch_bounds__(256) void k_enc_transform(const uint8_t *__restrict
 }
 
 // ---------------------------------------------------------------------------
-// k_enc_entropy: one lane per block (blockIdx.y = chunk)
+// k_enc_entropy: one lane per block (row blockIdx.y = the order[]-th chunk)
 // ---------------------------------------------------------------------------
 // Fraction map of one chunk (src/mc_codec.c:600-611): the 64 lanes stage the
 // 4096 nibbles in LDS, lane 0 range-codes them.  Output at fm: [u32 length]
@@ -284,26 +284,27 @@ __global__ __launch_bounds__(64) void k_enc_entropy(const EncChunkDesc *__restri
                                                     const uint32_t *__restrict__ corr,
                                                     uint32_t *__restrict__ slots,
                                                     uint32_t *__restrict__ plen, uint32_t *__restrict__ fmapbuf,
-                                                    int nblocks) {
+                                                    int nblocks, const uint32_t *__restrict__ order) {
+    // chunk of this row of workgroups (rows are dispatched in order)
+    const uint32_t cy = kBlocks || !order ? blockIdx.y : order[blockIdx.y];
     __shared__ uint32_t stage[kEncStage * 64];
     __shared__ QTab sq;
     __shared__ uint16_t sband[9];
     if (!kBlocks && blockIdx.x == 0) {   // one extra workgroup per chunk, dispatched first: the fraction map
-        encode_fracmap_wg(einfo + (size_t)blockIdx.y * kNVox, fmapbuf + (size_t)blockIdx.y * (kFmapStride / 4),
-                          stage);
+        encode_fracmap_wg(einfo + (size_t)cy * kNVox, fmapbuf + (size_t)cy * (kFmapStride / 4), stage);
         return;
     }
     const int bi = (kBlocks ? blockIdx.x : blockIdx.x - 1) * 64 + threadIdx.x;
     const QTab *qt;
     if (!kBlocks) {   // one chunk per workgroup: its priors and band starts into LDS
-        const uint32_t *src = (const uint32_t *)(qtabs + cd[blockIdx.y].qtab);
+        const uint32_t *src = (const uint32_t *)(qtabs + cd[cy].qtab);
         uint32_t *dst = (uint32_t *)&sq;
         for (uint32_t i = threadIdx.x; i < sizeof(QTab) / 4; i += 64) dst[i] = src[i];
         if (threadIdx.x < 9) sband[threadIdx.x] = c_band_start[threadIdx.x];
         __syncthreads();
     }
     if (kBlocks && bi >= nblocks) return;
-    const int c = kBlocks ? bi : (int)blockIdx.y;
+    const int c = kBlocks ? bi : (int)cy;
     const size_t gb = kBlocks ? (size_t)bi : (size_t)c * kNVox + bi;
     const uint4 inf = einfo[gb];
     if (!(inf.x & (1u << 12))) { plen[gb] = 0; return; }
@@ -411,14 +412,14 @@ void launch_enc_transform(const EncodeArgs &a, hipStream_t s) {
 }
 void launch_enc_entropy(const EncodeArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_enc_entropy<false>, dim3(kNVox / 64 + 1, a.n), dim3(64), 0, s, a.cd, a.qtabs, a.einfo,
-                       a.levels, a.corr, a.slots, a.plen, a.fmap, 0);
+                       a.levels, a.corr, a.slots, a.plen, a.fmap, 0, a.order);
 }
 // independent blocks: a.n blocks, descriptor per block
 void launch_enc_blocks(const EncodeArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_enc_transform<true>, dim3((unsigned)a.n), dim3(256), 0, s, a.in, a.cd, a.qtabs, a.einfo,
                        a.levels, a.corr, a.n);
     hipLaunchKernelGGL(k_enc_entropy<true>, dim3(((unsigned)a.n + 63) / 64), dim3(64), 0, s, a.cd, a.qtabs,
-                       a.einfo, a.levels, a.corr, a.slots, a.plen, nullptr, a.n);
+                       a.einfo, a.levels, a.corr, a.slots, a.plen, nullptr, a.n, nullptr);
 }
 void launch_enc_assemble(const EncodeArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_enc_assemble, dim3(a.n), dim3(256), 0, s, a.cd, a.einfo, a.plen, a.fmap, a.dst, a.out,

@@ -106,6 +106,8 @@ struct EncodeArgs {
     uint8_t *out;
     EncChunkResult *res;    // [n]
     int n;
+    const uint32_t *order;  // [n] chunk order of the entropy pass (null: 0..n-1)
+    int slices;             // chunk slices of the sequence on two streams (1: none)
 };
 
 void launch_enc_transform(const EncodeArgs &a, hipStream_t s);

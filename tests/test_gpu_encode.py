@@ -91,3 +91,59 @@ def test_encode_graph_replays_equal_reference(gpu_ctx):
         d_in.free()
         d_out.free()
         d_tmp.free()
+
+
+def test_sliced_encode_on_two_streams_equals_reference(gpu_ctx):
+    """An encode sequence split into chunk slices (alternate slices on two
+    streams, the entropy pass heaviest chunk first within each slice) writes
+    the reference's blobs (manifest) for every slice count, eagerly and as HIP
+    graph replays with eager work and a scrambled output between them."""
+    import mc_amd
+    man = json.load(open(os.path.join(ROOT, "tests", "golden", "chunk_manifest.json")))
+    specs = [man["plain64_q6"][3]] + [man["mixed256"][k] for k in (1, 4, 6, 11, 20)]
+    chunks = mc_amd.synth_chunks([s["seed"] for s in specs], [s["air"] for s in specs])
+    n = len(specs)
+    cap = 20 << 20
+    d_in = mc_amd.DeviceBuffer(n * mc_amd.CHUNK_BYTES)
+    d_out = mc_amd.DeviceBuffer(n * cap)
+    d_tmp = mc_amd.DeviceBuffer(cap)
+    gpu_ctx.h2d(d_in, chunks)
+    offs = np.arange(n, dtype=np.uint64) * cap
+    caps = np.full(n, cap, np.uint64)
+    qs = [s["q"] for s in specs]
+    taus = [s["tau"] for s in specs]
+
+    def check(what, lens):
+        for k, s in enumerate(specs):
+            assert int(lens[k]) == s["blob_len"], f"{what}: chunk {k} length"
+            b = np.empty(int(lens[k]), np.uint8)
+            gpu_ctx.d2h(b, d_out, offset=int(offs[k]))
+            assert f"{ob.xxh64(b):016x}" == s["blob_xxh64"], f"{what}: chunk {k}"
+
+    try:
+        for ns in (1, 2, 3, 6):
+            mc_amd.set_tuning("enc_slices", ns)
+            print(f"enc_slices={ns}", flush=True)
+            gpu_ctx.h2d(d_out, np.full(n * cap, 0x5A, np.uint8))
+            lens, st = gpu_ctx.encode_chunks(d_in, qs, taus, d_out, offs, caps)
+            assert (st == 0).all()
+            check(f"enc_slices={ns}", lens)
+        mc_amd.set_tuning("enc_slices", 3)
+        print("encode graph, 3 slices", flush=True)
+        g, gn = gpu_ctx.encode_graph(d_in, qs, taus, d_out, offs, caps)
+        try:
+            for rep in range(3):
+                gpu_ctx.encode_chunks(d_in, qs[rep:rep + 1], taus[rep:rep + 1], d_tmp, [0], [cap])
+                gpu_ctx.h2d(d_out, np.full(n * cap, 0x33, np.uint8))
+                gpu_ctx.graph_launch(g)
+                gpu_ctx.sync()
+                lens, st = gpu_ctx.encode_graph_results(g, gn)
+                assert (st == 0).all()
+                check(f"replay {rep}", lens)
+        finally:
+            gpu_ctx.graph_destroy(g)
+    finally:
+        mc_amd.set_tuning("enc_slices", 0)
+        d_in.free()
+        d_out.free()
+        d_tmp.free()
