@@ -134,7 +134,10 @@ __device__ __forceinline__ void decode_header(RcDec &d, const QTab *__restrict__
 #ifndef MCHIP_LANE_WIN
 #define MCHIP_LANE_WIN 64
 #endif
-constexpr int32_t kWin = MCHIP_LANE_WIN;   // window words per lane (multiple of 16)
+#ifndef MCHIP_PERSIST_WIN
+#define MCHIP_PERSIST_WIN 32
+#endif
+constexpr int32_t kWin = MCHIP_LANE_WIN;   // default window words per lane (multiple of 16)
 constexpr int32_t kWinAlloc = kWin + 1;   // words allocated per lane (see LaneCoder::norm_fast)
 
 #ifndef MCHIP_LANE_STRIDE
@@ -142,17 +145,18 @@ constexpr int32_t kWinAlloc = kWin + 1;   // words allocated per lane (see LaneC
 #endif
 constexpr uint32_t kLaneStride = MCHIP_LANE_STRIDE;   // lanes sharing one LDS array
 
-// Fill win[0, kWin) with the payload's aligned words first .. first + kWin.
-// Out of line: it runs once per kWin words.
+// Fill win[0, W) with the payload's aligned words first .. first + W.
+// Out of line: it runs once per W words.
+template <int W>
 __device__ __noinline__ void window_fill(const uint8_t *p, uint32_t len, int32_t first, uint32_t *win) {
     const uint32_t lead = (uint32_t)((uintptr_t)p & 3);
     const uint32_t *aw = (const uint32_t *)((uintptr_t)p - lead);
     const int32_t nwords = (int32_t)((lead + len + 3) >> 2);   // aligned words holding payload bytes
     #pragma unroll 1
-    for (int32_t j0 = 0; j0 < kWin; j0 += 16) {
+    for (int32_t j0 = 0; j0 < W; j0 += 16) {
         if (first + j0 >= nwords) {                            // rest is past the payload
             #pragma unroll 1
-            for (int32_t j = j0; j < kWin; ++j) win[j * kLaneStride] = 0u;
+            for (int32_t j = j0; j < W; ++j) win[j * kLaneStride] = 0u;
             break;
         }
         uint32_t a[16];
@@ -171,7 +175,10 @@ __device__ __noinline__ void window_fill(const uint8_t *p, uint32_t len, int32_t
     }
 }
 
+// A window of W words per lane (W + 1 allocated).
+template <int W = kWin>
 struct LaneCoder {
+    static_assert(W % 16 == 0 && W >= 16, "whole window fills");
     const uint8_t *pay;
     uint32_t len;
     uint32_t *win;
@@ -190,16 +197,16 @@ struct LaneCoder {
     __device__ __forceinline__ uint32_t next32() const {    // 32 bits (MSB first) at rd_bits
         return (uint32_t)(((((uint64_t)w_hi) << 32) | w_lo) << (rd_bits & 24) >> 32);
     }
-    __device__ __forceinline__ bool need_slide() const { return rd_bits >= 32u * (kWin - 1); }
+    __device__ __forceinline__ bool need_slide() const { return rd_bits >= 32u * (W - 1); }
     __device__ __forceinline__ void slide() {
         first += (int32_t)(rd_bits >> 5);
         rd_bits &= 31u;
-        window_fill(pay, len, first, win);
+        window_fill<W>(pay, len, first, win);
         fetch();
     }
     __device__ __forceinline__ void init(const uint8_t *p, uint32_t n, uint32_t *w) {
         pay = p; len = n; win = w; first = 0;
-        window_fill(p, n, 0, win);
+        window_fill<W>(p, n, 0, win);
         // code = payload bytes 1..4 (byte 0 is the encoder's leading cache byte)
         rd_bits = 8u * (uint32_t)((uintptr_t)p & 3) + 8u;
         fetch();
@@ -210,7 +217,7 @@ struct LaneCoder {
     }
     // Renormalise: shift in 0..3 bytes until range >= 2^24.  The _fast form
     // leaves a window slide to the caller: with rd_bits below the slide
-    // threshold on entry it reads at most word kWin (kWinAlloc words are
+    // threshold on entry it reads at most word W (W + 1 words are
     // allocated), and the words are valid again after slide().
     __device__ __forceinline__ void norm_fast() {
         const uint32_t sh = (uint32_t)__builtin_clz(range) & 24u;   // range >= 1
@@ -279,7 +286,8 @@ struct LaneSink {
 // Coefficients of one block; ctx[s * kLaneStride] holds slot s: 0-3 the
 // significance contexts of the current band, 4-15 the magnitude rungs, 16 a
 // dummy written by bypass bins.
-__device__ __forceinline__ uint32_t decode_coefs_lane(LaneCoder &d, const QTab *__restrict__ qt,
+template <int W>
+__device__ __forceinline__ uint32_t decode_coefs_lane(LaneCoder<W> &d, const QTab *__restrict__ qt,
                                                       const uint16_t *band_start, LaneSink &sink, uint32_t *ctx) {
     uint32_t k0 = 0;
     for (uint32_t i = 0; i < 13; ++i) {
@@ -369,12 +377,14 @@ __device__ __forceinline__ uint32_t decode_coefs_lane(LaneCoder &d, const QTab *
 // w (cap words).  Returns the block record like entropy_to_list.  Coefs is the
 // coefficient decoder (decode_coefs_lane, or the assembly loop of
 // mchip_lane_asm.hpp on the GPU); both read the same bins in the same order.
-using LaneCoefFn = uint32_t (*)(LaneCoder &, const QTab *__restrict__, const uint16_t *, LaneSink &, uint32_t *);
-template <LaneCoefFn Coefs>
+// W is the window size in words (win holds W + 1 words per lane).
+template <int W>
+using LaneCoefFn = uint32_t (*)(LaneCoder<W> &, const QTab *__restrict__, const uint16_t *, LaneSink &, uint32_t *);
+template <int W, LaneCoefFn<W> Coefs>
 __device__ __forceinline__ uint2 entropy_to_list_lane_with(const uint8_t *payload, uint32_t len,
                                                            const QTab *__restrict__ qt, const uint16_t *band_start,
                                                            uint32_t *w, uint32_t cap, uint32_t *ctx, uint32_t *win) {
-    LaneCoder d;
+    LaneCoder<W> d;
     d.init(payload, len, win);
     uint32_t f = 0, dcv = 0;
     f |= d.bit(qt->flag[0]) ? 1u : 0u;
@@ -402,10 +412,11 @@ __device__ __forceinline__ uint2 entropy_to_list_lane_with(const uint8_t *payloa
     return make_uint2(dcv | (f << 8) | (status << 16), nsig | (nc << 16));
 }
 
+template <int W = kWin>
 __device__ __forceinline__ uint2 entropy_to_list_lane(const uint8_t *payload, uint32_t len, const QTab *__restrict__ qt,
                                                       const uint16_t *band_start, uint32_t *w, uint32_t cap,
                                                       uint32_t *ctx, uint32_t *win) {
-    return entropy_to_list_lane_with<decode_coefs_lane>(payload, len, qt, band_start, w, cap, ctx, win);
+    return entropy_to_list_lane_with<W, decode_coefs_lane<W>>(payload, len, qt, band_start, w, cap, ctx, win);
 }
 
 // Entropy-decode one payload into a (raster index, level) list at w (cap

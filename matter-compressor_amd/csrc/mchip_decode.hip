@@ -148,14 +148,21 @@ __global__ __launch_bounds__(256) void k_index(const uint8_t *__restrict__ blobs
 //                 default.
 constexpr int kDecPosition = 0, kDecLane = 2, kDecAsm = 3;
 
-template <int kMode>
+// Payload window words per lane of the lane decoders: 64 for the launches of
+// one chunk (fewer window refills on the longest waves, which set a chunk's
+// time), 32 for the persistent batched launch, where the smaller LDS
+// footprint lets more waves share each CU (batched plain64 207 -> 216 GB/s,
+// mixed-quality 97 -> 115 GB/s; one chunk alone is 2 % slower at 32).
+constexpr int kWinGrid = kWin, kWinPersist = MCHIP_PERSIST_WIN;
+
+template <int kMode, int W = kWinGrid>
 __device__ __forceinline__ uint2 entropy_lane(const uint8_t *pay, uint32_t len, const QTab *qt, const uint16_t *band,
                                               uint32_t *w, uint32_t cap, uint32_t *stage, uint32_t *ctx,
                                               uint32_t *win) {
     if constexpr (kMode == kDecAsm)
-        return entropy_to_list_asm(pay, len, qt, band, w, cap, ctx, win);
+        return entropy_to_list_asm<W>(pay, len, qt, band, w, cap, ctx, win);
     else if constexpr (kMode == kDecLane)
-        return entropy_to_list_lane(pay, len, qt, band, w, cap, ctx, win);
+        return entropy_to_list_lane<W>(pay, len, qt, band, w, cap, ctx, win);
     else
         return entropy_to_list(pay, len, qt, band, w, cap, stage, 64);
 }
@@ -259,7 +266,7 @@ __global__ __launch_bounds__(64) void k_entropy_persist(const uint8_t *__restric
                                                         uint32_t *__restrict__ pctl) {
     static_assert(kMode == kDecLane || kMode == kDecAsm, "persistent launch takes a lane decoder");
     __shared__ uint32_t sctx[kCtxSlots * 64];
-    __shared__ uint32_t pwin[kWinAlloc * 64];
+    __shared__ uint32_t pwin[(kWinPersist + 1) * 64];
     __shared__ QTab sq;
     __shared__ uint16_t sband[9];
     if (threadIdx.x < 9) sband[threadIdx.x] = c_band_start[threadIdx.x];
@@ -290,8 +297,9 @@ __global__ __launch_bounds__(64) void k_entropy_persist(const uint8_t *__restric
             } else {
                 const uint8_t *pay = blobs + cd[c].blob_off + r.off;
                 const uint64_t wofs = cd[c].scratch_off + 8ull * (r.off - cinfo[c].pay0) + 32ull * r.slot;
-                binfo[cb + bi] = entropy_lane<kMode>(pay, r.len, &sq, sband, scratch + wofs, 8u * r.len + 32u,
-                                                     nullptr, sctx + threadIdx.x, pwin + threadIdx.x);
+                binfo[cb + bi] = entropy_lane<kMode, kWinPersist>(pay, r.len, &sq, sband, scratch + wofs,
+                                                                  8u * r.len + 32u, nullptr, sctx + threadIdx.x,
+                                                                  pwin + threadIdx.x);
             }
         }
     }

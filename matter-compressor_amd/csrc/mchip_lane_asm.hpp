@@ -41,7 +41,7 @@ namespace mchip {
 
 constexpr uint32_t kHoldBins = MCHIP_HOLD_BINS;        // bins a paused lane waits at most
 constexpr uint32_t kCtxSlots = 18;                      // context slots per lane
-static_assert(kLaneStride == 64 && kWin % 16 == 0 && kWin >= 16, "slot stride of 256 bytes; whole window fills");
+static_assert(kLaneStride == 64, "slot stride of 256 bytes");
 static_assert(kHoldBins >= 1 && kHoldBins <= 64, "hold window");
 
 // LDS byte offset of a pointer into a __shared__ array.
@@ -54,8 +54,9 @@ __device__ __forceinline__ uint32_t lds_read_volatile(uint32_t off) {
 }
 
 // Coefficients of one block into the list (as decode_coefs_lane); ctx holds
-// kCtxSlots slots of this lane, d.win its payload window.
-__device__ __forceinline__ uint32_t decode_coefs_asm(LaneCoder &d, const QTab *__restrict__ qt,
+// kCtxSlots slots of this lane, d.win its payload window of W words.
+template <int W>
+__device__ __forceinline__ uint32_t decode_coefs_asm(LaneCoder<W> &d, const QTab *__restrict__ qt,
                                                      const uint16_t *band_start, LaneSink &sink, uint32_t *ctx) {
     uint32_t k0 = 0;
     for (uint32_t i = 0; i < 13; ++i) {
@@ -186,7 +187,7 @@ __device__ __forceinline__ uint32_t decode_coefs_asm(LaneCoder &d, const QTab *_
               [la] "=&v"(la), [sg] "=&s"(sg), [ss] "=&s"(ss), [si] "=&s"(si), [sj] "=&s"(sj), [sr] "=&s"(sr),
               [held] "=&s"(held), [age] "=&s"(age), [sx] "=&s"(sx)
             : [ctxb] "v"(ctxb), [winb] "v"(winb), [evp] "v"(evp), [capm1] "v"(capm1), [lst] "v"(lst),
-              [hold] "n"(kHoldBins - 1), [thr] "n"(32 * (kWin - 1) - 1)
+              [hold] "n"(kHoldBins - 1), [thr] "n"(32 * (W - 1) - 1)
             : "vcc", "v100", "v101", "v102", "v103", "v104", "v105", "memory");
         // the paused lanes' events, in the order the bins need them
         d.code = code; d.range = range; d.rd_bits = rd;
@@ -220,11 +221,13 @@ __device__ __forceinline__ uint32_t decode_coefs_asm(LaneCoder &d, const QTab *_
 }
 
 // Header, coefficients and corrections of one block into the list at w (cap
-// words), as entropy_to_list_lane; ctx has kCtxSlots slots per lane.
+// words), as entropy_to_list_lane; ctx has kCtxSlots slots per lane, win a
+// window of W words (W + 1 allocated) per lane.
+template <int W = kWin>
 __device__ __forceinline__ uint2 entropy_to_list_asm(const uint8_t *payload, uint32_t len, const QTab *__restrict__ qt,
                                                      const uint16_t *band_start, uint32_t *w, uint32_t cap,
                                                      uint32_t *ctx, uint32_t *win) {
-    return entropy_to_list_lane_with<decode_coefs_asm>(payload, len, qt, band_start, w, cap, ctx, win);
+    return entropy_to_list_lane_with<W, decode_coefs_asm<W>>(payload, len, qt, band_start, w, cap, ctx, win);
 }
 
 }  // namespace mchip
