@@ -48,8 +48,9 @@ int         mchip_set_priors(const uint16_t *plo, const uint16_t *phi);
  * position; 2 = one bin per loop step, branch-free, payload in an LDS
  * window; 3 = the same bins with the loop in assembly and lanes that meet a
  * rare event paused and served together, the default); "persistent",
- * "persist_min" and "persist_wg" shape
- * the persistent entropy launch of batched decodes; "len_shift" and
+ * "persist_min", "persist_wg" and "persist_win" (payload window words per
+ * lane: 16, 32 = default, 64) shape the persistent entropy launch of batched
+ * decodes; "len_shift" and
  * "group_shift" the payload-length buckets of the lane order and of the
  * persistent group order; "recon_bpw" the reconstruction blocks per
  * workgroup; "slices" the chunk slices of a batched persistent decode, whose

@@ -89,6 +89,7 @@ struct Tuning {
     int persistent = 1;       // batched decode (>= persist_min chunks): persistent entropy launch
     int persist_min = 2;
     int persist_wg = 0;       // workgroups of the persistent launch (0: occupancy x CUs)
+    int persist_win = 32;     // payload window words per lane of the persistent launch (16, 32, 64)
     int len_shift = 3;        // k_index lane order: length buckets of 1 << len_shift bytes (2..5)
     int group_shift = 6;      // k_group_order: first-block length buckets of 1 << group_shift bytes (4..8)
     int recon_bpw = 1;        // k_recon blocks per workgroup (1, 2, 4)
@@ -443,17 +444,17 @@ int collect_timing(mchip_ctx *c) {
 }
 
 // Workgroups of the persistent entropy launch: as many as the device keeps
-// resident at once (occupancy per CU x CUs).
-int persistent_grid(int decoder) {
-    static std::atomic<int> cached[2];                 // lane decoder, assembly lane decoder
-    std::atomic<int> &slot = cached[decoder == 3 ? 1 : 0];
+// resident at once (occupancy per CU x CUs), per decoder and window size.
+int persistent_grid(int decoder, int win) {
+    static std::atomic<int> cached[2][3];              // lane / assembly lane decoder; window 16, 32, 64
+    std::atomic<int> &slot = cached[decoder == 3 ? 1 : 0][win == 16 ? 0 : (win == 64 ? 2 : 1)];
     const int have = slot.load(std::memory_order_relaxed);
     if (have) return have;
     int dev = 0, cus = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 1024;
-    if (occupancy_persist(decoder, &per) != hipSuccess || per <= 0) per = 4;
+    if (occupancy_persist(decoder, win, &per) != hipSuccess || per <= 0) per = 4;
     slot.store(cus * per, std::memory_order_relaxed);
     return cus * per;
 }
@@ -524,6 +525,7 @@ static int prepare_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
     a.n = n;
     a.decoder = tn.decoder;
     a.persistent = tn.persistent && n >= tn.persist_min && a.decoder >= 2;
+    a.persist_win = tn.persist_win;
     a.len_shift = tn.len_shift;
     a.group_shift = tn.group_shift;
     a.recon_bpw = tn.recon_bpw;
@@ -535,7 +537,7 @@ static int prepare_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
         if (ensure(c, &c->d_order, &c->cap_order, sizeof(uint32_t) * ((size_t)n * 64 + 2 * kMaxSlices))) return -1;
         a.order = (uint32_t *)c->d_order;
         a.pctl = a.order + (size_t)n * 64;
-        a.nwg = tn.persist_wg > 0 ? tn.persist_wg : persistent_grid(tn.decoder);
+        a.nwg = tn.persist_wg > 0 ? tn.persist_wg : persistent_grid(tn.decoder, tn.persist_win);
         if (a.nwg > n * 64) a.nwg = n * 64;
         // slices need descriptors in device memory (a slice addresses its own)
         // (automatic: two slices from 16 chunks; more slices lengthen the
@@ -639,6 +641,11 @@ int mchip_set_tuning(const char *key, int value) {
     if (!strcmp(key, "persist_wg")) {
         if (value < 0 || value > 1 << 20) return fail("persist_wg out of range");
         g_tune.persist_wg = value;
+        return 0;
+    }
+    if (!strcmp(key, "persist_win")) {
+        if (value != 16 && value != 32 && value != 64) return fail("persist_win must be 16, 32 or 64");
+        g_tune.persist_win = value;
         return 0;
     }
     if (!strcmp(key, "enc_batch")) {
@@ -850,6 +857,7 @@ int mchip_decode_chunks_static(const uint8_t *d_blobs, const uint64_t *blob_off,
     a.n = n;
     a.decoder = tn.decoder;
     a.persistent = tn.persistent && n >= tn.persist_min && a.decoder >= 2;
+    a.persist_win = tn.persist_win;
     a.len_shift = tn.len_shift;
     a.group_shift = tn.group_shift;
     a.recon_bpw = tn.recon_bpw;
@@ -857,7 +865,7 @@ int mchip_decode_chunks_static(const uint8_t *d_blobs, const uint64_t *blob_off,
     a.pctl = a.order + (size_t)n * 64;
     a.nwg = 0;
     if (a.persistent) {
-        a.nwg = tn.persist_wg > 0 ? tn.persist_wg : persistent_grid(tn.decoder);
+        a.nwg = tn.persist_wg > 0 ? tn.persist_wg : persistent_grid(tn.decoder, tn.persist_win);
         if (a.nwg > n * 64) a.nwg = n * 64;
     }
     hipStream_t s = (hipStream_t)stream;
@@ -935,7 +943,6 @@ int mchip_autotune_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
     // decodes never see a partial change.
     Tuning best = tuning_for(n, blob_len, q);
     best.persist_min = 1;
-    const int full = persistent_grid(best.decoder);
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -957,22 +964,24 @@ int mchip_autotune_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
         }
         char buf[256];
         snprintf(buf, sizeof buf,
-                 "%s{\"decoder\": %d, \"persistent\": %d, \"persist_wg\": %d, \"recon_bpw\": %d, "
-                 "\"len_shift\": %d, \"group_shift\": %d, \"slices\": %d, \"ms\": %.3f}",
-                 nrep++ ? ", " : "", tn.decoder, tn.persistent, tn.persist_wg, tn.recon_bpw, tn.len_shift,
-                 tn.group_shift, tn.slices, t);
+                 "%s{\"decoder\": %d, \"persistent\": %d, \"persist_win\": %d, \"persist_wg\": %d, "
+                 "\"recon_bpw\": %d, \"len_shift\": %d, \"group_shift\": %d, \"slices\": %d, \"ms\": %.3f}",
+                 nrep++ ? ", " : "", tn.decoder, tn.persistent, tn.persist_win, tn.persist_wg, tn.recon_bpw,
+                 tn.len_shift, tn.group_shift, tn.slices, t);
         rep += buf;
         *out_ms = t;
         return 0;
     };
     float best_ms;
     if (time_of(best, &best_ms)) return -1;
-    enum Knob { kDecoder, kPersistent, kPersistWg, kReconBpw, kLenShift, kGroupShift, kSlices };
+    enum Knob { kDecoder, kPersistent, kPersistWin, kPersistWg, kReconBpw, kLenShift, kGroupShift, kSlices };
     struct Step { Knob k; std::vector<int> vals; };
+    // kPersistWg candidates are quarters of the resident grid of the window size in use
     std::vector<Step> steps = {
         {kDecoder, {3, 2, 0}},
         {kPersistent, n >= 2 ? std::vector<int>{1, 0} : std::vector<int>{0}},
-        {kPersistWg, {full, full * 3 / 4, full / 2, full / 4}},
+        {kPersistWin, {32, 16, 64}},
+        {kPersistWg, {4, 3, 2, 1}},
         {kReconBpw, {1, 2, 4}},
         {kLenShift, {2, 3, 4}},
         {kGroupShift, {5, 6, 7}},
@@ -984,11 +993,19 @@ int mchip_autotune_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
             switch (st.k) {
             case kDecoder: if (tn.decoder == v) continue; tn.decoder = v; break;
             case kPersistent: if (tn.persistent == v) continue; tn.persistent = v; break;
-            case kPersistWg:
-                if (!tn.persistent || tn.decoder < 2 || n < 2) continue;
-                if ((tn.persist_wg ? tn.persist_wg : full) == v) continue;
-                tn.persist_wg = v;
+            case kPersistWin:
+                if (!tn.persistent || tn.decoder < 2 || n < 2 || tn.persist_win == v) continue;
+                tn.persist_win = v;
+                tn.persist_wg = 0;
                 break;
+            case kPersistWg: {
+                if (!tn.persistent || tn.decoder < 2 || n < 2) continue;
+                const int full = persistent_grid(tn.decoder, tn.persist_win);
+                const int wg = v == 4 ? 0 : full * v / 4;
+                if (tn.persist_wg == wg) continue;
+                tn.persist_wg = wg;
+                break;
+            }
             case kReconBpw: if (tn.recon_bpw == v) continue; tn.recon_bpw = v; break;
             case kLenShift: if (tn.len_shift == v) continue; tn.len_shift = v; break;
             case kGroupShift:

@@ -134,9 +134,6 @@ __device__ __forceinline__ void decode_header(RcDec &d, const QTab *__restrict__
 #ifndef MCHIP_LANE_WIN
 #define MCHIP_LANE_WIN 64
 #endif
-#ifndef MCHIP_PERSIST_WIN
-#define MCHIP_PERSIST_WIN 32
-#endif
 constexpr int32_t kWin = MCHIP_LANE_WIN;   // default window words per lane (multiple of 16)
 constexpr int32_t kWinAlloc = kWin + 1;   // words allocated per lane (see LaneCoder::norm_fast)
 

@@ -150,10 +150,11 @@ constexpr int kDecPosition = 0, kDecLane = 2, kDecAsm = 3;
 
 // Payload window words per lane of the lane decoders: 64 for the launches of
 // one chunk (fewer window refills on the longest waves, which set a chunk's
-// time), 32 for the persistent batched launch, where the smaller LDS
-// footprint lets more waves share each CU (batched plain64 207 -> 216 GB/s,
-// mixed-quality 97 -> 115 GB/s; one chunk alone is 2 % slower at 32).
-constexpr int kWinGrid = kWin, kWinPersist = MCHIP_PERSIST_WIN;
+// time); the persistent batched launch takes 16, 32 (the default) or 64
+// (tuning "persist_win"), where a smaller LDS footprint lets more waves share
+// each CU (batched plain64 207 -> 214 GB/s and mixed-quality 97 -> 114 GB/s
+// at 32; one chunk alone is 2 % slower at 32).
+constexpr int kWinGrid = kWin;
 
 template <int kMode, int W = kWinGrid>
 __device__ __forceinline__ uint2 entropy_lane(const uint8_t *pay, uint32_t len, const QTab *qt, const uint16_t *band,
@@ -255,7 +256,7 @@ __global__ __launch_bounds__(1024) void k_group_order(const BlkRec *__restrict__
     }
 }
 
-template <int kMode>
+template <int kMode, int W>
 __global__ __launch_bounds__(64) void k_entropy_persist(const uint8_t *__restrict__ blobs, const DescRef cd,
                                                         const QTab *__restrict__ qtabs,
                                                         const BlkRec *__restrict__ rec,
@@ -266,7 +267,7 @@ __global__ __launch_bounds__(64) void k_entropy_persist(const uint8_t *__restric
                                                         uint32_t *__restrict__ pctl) {
     static_assert(kMode == kDecLane || kMode == kDecAsm, "persistent launch takes a lane decoder");
     __shared__ uint32_t sctx[kCtxSlots * 64];
-    __shared__ uint32_t pwin[(kWinPersist + 1) * 64];
+    __shared__ uint32_t pwin[(W + 1) * 64];
     __shared__ QTab sq;
     __shared__ uint16_t sband[9];
     if (threadIdx.x < 9) sband[threadIdx.x] = c_band_start[threadIdx.x];
@@ -297,7 +298,7 @@ __global__ __launch_bounds__(64) void k_entropy_persist(const uint8_t *__restric
             } else {
                 const uint8_t *pay = blobs + cd[c].blob_off + r.off;
                 const uint64_t wofs = cd[c].scratch_off + 8ull * (r.off - cinfo[c].pay0) + 32ull * r.slot;
-                binfo[cb + bi] = entropy_lane<kMode, kWinPersist>(pay, r.len, &sq, sband, scratch + wofs,
+                binfo[cb + bi] = entropy_lane<kMode, W>(pay, r.len, &sq, sband, scratch + wofs,
                                                                   8u * r.len + 32u, nullptr, sctx + threadIdx.x,
                                                                   pwin + threadIdx.x);
             }
@@ -524,22 +525,36 @@ void launch_index(const DecodeArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_index, dim3(a.n), dim3(256), 0, s, a.blobs, a.cd, a.rec, a.plist, a.cinfo,
                        (uint32_t)a.len_shift);
 }
-hipError_t occupancy_persist(int decoder, int *per_cu) {
-    return decoder == kDecLane
-               ? hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_entropy_persist<kDecLane>, 64, 0)
-               : hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_entropy_persist<kDecAsm>, 64, 0);
+template <int kMode>
+static hipError_t occupancy_persist_mode(int win, int *per_cu) {
+    if (win == 16) return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_entropy_persist<kMode, 16>, 64, 0);
+    if (win == 64) return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_entropy_persist<kMode, 64>, 64, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_entropy_persist<kMode, 32>, 64, 0);
+}
+hipError_t occupancy_persist(int decoder, int win, int *per_cu) {
+    return decoder == kDecLane ? occupancy_persist_mode<kDecLane>(win, per_cu)
+                               : occupancy_persist_mode<kDecAsm>(win, per_cu);
+}
+
+template <int kMode>
+static void launch_persist(const DecodeArgs &a, hipStream_t s) {
+    if (a.persist_win == 16)
+        hipLaunchKernelGGL((k_entropy_persist<kMode, 16>), dim3(a.nwg), dim3(64), 0, s, a.blobs, a.cd, a.qtabs,
+                           a.rec, a.plist, a.cinfo, a.scratch, a.binfo, a.order, a.pctl);
+    else if (a.persist_win == 64)
+        hipLaunchKernelGGL((k_entropy_persist<kMode, 64>), dim3(a.nwg), dim3(64), 0, s, a.blobs, a.cd, a.qtabs,
+                           a.rec, a.plist, a.cinfo, a.scratch, a.binfo, a.order, a.pctl);
+    else
+        hipLaunchKernelGGL((k_entropy_persist<kMode, 32>), dim3(a.nwg), dim3(64), 0, s, a.blobs, a.cd, a.qtabs,
+                           a.rec, a.plist, a.cinfo, a.scratch, a.binfo, a.order, a.pctl);
 }
 
 void launch_entropy(const DecodeArgs &a, hipStream_t s) {
     if (a.persistent && (a.decoder == kDecLane || a.decoder == kDecAsm)) {
         hipLaunchKernelGGL(k_group_order, dim3(1), dim3(1024), 0, s, a.rec, a.plist, a.cinfo, a.n, a.order, a.pctl,
                            (uint32_t)a.group_shift);
-        if (a.decoder == kDecAsm)
-            hipLaunchKernelGGL(k_entropy_persist<kDecAsm>, dim3(a.nwg), dim3(64), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
-                               a.plist, a.cinfo, a.scratch, a.binfo, a.order, a.pctl);
-        else
-            hipLaunchKernelGGL(k_entropy_persist<kDecLane>, dim3(a.nwg), dim3(64), 0, s, a.blobs, a.cd, a.qtabs, a.rec,
-                               a.plist, a.cinfo, a.scratch, a.binfo, a.order, a.pctl);
+        if (a.decoder == kDecAsm) launch_persist<kDecAsm>(a, s);
+        else launch_persist<kDecLane>(a, s);
         return;
     }
     if (a.decoder == kDecAsm)

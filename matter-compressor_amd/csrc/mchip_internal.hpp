@@ -53,6 +53,7 @@ struct DecodeArgs {
     uint8_t *out;
     int n;
     int decoder;            // entropy decoder variant (tuning: 0 position, 2 lane, 3 lane in assembly)
+    int persist_win;        // payload window words per lane of the persistent launch (16, 32 or 64)
     // persistent entropy launch (lane decoder): nwg workgroups take 64-block
     // groups from order[] (longest payloads first, over all chunks) through
     // the counter at pctl[0]; pctl[1] holds the number of groups
@@ -200,7 +201,7 @@ constexpr size_t kStaticScratchBytes = 128ull << 20;
 hipError_t static_scratch_address(void **p);
 void launch_put_qtabs(const QTab *tabs, int n, QTab *dst, hipStream_t s);
 void launch_index(const DecodeArgs &a, hipStream_t s);
-hipError_t occupancy_persist(int decoder, int *per_cu);   // resident persistent-entropy workgroups per CU
+hipError_t occupancy_persist(int decoder, int win, int *per_cu);   // resident persistent-entropy workgroups per CU
 void launch_entropy(const DecodeArgs &a, hipStream_t s);
 void launch_recon(const DecodeArgs &a, hipStream_t s);
 void launch_block_single(const uint8_t *base, const uint64_t *off, const uint32_t *len, const uint32_t *qi,

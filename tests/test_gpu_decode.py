@@ -238,8 +238,8 @@ def test_every_launch_shape_and_the_autotuner_keep_results_exact(gpu_ctx):
     d_in, offs, lens, qs = _resident(gpu_ctx, blobs)
     d_out = mc_amd.DeviceBuffer(len(blobs) * mc_amd.CHUNK_BYTES)
     got = np.empty((len(blobs), 256, 256, 256), np.uint8)
-    defaults = [("decoder", 3), ("persistent", 1), ("persist_min", 2), ("persist_wg", 0), ("recon_bpw", 1),
-                ("len_shift", 3), ("group_shift", 6)]
+    defaults = [("decoder", 3), ("persistent", 1), ("persist_min", 2), ("persist_wg", 0), ("persist_win", 32),
+                ("recon_bpw", 1), ("len_shift", 3), ("group_shift", 6)]
 
     def check(what):
         gpu_ctx.decode_chunks(d_in, offs, lens, qs, d_out)
@@ -250,7 +250,8 @@ def test_every_launch_shape_and_the_autotuner_keep_results_exact(gpu_ctx):
 
     try:
         knobs = [("decoder", [0, 2, 3]), ("persistent", [0, 1]), ("persist_wg", [0, 64, 300]),
-                 ("recon_bpw", [1, 2, 4]), ("len_shift", [2, 3, 4, 5]), ("group_shift", [4, 5, 6, 7, 8])]
+                 ("persist_win", [16, 64, 32]), ("recon_bpw", [1, 2, 4]), ("len_shift", [2, 3, 4, 5]),
+                 ("group_shift", [4, 5, 6, 7, 8])]
         for key, vals in knobs:
             for v in vals:
                 mc_amd.set_tuning(key, v)
