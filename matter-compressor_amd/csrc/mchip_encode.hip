@@ -288,6 +288,7 @@ __global__ __launch_bounds__(64) void k_enc_entropy(const EncChunkDesc *__restri
     // chunk of this row of workgroups (rows are dispatched in order)
     const uint32_t cy = kBlocks || !order ? blockIdx.y : order[blockIdx.y];
     __shared__ uint32_t stage[kEncStage * 64];
+    static_assert(sizeof(stage) >= kNVox, "the fraction map stages its 4096 nibbles in the staging area");
     __shared__ QTab sq;
     __shared__ uint16_t sband[9];
     if (!kBlocks && blockIdx.x == 0) {   // one extra workgroup per chunk, dispatched first: the fraction map
