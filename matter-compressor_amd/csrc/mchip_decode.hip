@@ -399,6 +399,15 @@ __device__ void recon_single_lane(const uint8_t *payload, uint32_t len, const QT
     __syncthreads();
 }
 
+// Whether any of a line's 16 values is nonzero (either sign of zero counts
+// as zero).
+__device__ __forceinline__ bool line_nonzero(const float (&v)[16]) {
+    uint32_t m = 0;
+    #pragma unroll
+    for (int k = 0; k < 16; ++k) m |= __float_as_uint(v[k]) << 1;
+    return m != 0;
+}
+
 // Dequantise a level list into the LDS tile, inverse transform, round, add
 // dc, clamp and apply the corrections; leaves the voxels in u8t.
 __device__ __forceinline__ void recon_from_list(const uint2 inf, const uint32_t *__restrict__ w,
@@ -419,32 +428,47 @@ __device__ __forceinline__ void recon_from_list(const uint2 inf, const uint32_t 
             tile[(idx >> 4) * kPad + (idx & 15)] = dequant((int)(int16_t)(e >> 16), qt->step[l1]);
         }
         __syncthreads();
+        // A line whose 16 inputs are all zero transforms to zeros (every sum
+        // starts at +0 and adding a signed zero never changes it), so its
+        // transform is skipped: the tile already holds zeros there, and in
+        // the last pass its voxels are dc.  A wave skips the arithmetic when
+        // all its lines are zero, which for the sparse high-q blocks is three
+        // of the four waves in the x and y passes.
         float v[16];
         {   // x pass: line (z,y) = t
             float *ln = tile + t * kPad;
             #pragma unroll
             for (int k = 0; k < 16; ++k) v[k] = ln[k];
-            idct16_line(v, cmf);
-            #pragma unroll
-            for (int k = 0; k < 16; ++k) ln[k] = v[k];
+            if (line_nonzero(v)) {
+                idct16_line(v, cmf);
+                #pragma unroll
+                for (int k = 0; k < 16; ++k) ln[k] = v[k];
+            }
         }
         __syncthreads();
         {   // y pass: line (z,x)
             float *ln = tile + (t >> 4) * 16 * kPad + (t & 15);
             #pragma unroll
             for (int k = 0; k < 16; ++k) v[k] = ln[k * kPad];
-            idct16_line(v, cmf);
-            #pragma unroll
-            for (int k = 0; k < 16; ++k) ln[k * kPad] = v[k];
+            if (line_nonzero(v)) {
+                idct16_line(v, cmf);
+                #pragma unroll
+                for (int k = 0; k < 16; ++k) ln[k * kPad] = v[k];
+            }
         }
         __syncthreads();
         {   // z pass: line (y,x); round, + dc, clamp into the byte tile
             float *ln = tile + (t >> 4) * kPad + (t & 15);
             #pragma unroll
             for (int k = 0; k < 16; ++k) v[k] = ln[k * 16 * kPad];
-            idct16_line(v, cmf);
-            #pragma unroll
-            for (int k = 0; k < 16; ++k) u8t[k * 256 + t] = recon_voxel(v[k], dc);
+            if (line_nonzero(v)) {
+                idct16_line(v, cmf);
+                #pragma unroll
+                for (int k = 0; k < 16; ++k) u8t[k * 256 + t] = recon_voxel(v[k], dc);
+            } else {
+                #pragma unroll
+                for (int k = 0; k < 16; ++k) u8t[k * 256 + t] = recon_voxel(0.0f, dc);
+            }
         }
     }
     if (flags & 2) {
