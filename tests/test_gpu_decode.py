@@ -73,9 +73,23 @@ def test_chunk_decode_truncated_and_corrupt_blobs_match_oracle(gpu_ctx):
     for _ in range(50):
         b2[int(rng.integers(700, len(b2)))] ^= 0x5A
     variants.append(bytes(b2))
+    want = [ob.decode_chunk(v) for v in variants]
     out = gpu_ctx.decode_chunks_host(variants)
     for k, v in enumerate(variants):
-        assert np.array_equal(out[k], ob.decode_chunk(v)), f"variant {k}"
+        assert np.array_equal(out[k], want[k]), f"variant {k}"
+    # the persistent decoder with every payload window size (a 16-word window
+    # slides every 64 payload bytes, also across the corrupt and cut-off ends)
+    try:
+        mc_amd.set_tuning("persist_min", 1)
+        for win in (16, 64, 32):
+            mc_amd.set_tuning("persist_win", win)
+            print(f"persist_win={win}", flush=True)
+            out = gpu_ctx.decode_chunks_host(variants)
+            for k in range(len(variants)):
+                assert np.array_equal(out[k], want[k]), f"persist_win={win} variant {k}"
+    finally:
+        mc_amd.set_tuning("persist_win", 32)
+        mc_amd.set_tuning("persist_min", 2)
 
 
 def test_all_air_and_sparse_chunks(gpu_ctx):
