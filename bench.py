@@ -447,6 +447,10 @@ def mixed_leg(nthreads):
     mc_amd.set_tuning("enc_keep_mib", 1 << 20)
     ctx.encode_chunks(d_in, qv, tv, d_enc, eoffs, ecaps)          # workspace allocation
     ctx.sync()
+    # the encode shapes autotuned on the batch (cached for its signature and
+    # used by the timed call)
+    enc_tune = ctx.autotune_encode(d_in, qv, tv, d_enc, eoffs, ecaps)
+    log(f"[bench] mixed encode autotune: {enc_tune}")
     t0 = time.perf_counter()
     elens, est = ctx.encode_chunks(d_in, qv, tv, d_enc, eoffs, ecaps)
     enc_s = time.perf_counter() - t0
@@ -519,7 +523,8 @@ def mixed_leg(nthreads):
            "blob_bytes": int(np.sum(elens)),
            "encode_gbs": round(n * mc_amd.CHUNK_BYTES / enc_s / 1e9, 3), "encode_byte_identical": ok_enc,
            "encode_graph_gbs": (round(n * mc_amd.CHUNK_BYTES / enc_graph_s / 1e9, 3) if enc_graph_s else None),
-           "encode_graph_byte_identical": ok_graph_enc,
+           "encode_graph_byte_identical": ok_graph_enc, "encode_autotune": enc_tune,
+           "encode_chosen": min(enc_tune, key=lambda r: r["ms"]),
            "decode_gbs": round(n * mc_amd.CHUNK_BYTES / min(ts) / 1e9, 3),
            "decode_graph_gbs": round(n * mc_amd.CHUNK_BYTES / min(tg) / 1e9, 3),
            "bit_exact": bad_eager == 0 and bad_graph == 0, "autotune": tune, "chosen": best,

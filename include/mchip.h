@@ -59,7 +59,9 @@ int         mchip_set_priors(const uint16_t *plo, const uint16_t *phi);
  * chunks per encode launch sequence (0 = sized from free device memory),
  * "enc_slices" the chunk slices of one encode sequence, run alternately on
  * two streams (0 = one slice; more slices measured slower on the mixed
- * workload, as each slice's entropy pass then ends in its own tail), and
+ * workload, as each slice's entropy pass then ends in its own tail),
+ * "enc_order" the entropy pass's chunk order (1 = heaviest chunk first, the
+ * default; 0 = sequence order), and
  * "enc_keep_mib" the encode workspace a context keeps after a call;
  * "merge" (default 1), "merge_inflight" and "merge_min" shape the merging of
  * concurrent one-chunk host decodes.  Results
@@ -265,7 +267,20 @@ int          mchip_autotune_decode(mchip_ctx *ctx, const uint8_t *d_blobs, const
                                    const uint64_t *blob_len, const float *q, int n, uint8_t *d_out,
                                    uint64_t out_stride, char *report, size_t cap);
 
-/* The five launches of one mchip_encode_chunks sequence (n chunks, at most
+/* Time the encode shapes on these chunks by coordinate descent (the
+ * entropy pass's chunk order, chunks per launch sequence, chunk slices),
+ * each candidate a whole mchip_encode_chunks of the caller's chunks into the
+ * caller's output (the blobs are rewritten, byte for byte the same); keep
+ * the fastest for encodes of the same signature (chunk count as a bit length,
+ * mixed or single quality, corrections or none) -- later mchip_encode_chunks
+ * calls with that signature use it -- and write the timings as JSON to report
+ * (cap bytes).  No counterpart in the reference, whose only encode knob is
+ * the thread count of mc_archive_append_chunk_par (src/mc_archive.c:805-809). */
+int          mchip_autotune_encode(mchip_ctx *ctx, const uint8_t *d_in, uint64_t in_stride, const float *q,
+                                   const int *tau, const float *q_store, int n, uint8_t *d_out,
+                                   const uint64_t *out_off, const uint64_t *out_cap, char *report, size_t cap);
+
+/* The launches of one mchip_encode_chunks sequence (n chunks, at most
  * one encode sequence) captured as a HIP graph over a private workspace;
  * mchip_graph_launch replays it on ctx's stream, encoding the rasters at
  * the captured addresses into the captured output, and

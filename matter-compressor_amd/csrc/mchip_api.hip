@@ -94,6 +94,7 @@ struct Tuning {
     int group_shift = 6;      // k_group_order: first-block length buckets of 1 << group_shift bytes (4..8)
     int recon_bpw = 1;        // k_recon blocks per workgroup (1, 2, 4)
     int enc_slices = 0;       // encode sequence: chunk slices on two streams (0: one slice)
+    int enc_order = 1;        // encode entropy pass: 1 heaviest chunk first, 0 in sequence order
     int slices = 0;           // batched persistent decode: chunk slices whose reconstruction overlaps the
                               // next slice's entropy pass on a second stream (0: 2 from 16 chunks, else 1)
     int enc_keep_mib = 2048;  // encode workspace a context keeps after a call (MiB); more is freed
@@ -141,6 +142,38 @@ Tuning tuning_for(int n, const uint64_t *len, const float *q) {
         }
     }
     return g_tune;
+}
+
+// Encode shapes chosen by mchip_autotune_encode, per encode signature: chunk
+// count as a bit length, whether the qualities differ and whether any chunk
+// has corrections.  Guarded by g_tune_mu.
+std::map<uint32_t, Tuning> g_tuned_enc;
+
+uint32_t encode_signature(int n, const float *q, const int *tau) {
+    bool mixed = false, corr = false;
+    for (int i = 0; i < n; ++i) {
+        mixed |= q[i] != q[0];
+        corr |= tau && tau[i] > 0;
+    }
+    uint32_t b = 0;
+    for (uint32_t x = (uint32_t)n; x; x >>= 1) b++;
+    return b | ((mixed ? 1u : 0u) << 8) | ((corr ? 1u : 0u) << 9);
+}
+
+// The tuning an encode of these chunks uses: the process tuning with the
+// autotuned encode shapes for its signature, if any.
+Tuning tuning_for_encode(int n, const float *q, const int *tau) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    Tuning t = g_tune;
+    if (!g_tuned_enc.empty()) {
+        auto it = g_tuned_enc.find(encode_signature(n, q, tau));
+        if (it != g_tuned_enc.end()) {
+            t.enc_batch = it->second.enc_batch;
+            t.enc_slices = it->second.enc_slices;
+            t.enc_order = it->second.enc_order;
+        }
+    }
+    return t;
 }
 
 // One prior configuration: the q=1 and q=12 endpoint tables.
@@ -684,7 +717,12 @@ int mchip_set_tuning(const char *key, int value) {
         g_tune.slices = value;
         return 0;
     }
-    if (!strcmp(key, "tuned_clear")) { g_tuned.clear(); return 0; }
+    if (!strcmp(key, "tuned_clear")) { g_tuned.clear(); g_tuned_enc.clear(); return 0; }
+    if (!strcmp(key, "enc_order")) {
+        if (value != 0 && value != 1) return fail("enc_order must be 0 or 1");
+        g_tune.enc_order = value;
+        return 0;
+    }
     if (!strcmp(key, "enc_keep_mib")) {
         if (value < 0) return fail("enc_keep_mib must be >= 0");
         g_tune.enc_keep_mib = value;
@@ -1326,8 +1364,8 @@ constexpr int kEncBatch = 32;
 // (measured on 256 mixed-quality chunks: 26 GB/s at 64 per sequence, 37 at
 // 128, 49 at 256).  Automatic: a quarter of the free device memory, from 16
 // to 256 chunks.
-static int enc_batch_for() {
-    const int fixed = tuning().enc_batch;
+static int enc_batch_for(const Tuning &tn) {
+    const int fixed = tn.enc_batch;
     if (fixed > 0) return fixed;
     constexpr size_t kPerChunk = (size_t)kNVox * kNVox * (sizeof(int16_t) + sizeof(uint32_t)) +
                                  (size_t)kNVox * kSlotBytes + (size_t)kNVox * 32 + kFmapStride;
@@ -1344,13 +1382,12 @@ static int enc_batch_for() {
 // out_off and out_cap are indexed from 0.
 static int prepare_encode(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride, const float *q, const int *tau,
                           const float *q_store, int m, uint8_t *d_out, const uint64_t *out_off,
-                          const uint64_t *out_cap, EncodeArgs &a) {
+                          const uint64_t *out_cap, const Tuning &tn, EncodeArgs &a) {
     void *h = nullptr;
     const size_t order_at = sizeof(EncChunkDesc) * (size_t)m;
     if (desc_buffer(c, order_at + sizeof(uint32_t) * (size_t)m, &h)) return -1;
     EncChunkDesc *cd = (EncChunkDesc *)h;
     // Chunk slices of the sequence (run on two streams, see launch_encode)
-    const Tuning tn = tuning();
     const int want = tn.enc_slices > 0 ? tn.enc_slices : 1;
     const int ns = std::max(1, std::min(want, std::min(kMaxSlices, m)));
     if (ns > 1 && !c->stream_b &&
@@ -1367,11 +1404,12 @@ static int prepare_encode(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride,
     for (int si = 0; si < ns; ++si) {
         const int c0 = (int)((int64_t)m * si / ns), c1 = (int)((int64_t)m * (si + 1) / ns);
         for (int i = c0; i < c1; ++i) ord[i] = (uint32_t)i;
-        std::stable_sort(ord + c0, ord + c1, [&](uint32_t x, uint32_t y) {
-            const float kx = qkey(x), ky = qkey(y);
-            const int tx = tau ? tau[x] : 0, ty = tau ? tau[y] : 0;
-            return kx < ky || (kx == ky && tx > ty);
-        });
+        if (tn.enc_order)
+            std::stable_sort(ord + c0, ord + c1, [&](uint32_t x, uint32_t y) {
+                const float kx = qkey(x), ky = qkey(y);
+                const int tx = tau ? tau[x] : 0, ty = tau ? tau[y] : 0;
+                return kx < ky || (kx == ky && tx > ty);
+            });
         for (int i = c0; i < c1; ++i) ord[i] -= (uint32_t)c0;
     }
     for (int i = 0; i < m; ++i) {
@@ -1487,16 +1525,11 @@ static int trim_encode_workspace(mchip_ctx *c) {
     return 0;
 }
 
-extern "C" {
-
-int mchip_encode_chunks(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride, const float *q, const int *tau,
-                        const float *q_store, int n, uint8_t *d_out, const uint64_t *out_off,
-                        const uint64_t *out_cap, uint64_t *blob_len, int *status) {
-    if (!c) return fail("null context");
-    if (n <= 0) return 0;
-    if (!d_in || !q || !d_out || !out_off || !out_cap || !blob_len) return fail("null argument");
-    if (in_stride % 16 || ((uintptr_t)d_in) % 16) return fail("input rasters must be 16-byte aligned");
-    const int eb = enc_batch_for();
+// The encode of mchip_encode_chunks with a given tuning (arguments checked).
+static int encode_sequences(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride, const float *q, const int *tau,
+                            const float *q_store, int n, uint8_t *d_out, const uint64_t *out_off,
+                            const uint64_t *out_cap, uint64_t *blob_len, int *status, const Tuning &tn) {
+    const int eb = enc_batch_for(tn);
     for (int b0 = 0; b0 < n; b0 += eb) {
         const int m = n - b0 < eb ? n - b0 : eb;
         if ((size_t)m > c->cap_h_res) {
@@ -1507,7 +1540,7 @@ int mchip_encode_chunks(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride, c
         }
         EncodeArgs a;
         if (prepare_encode(c, d_in + (size_t)b0 * in_stride, in_stride, q + b0, tau ? tau + b0 : nullptr,
-                           q_store ? q_store + b0 : nullptr, m, d_out, out_off + b0, out_cap + b0, a))
+                           q_store ? q_store + b0 : nullptr, m, d_out, out_off + b0, out_cap + b0, tn, a))
             return -1;
         launch_encode(c, a);
         HIPCHK(hipGetLastError());
@@ -1522,6 +1555,88 @@ int mchip_encode_chunks(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride, c
     return trim_encode_workspace(c);
 }
 
+extern "C" {
+
+int mchip_encode_chunks(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride, const float *q, const int *tau,
+                        const float *q_store, int n, uint8_t *d_out, const uint64_t *out_off,
+                        const uint64_t *out_cap, uint64_t *blob_len, int *status) {
+    if (!c) return fail("null context");
+    if (n <= 0) return 0;
+    if (!d_in || !q || !d_out || !out_off || !out_cap || !blob_len) return fail("null argument");
+    if (in_stride % 16 || ((uintptr_t)d_in) % 16) return fail("input rasters must be 16-byte aligned");
+    return encode_sequences(c, d_in, in_stride, q, tau, q_store, n, d_out, out_off, out_cap, blob_len, status,
+                            tuning_for_encode(n, q, tau));
+}
+
+int mchip_autotune_encode(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride, const float *q, const int *tau,
+                          const float *q_store, int n, uint8_t *d_out, const uint64_t *out_off,
+                          const uint64_t *out_cap, char *report, size_t cap) {
+    if (!c || n <= 0 || !d_in || !q || !d_out || !out_off || !out_cap || in_stride % 16 || ((uintptr_t)d_in) % 16)
+        return fail("mchip_autotune_encode: bad argument");
+    // Coordinate descent over the encode shapes (chunks per launch sequence,
+    // entropy-pass chunk order, chunk slices), each candidate timed (best of
+    // two whole encodes of the caller's chunks, which rewrite the same
+    // blobs) with a private tuning; the winner is cached for this encode's
+    // signature (chunk count, mixed qualities, corrections).
+    Tuning best = tuning_for_encode(n, q, tau);
+    std::vector<uint64_t> lens(n);
+    std::string rep = "[";
+    int nrep = 0;
+    auto time_of = [&](const Tuning &tn, float *out_ms) -> int {
+        double t = 1e30;
+        for (int rep_i = 0; rep_i < 2; ++rep_i) {
+            const auto t0 = std::chrono::steady_clock::now();
+            if (encode_sequences(c, d_in, in_stride, q, tau, q_store, n, d_out, out_off, out_cap, lens.data(),
+                                 nullptr, tn))
+                return -1;
+            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (ms < t) t = ms;
+        }
+        char buf[192];
+        snprintf(buf, sizeof buf, "%s{\"enc_batch\": %d, \"enc_order\": %d, \"enc_slices\": %d, \"ms\": %.3f}",
+                 nrep++ ? ", " : "", enc_batch_for(tn), tn.enc_order, tn.enc_slices > 0 ? tn.enc_slices : 1, t);
+        rep += buf;
+        *out_ms = (float)t;
+        return 0;
+    };
+    float best_ms;
+    if (time_of(best, &best_ms)) return -1;
+    const int full = enc_batch_for(best);
+    enum Knob { kOrder, kBatch, kSlices };
+    struct Step { Knob k; std::vector<int> vals; };
+    const std::vector<Step> steps = {
+        {kOrder, {1, 0}},
+        {kBatch, {0, std::max(1, full / 2), std::max(1, full / 4)}},
+        {kSlices, {1, 2}},
+    };
+    for (const Step &st : steps) {
+        for (int v : st.vals) {
+            Tuning tn = best;
+            switch (st.k) {
+            case kOrder: if (tn.enc_order == v) continue; tn.enc_order = v; break;
+            case kBatch:
+                if (tn.enc_batch == v || (v > 0 && v >= std::min(n, full))) continue;
+                tn.enc_batch = v;
+                break;
+            case kSlices:
+                if ((tn.enc_slices > 0 ? tn.enc_slices : 1) == v || v > std::min(n, enc_batch_for(tn))) continue;
+                tn.enc_slices = v;
+                break;
+            }
+            float ms;
+            if (time_of(tn, &ms)) return -1;
+            if (ms < best_ms * 0.99f) { best_ms = ms; best = tn; }
+        }
+    }
+    rep += "]";
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned_enc[encode_signature(n, q, tau)] = best;
+    }
+    if (report && cap) snprintf(report, cap, "%s", rep.c_str());
+    return 0;
+}
+
 mchip_graph *mchip_encode_graph_create(mchip_ctx *caller, const uint8_t *d_in, uint64_t in_stride, const float *q,
                                        const int *tau, const float *q_store, int n, uint8_t *d_out,
                                        const uint64_t *out_off, const uint64_t *out_cap) {
@@ -1530,8 +1645,9 @@ mchip_graph *mchip_encode_graph_create(mchip_ctx *caller, const uint8_t *d_in, u
         fail("mchip_encode_graph_create: bad argument");
         return nullptr;
     }
-    if (n > enc_batch_for()) {
-        fail("mchip_encode_graph_create: %d chunks exceed one encode sequence (%d)", n, enc_batch_for());
+    const Tuning tn = tuning_for_encode(n, q, tau);
+    if (n > enc_batch_for(tn)) {
+        fail("mchip_encode_graph_create: %d chunks exceed one encode sequence (%d)", n, enc_batch_for(tn));
         return nullptr;
     }
     auto *g = new mchip_graph;
@@ -1539,7 +1655,7 @@ mchip_graph *mchip_encode_graph_create(mchip_ctx *caller, const uint8_t *d_in, u
     g->ctx = mchip_ctx_create();
     g->n = n;
     EncodeArgs a;
-    bool ok = g->ctx && !prepare_encode(g->ctx, d_in, in_stride, q, tau, q_store, n, d_out, out_off, out_cap, a) &&
+    bool ok = g->ctx && !prepare_encode(g->ctx, d_in, in_stride, q, tau, q_store, n, d_out, out_off, out_cap, tn, a) &&
               hipStreamSynchronize(g->ctx->stream) == hipSuccess;
     if (ok) {
         // the five launches only: descriptors stay in the graph's workspace
@@ -1591,7 +1707,7 @@ int mchip_encode_chunks_host(mchip_ctx *c, const uint8_t *chunks, const float *q
         tot += (caps[i] + 255) & ~255ull;
     }
     std::vector<int> st(n, 0);
-    const int eb = enc_batch_for();
+    const int eb = enc_batch_for(tuning_for_encode(n, q, tau));
     for (int b0 = 0; b0 < n; b0 += eb) {
         const int m = n - b0 < eb ? n - b0 : eb;
         uint64_t base = off[b0], span = 0;

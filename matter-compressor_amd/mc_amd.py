@@ -58,6 +58,7 @@ def load():
         "mchip_encode_graph_results": (i, [vp, vp, vp]),
         "mchip_graph_destroy": (None, [vp]),
         "mchip_autotune_decode": (i, [vp, vp, vp, vp, vp, i, vp, C.c_uint64, C.c_char_p, C.c_size_t]),
+        "mchip_autotune_encode": (i, [vp, vp, C.c_uint64, vp, vp, vp, i, vp, vp, vp, C.c_char_p, C.c_size_t]),
         "mchip_store_create": (vp, [vp, C.c_uint64, vp, vp, vp, i, i, i]),
         "mchip_store_destroy": (None, [vp]),
         "mchip_store_device_bytes": (C.c_uint64, [vp]),
@@ -286,6 +287,20 @@ class Ctx:
         rep = C.create_string_buffer(16384)
         check(load().mchip_autotune_decode(self.h, C.c_void_p(d_blobs.ptr), _ptr(offs), _ptr(lens), _ptr(qs),
                                            len(offs), C.c_void_p(d_out.ptr), out_stride, rep, 16384))
+        return _json.loads(rep.value.decode())
+
+    def autotune_encode(self, d_in, qs, taus, d_out, out_offs, out_caps, q_store=None, in_stride=CHUNK_BYTES):
+        """Time the encode shapes on these chunks; keeps the fastest.  Returns the timings."""
+        import json as _json
+        qs = np.ascontiguousarray(qs, np.float32)
+        taus = np.ascontiguousarray(taus, np.int32)
+        offs = np.ascontiguousarray(out_offs, np.uint64)
+        caps = np.ascontiguousarray(out_caps, np.uint64)
+        qst = None if q_store is None else np.ascontiguousarray(q_store, np.float32)
+        rep = C.create_string_buffer(8192)
+        check(load().mchip_autotune_encode(self.h, C.c_void_p(d_in.ptr), in_stride, _ptr(qs), _ptr(taus),
+                                           None if qst is None else _ptr(qst), len(qs), C.c_void_p(d_out.ptr),
+                                           _ptr(offs), _ptr(caps), rep, 8192))
         return _json.loads(rep.value.decode())
 
     def decode_chunks_host(self, blobs):

@@ -184,3 +184,51 @@ def test_encode_tiny_blocks_and_unaligned_outputs_match_oracle(gpu_ctx):
     finally:
         d_in.free()
         d_out.free()
+
+
+def test_encode_autotuner_and_every_encode_shape_keep_blobs_identical(gpu_ctx):
+    """Every encode shape the autotuner searches (entropy-pass chunk order,
+    chunks per launch sequence, chunk slices) writes the reference's blobs
+    (manifest); so do the autotuned shapes cached for the chunks' signature."""
+    import mc_amd
+    man = json.load(open(os.path.join(ROOT, "tests", "golden", "chunk_manifest.json")))
+    specs = [man["mixed256"][k] for k in (2, 5, 7, 12, 30)]
+    chunks = mc_amd.synth_chunks([s["seed"] for s in specs], [s["air"] for s in specs])
+    n = len(specs)
+    cap = 20 << 20
+    d_in = mc_amd.DeviceBuffer(n * mc_amd.CHUNK_BYTES)
+    d_out = mc_amd.DeviceBuffer(n * cap)
+    gpu_ctx.h2d(d_in, chunks)
+    offs = np.arange(n, dtype=np.uint64) * cap
+    caps = np.full(n, cap, np.uint64)
+    qs = [s["q"] for s in specs]
+    taus = [s["tau"] for s in specs]
+
+    def check(what):
+        gpu_ctx.h2d(d_out, np.full(n * cap, 0x77, np.uint8))
+        lens, st = gpu_ctx.encode_chunks(d_in, qs, taus, d_out, offs, caps)
+        assert (st == 0).all(), what
+        for k, s in enumerate(specs):
+            assert int(lens[k]) == s["blob_len"], f"{what}: chunk {k} length"
+            b = np.empty(int(lens[k]), np.uint8)
+            gpu_ctx.d2h(b, d_out, offset=int(offs[k]))
+            assert f"{ob.xxh64(b):016x}" == s["blob_xxh64"], f"{what}: chunk {k}"
+
+    try:
+        for key, vals, default in [("enc_order", [0, 1], 1), ("enc_batch", [2, 3, 0], 0), ("enc_slices", [2, 0], 0)]:
+            for v in vals:
+                mc_amd.set_tuning(key, v)
+                print(f"{key}={v}", flush=True)
+                check(f"{key}={v}")
+            mc_amd.set_tuning(key, default)
+        print("autotune encode", flush=True)
+        rep = gpu_ctx.autotune_encode(d_in, qs, taus, d_out, offs, caps)
+        assert len(rep) >= 2 and all(r["ms"] > 0 for r in rep)
+        check("autotuned")
+    finally:
+        mc_amd.set_tuning("tuned_clear", 1)
+        mc_amd.set_tuning("enc_order", 1)
+        mc_amd.set_tuning("enc_batch", 0)
+        mc_amd.set_tuning("enc_slices", 0)
+        d_in.free()
+        d_out.free()
