@@ -499,7 +499,7 @@ def mixed_leg(nthreads):
         return bad
 
     tune = ctx.autotune_decode(d_enc, eoffs, elens, qs, d_out)
-    best = min(tune, key=lambda r: r["ms"])
+    best = next((r for r in tune if r.get("chosen")), min(tune, key=lambda r: r["ms"]))
     log(f"[bench] mixed autotune: {tune}")
     ts = []
     for _ in range(3):
@@ -524,7 +524,7 @@ def mixed_leg(nthreads):
            "encode_gbs": round(n * mc_amd.CHUNK_BYTES / enc_s / 1e9, 3), "encode_byte_identical": ok_enc,
            "encode_graph_gbs": (round(n * mc_amd.CHUNK_BYTES / enc_graph_s / 1e9, 3) if enc_graph_s else None),
            "encode_graph_byte_identical": ok_graph_enc, "encode_autotune": enc_tune,
-           "encode_chosen": min(enc_tune, key=lambda r: r["ms"]),
+           "encode_chosen": next((r for r in enc_tune if r.get("chosen")), min(enc_tune, key=lambda r: r["ms"])),
            "decode_gbs": round(n * mc_amd.CHUNK_BYTES / min(ts) / 1e9, 3),
            "decode_graph_gbs": round(n * mc_amd.CHUNK_BYTES / min(tg) / 1e9, 3),
            "bit_exact": bad_eager == 0 and bad_graph == 0, "autotune": tune, "chosen": best,

@@ -984,8 +984,7 @@ int mchip_autotune_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    std::string rep = "[";
-    int nrep = 0;
+    std::vector<std::string> entries;          // one JSON object per candidate timed
     auto time_of = [&](const Tuning &tn, float *out_ms) -> int {
         float t = 1e30f;
         for (int rep_i = 0; rep_i < 2; ++rep_i) {
@@ -1002,16 +1001,17 @@ int mchip_autotune_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
         }
         char buf[256];
         snprintf(buf, sizeof buf,
-                 "%s{\"decoder\": %d, \"persistent\": %d, \"persist_win\": %d, \"persist_wg\": %d, "
-                 "\"recon_bpw\": %d, \"len_shift\": %d, \"group_shift\": %d, \"slices\": %d, \"ms\": %.3f}",
-                 nrep++ ? ", " : "", tn.decoder, tn.persistent, tn.persist_win, tn.persist_wg, tn.recon_bpw,
-                 tn.len_shift, tn.group_shift, tn.slices, t);
-        rep += buf;
+                 "{\"decoder\": %d, \"persistent\": %d, \"persist_win\": %d, \"persist_wg\": %d, "
+                 "\"recon_bpw\": %d, \"len_shift\": %d, \"group_shift\": %d, \"slices\": %d, \"ms\": %.3f",
+                 tn.decoder, tn.persistent, tn.persist_win, tn.persist_wg, tn.recon_bpw, tn.len_shift,
+                 tn.group_shift, tn.slices, t);
+        entries.push_back(buf);
         *out_ms = t;
         return 0;
     };
     float best_ms;
     if (time_of(best, &best_ms)) return -1;
+    size_t best_at = 0;
     enum Knob { kDecoder, kPersistent, kPersistWin, kPersistWg, kReconBpw, kLenShift, kGroupShift, kSlices };
     struct Step { Knob k; std::vector<int> vals; };
     // kPersistWg candidates are quarters of the resident grid of the window size in use
@@ -1025,41 +1025,49 @@ int mchip_autotune_decode(mchip_ctx *c, const uint8_t *d_blobs, const uint64_t *
         {kGroupShift, {5, 6, 7}},
         {kSlices, {1, 2, 4, 8}},
     };
-    for (const Step &st : steps) {
-        for (int v : st.vals) {
-            Tuning tn = best;
-            switch (st.k) {
-            case kDecoder: if (tn.decoder == v) continue; tn.decoder = v; break;
-            case kPersistent: if (tn.persistent == v) continue; tn.persistent = v; break;
-            case kPersistWin:
-                if (!tn.persistent || tn.decoder < 2 || n < 2 || tn.persist_win == v) continue;
-                tn.persist_win = v;
-                tn.persist_wg = 0;
-                break;
-            case kPersistWg: {
-                if (!tn.persistent || tn.decoder < 2 || n < 2) continue;
-                const int full = persistent_grid(tn.decoder, tn.persist_win);
-                const int wg = v == 4 ? 0 : full * v / 4;
-                if (tn.persist_wg == wg) continue;
-                tn.persist_wg = wg;
-                break;
+    // passes over the knobs until one changes nothing (at most three)
+    for (int pass = 0; pass < 3; ++pass) {
+        bool changed = false;
+        for (const Step &st : steps) {
+            for (int v : st.vals) {
+                Tuning tn = best;
+                switch (st.k) {
+                case kDecoder: if (tn.decoder == v) continue; tn.decoder = v; break;
+                case kPersistent: if (tn.persistent == v) continue; tn.persistent = v; break;
+                case kPersistWin:
+                    if (!tn.persistent || tn.decoder < 2 || n < 2 || tn.persist_win == v) continue;
+                    tn.persist_win = v;
+                    tn.persist_wg = 0;
+                    break;
+                case kPersistWg: {
+                    if (!tn.persistent || tn.decoder < 2 || n < 2) continue;
+                    const int full = persistent_grid(tn.decoder, tn.persist_win);
+                    const int wg = v == 4 ? 0 : full * v / 4;
+                    if (tn.persist_wg == wg) continue;
+                    tn.persist_wg = wg;
+                    break;
+                }
+                case kReconBpw: if (tn.recon_bpw == v) continue; tn.recon_bpw = v; break;
+                case kLenShift: if (tn.len_shift == v) continue; tn.len_shift = v; break;
+                case kGroupShift:
+                    if (!tn.persistent || tn.decoder < 2 || n < 2 || tn.group_shift == v) continue;
+                    tn.group_shift = v;
+                    break;
+                case kSlices:
+                    if (!tn.persistent || tn.decoder < 2 || n <= kInlineDesc || v > n || tn.slices == v) continue;
+                    tn.slices = v;
+                    break;
+                }
+                float ms;
+                if (time_of(tn, &ms)) return -1;
+                if (ms < best_ms * 0.99f) { best_ms = ms; best = tn; best_at = entries.size() - 1; changed = true; }
             }
-            case kReconBpw: if (tn.recon_bpw == v) continue; tn.recon_bpw = v; break;
-            case kLenShift: if (tn.len_shift == v) continue; tn.len_shift = v; break;
-            case kGroupShift:
-                if (!tn.persistent || tn.decoder < 2 || n < 2 || tn.group_shift == v) continue;
-                tn.group_shift = v;
-                break;
-            case kSlices:
-                if (!tn.persistent || tn.decoder < 2 || n <= kInlineDesc || v > n || tn.slices == v) continue;
-                tn.slices = v;
-                break;
-            }
-            float ms;
-            if (time_of(tn, &ms)) return -1;
-            if (ms < best_ms * 0.99f) { best_ms = ms; best = tn; }
         }
+        if (!changed) break;
     }
+    std::string rep = "[";
+    for (size_t i = 0; i < entries.size(); ++i)
+        rep += (i ? ", " : "") + entries[i] + (i == best_at ? ", \"chosen\": true}" : "}");
     rep += "]";
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1580,8 +1588,7 @@ int mchip_autotune_encode(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride,
     // signature (chunk count, mixed qualities, corrections).
     Tuning best = tuning_for_encode(n, q, tau);
     std::vector<uint64_t> lens(n);
-    std::string rep = "[";
-    int nrep = 0;
+    std::vector<std::string> entries;          // one JSON object per candidate timed
     auto time_of = [&](const Tuning &tn, float *out_ms) -> int {
         double t = 1e30;
         for (int rep_i = 0; rep_i < 2; ++rep_i) {
@@ -1593,14 +1600,15 @@ int mchip_autotune_encode(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride,
             if (ms < t) t = ms;
         }
         char buf[192];
-        snprintf(buf, sizeof buf, "%s{\"enc_batch\": %d, \"enc_order\": %d, \"enc_slices\": %d, \"ms\": %.3f}",
-                 nrep++ ? ", " : "", enc_batch_for(tn), tn.enc_order, tn.enc_slices > 0 ? tn.enc_slices : 1, t);
-        rep += buf;
+        snprintf(buf, sizeof buf, "{\"enc_batch\": %d, \"enc_order\": %d, \"enc_slices\": %d, \"ms\": %.3f",
+                 enc_batch_for(tn), tn.enc_order, tn.enc_slices > 0 ? tn.enc_slices : 1, t);
+        entries.push_back(buf);
         *out_ms = (float)t;
         return 0;
     };
     float best_ms;
     if (time_of(best, &best_ms)) return -1;
+    size_t best_at = 0;
     const int full = enc_batch_for(best);
     enum Knob { kOrder, kBatch, kSlices };
     struct Step { Knob k; std::vector<int> vals; };
@@ -1609,25 +1617,33 @@ int mchip_autotune_encode(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride,
         {kBatch, {0, std::max(1, full / 2), std::max(1, full / 4)}},
         {kSlices, {1, 2}},
     };
-    for (const Step &st : steps) {
-        for (int v : st.vals) {
-            Tuning tn = best;
-            switch (st.k) {
-            case kOrder: if (tn.enc_order == v) continue; tn.enc_order = v; break;
-            case kBatch:
-                if (tn.enc_batch == v || (v > 0 && v >= std::min(n, full))) continue;
-                tn.enc_batch = v;
-                break;
-            case kSlices:
-                if ((tn.enc_slices > 0 ? tn.enc_slices : 1) == v || v > std::min(n, enc_batch_for(tn))) continue;
-                tn.enc_slices = v;
-                break;
+    // passes over the knobs until one changes nothing (at most three)
+    for (int pass = 0; pass < 3; ++pass) {
+        bool changed = false;
+        for (const Step &st : steps) {
+            for (int v : st.vals) {
+                Tuning tn = best;
+                switch (st.k) {
+                case kOrder: if (tn.enc_order == v) continue; tn.enc_order = v; break;
+                case kBatch:
+                    if (tn.enc_batch == v || (v > 0 && v >= std::min(n, full))) continue;
+                    tn.enc_batch = v;
+                    break;
+                case kSlices:
+                    if ((tn.enc_slices > 0 ? tn.enc_slices : 1) == v || v > std::min(n, enc_batch_for(tn))) continue;
+                    tn.enc_slices = v;
+                    break;
+                }
+                float ms;
+                if (time_of(tn, &ms)) return -1;
+                if (ms < best_ms * 0.99f) { best_ms = ms; best = tn; best_at = entries.size() - 1; changed = true; }
             }
-            float ms;
-            if (time_of(tn, &ms)) return -1;
-            if (ms < best_ms * 0.99f) { best_ms = ms; best = tn; }
         }
+        if (!changed) break;
     }
+    std::string rep = "[";
+    for (size_t i = 0; i < entries.size(); ++i)
+        rep += (i ? ", " : "") + entries[i] + (i == best_at ? ", \"chosen\": true}" : "}");
     rep += "]";
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
