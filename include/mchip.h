@@ -61,7 +61,10 @@ int         mchip_set_priors(const uint16_t *plo, const uint16_t *phi);
  * two streams (0 = one slice; more slices measured slower on the mixed
  * workload, as each slice's entropy pass then ends in its own tail),
  * "enc_order" the entropy pass's chunk order (1 = heaviest chunk first, the
- * default; 0 = sequence order), and
+ * default; 0 = sequence order), "enc_lanes" its block order within a chunk
+ * (0 = raster, the default; 1 = costliest blocks first, so that each wave's
+ * 64 lanes code blocks of like cost: faster on mixed-quality chunks, slower
+ * where every wave is resident at once), and
  * "enc_keep_mib" the encode workspace a context keeps after a call;
  * "merge" (default 1), "merge_inflight" and "merge_min" shape the merging of
  * concurrent one-chunk host decodes.  Results

@@ -95,6 +95,7 @@ struct Tuning {
     int recon_bpw = 1;        // k_recon blocks per workgroup (1, 2, 4)
     int enc_slices = 0;       // encode sequence: chunk slices on two streams (0: one slice)
     int enc_order = 1;        // encode entropy pass: 1 heaviest chunk first, 0 in sequence order
+    int enc_lanes = 0;        // encode entropy pass: 1 blocks of each chunk costliest first, 0 raster order
     int slices = 0;           // batched persistent decode: chunk slices whose reconstruction overlaps the
                               // next slice's entropy pass on a second stream (0: 2 from 16 chunks, else 1)
     int enc_keep_mib = 2048;  // encode workspace a context keeps after a call (MiB); more is freed
@@ -171,6 +172,7 @@ Tuning tuning_for_encode(int n, const float *q, const int *tau) {
             t.enc_batch = it->second.enc_batch;
             t.enc_slices = it->second.enc_slices;
             t.enc_order = it->second.enc_order;
+            t.enc_lanes = it->second.enc_lanes;
         }
     }
     return t;
@@ -387,6 +389,7 @@ struct mchip_ctx {
     void *d_dst = nullptr; size_t cap_dst = 0;
     void *d_res = nullptr; size_t cap_res = 0;
     void *d_fmap = nullptr; size_t cap_fmap = 0;
+    void *d_lanes = nullptr; size_t cap_lanes = 0;
     void *h_res = nullptr; size_t cap_h_res = 0;
     // block jobs / crops
     void *d_jobs = nullptr; size_t cap_jobs = 0;
@@ -723,6 +726,11 @@ int mchip_set_tuning(const char *key, int value) {
         g_tune.enc_order = value;
         return 0;
     }
+    if (!strcmp(key, "enc_lanes")) {
+        if (value != 0 && value != 1) return fail("enc_lanes must be 0 or 1");
+        g_tune.enc_lanes = value;
+        return 0;
+    }
     if (!strcmp(key, "enc_keep_mib")) {
         if (value < 0) return fail("enc_keep_mib must be >= 0");
         g_tune.enc_keep_mib = value;
@@ -760,7 +768,7 @@ void mchip_ctx_destroy(mchip_ctx *c) {
     collect_timing(c);
     void *bufs[] = {c->d_cd, c->d_rec, c->d_plist, c->d_cinfo, c->d_binfo, c->d_scratch,
                     c->d_stage_in, c->d_stage_out, c->d_aux, c->d_ecd, c->d_einfo, c->d_levels,
-                    c->d_corr, c->d_slots, c->d_plen, c->d_dst, c->d_res, c->d_fmap,
+                    c->d_corr, c->d_slots, c->d_plen, c->d_dst, c->d_res, c->d_fmap, c->d_lanes,
                     c->d_jobs, c->d_jscratch, c->d_jinfo, c->d_bmap, c->d_arena, c->d_order};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
@@ -1440,7 +1448,8 @@ static int prepare_encode(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride,
         ensure(c, &c->d_plen, &c->cap_plen, sizeof(uint32_t) * nb) ||
         ensure(c, &c->d_dst, &c->cap_dst, sizeof(uint32_t) * nb) ||
         ensure(c, &c->d_res, &c->cap_res, sizeof(EncChunkResult) * m) ||
-        ensure(c, &c->d_fmap, &c->cap_fmap, (size_t)kFmapStride * m))
+        ensure(c, &c->d_fmap, &c->cap_fmap, (size_t)kFmapStride * m) ||
+        (tn.enc_lanes && ensure(c, &c->d_lanes, &c->cap_lanes, sizeof(uint16_t) * nb)))
         return -1;
     HIPCHK(hipMemcpyAsync(c->d_ecd, cd, order_at + sizeof(uint32_t) * (size_t)m, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipEventRecord(c->ev_desc, c->stream));
@@ -1458,6 +1467,7 @@ static int prepare_encode(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride,
     a.out = d_out;
     a.res = (EncChunkResult *)c->d_res;
     a.fmap = (uint32_t *)c->d_fmap;
+    a.lanes = tn.enc_lanes ? (uint16_t *)c->d_lanes : nullptr;
     a.n = m;
     a.slices = ns;
     return 0;
@@ -1481,6 +1491,7 @@ static EncodeArgs enc_slice_of(const EncodeArgs &a, int i, int ns) {
     s.fmap = a.fmap + (size_t)c0 * (kFmapStride / 4);
     s.res = a.res + c0;
     s.order = a.order + c0;
+    s.lanes = a.lanes ? a.lanes + b0 : nullptr;
     s.slices = 1;
     return s;
 }
@@ -1582,7 +1593,8 @@ int mchip_autotune_encode(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride,
     if (!c || n <= 0 || !d_in || !q || !d_out || !out_off || !out_cap || in_stride % 16 || ((uintptr_t)d_in) % 16)
         return fail("mchip_autotune_encode: bad argument");
     // Coordinate descent over the encode shapes (chunks per launch sequence,
-    // entropy-pass chunk order, chunk slices), each candidate timed (best of
+    // entropy-pass chunk order and block order within a chunk, chunk
+    // slices), each candidate timed (best of
     // two whole encodes of the caller's chunks, which rewrite the same
     // blobs) with a private tuning; the winner is cached for this encode's
     // signature (chunk count, mixed qualities, corrections).
@@ -1600,8 +1612,9 @@ int mchip_autotune_encode(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride,
             if (ms < t) t = ms;
         }
         char buf[192];
-        snprintf(buf, sizeof buf, "{\"enc_batch\": %d, \"enc_order\": %d, \"enc_slices\": %d, \"ms\": %.3f",
-                 enc_batch_for(tn), tn.enc_order, tn.enc_slices > 0 ? tn.enc_slices : 1, t);
+        snprintf(buf, sizeof buf,
+                 "{\"enc_batch\": %d, \"enc_order\": %d, \"enc_lanes\": %d, \"enc_slices\": %d, \"ms\": %.3f",
+                 enc_batch_for(tn), tn.enc_order, tn.enc_lanes, tn.enc_slices > 0 ? tn.enc_slices : 1, t);
         entries.push_back(buf);
         *out_ms = (float)t;
         return 0;
@@ -1610,10 +1623,11 @@ int mchip_autotune_encode(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride,
     if (time_of(best, &best_ms)) return -1;
     size_t best_at = 0;
     const int full = enc_batch_for(best);
-    enum Knob { kOrder, kBatch, kSlices };
+    enum Knob { kOrder, kLanes, kBatch, kSlices };
     struct Step { Knob k; std::vector<int> vals; };
     const std::vector<Step> steps = {
         {kOrder, {1, 0}},
+        {kLanes, {0, 1}},
         {kBatch, {0, std::max(1, full / 2), std::max(1, full / 4)}},
         {kSlices, {1, 2}},
     };
@@ -1625,6 +1639,7 @@ int mchip_autotune_encode(mchip_ctx *c, const uint8_t *d_in, uint64_t in_stride,
                 Tuning tn = best;
                 switch (st.k) {
                 case kOrder: if (tn.enc_order == v) continue; tn.enc_order = v; break;
+                case kLanes: if (tn.enc_lanes == v) continue; tn.enc_lanes = v; break;
                 case kBatch:
                     if (tn.enc_batch == v || (v > 0 && v >= std::min(n, full))) continue;
                     tn.enc_batch = v;

@@ -276,6 +276,47 @@ __device__ void encode_fracmap_wg(const uint4 *__restrict__ ei, uint32_t *__rest
     fm[0] = e.n < (uint32_t)kFmapCap ? e.n : (uint32_t)kFmapCap;
 }
 
+// k_enc_lane_order: one workgroup per chunk.  The entropy pass codes a
+// block per lane, so a wave takes as long as its costliest lane and runs every
+// branch any of its lanes takes.  Waves of blocks of like cost waste less: the
+// chunk's blocks are counting-sorted by an estimate of their coding work,
+// eob + 2 * nonzero levels + 4 * corrections (one scan position per eob, a
+// magnitude and a sign per nonzero level, three Exp-Golomb codes per
+// correction), costliest first, empty blocks last.  The order within a cost
+// bucket is whatever the LDS atomics give; each block's stream goes to its own
+// slot, so the blobs do not depend on it.
+constexpr int kLaneBuckets = 1024;
+__global__ __launch_bounds__(256) void k_enc_lane_order(const uint4 *__restrict__ einfo,
+                                                        uint16_t *__restrict__ lanes) {
+    __shared__ uint32_t cnt[kLaneBuckets];
+    __shared__ uint32_t lds4[4];
+    const uint4 *ei = einfo + (size_t)blockIdx.x * kNVox;
+    uint16_t *lo = lanes + (size_t)blockIdx.x * kNVox;
+    for (int i = threadIdx.x; i < kLaneBuckets; i += 256) cnt[i] = 0;
+    __syncthreads();
+    constexpr int kPer = kNVox / 256;
+    uint32_t bk[kPer], at[kPer];
+    #pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const uint4 inf = ei[threadIdx.x + 256 * k];
+        const uint32_t cost = min(inf.z + 2u * inf.y + 4u * inf.w, (uint32_t)(kLaneBuckets - 2) * 16u);
+        bk[k] = (inf.x >> 12) & 1u ? (uint32_t)(kLaneBuckets - 2) - (cost >> 4) : (uint32_t)(kLaneBuckets - 1);
+        at[k] = atomicAdd(&cnt[bk[k]], 1u);
+    }
+    __syncthreads();
+    // exclusive scan of the bucket counts, four buckets per thread
+    uint32_t c4[4], s4 = 0;
+    #pragma unroll
+    for (int j = 0; j < 4; ++j) { c4[j] = cnt[threadIdx.x * 4 + j]; s4 += c4[j]; }
+    uint32_t total;
+    uint32_t base = wg_excl_scan(s4, lds4, total);
+    #pragma unroll
+    for (int j = 0; j < 4; ++j) { cnt[threadIdx.x * 4 + j] = base; base += c4[j]; }
+    __syncthreads();
+    #pragma unroll
+    for (int k = 0; k < kPer; ++k) lo[cnt[bk[k]] + at[k]] = (uint16_t)(threadIdx.x + 256 * k);
+}
+
 template <bool kBlocks>
 __global__ __launch_bounds__(64) void k_enc_entropy(const EncChunkDesc *__restrict__ cd,
                                                     const QTab *__restrict__ qtabs,
@@ -284,7 +325,8 @@ __global__ __launch_bounds__(64) void k_enc_entropy(const EncChunkDesc *__restri
                                                     const uint32_t *__restrict__ corr,
                                                     uint32_t *__restrict__ slots,
                                                     uint32_t *__restrict__ plen, uint32_t *__restrict__ fmapbuf,
-                                                    int nblocks, const uint32_t *__restrict__ order) {
+                                                    int nblocks, const uint32_t *__restrict__ order,
+                                                    const uint16_t *__restrict__ lanes) {
     // chunk of this row of workgroups (rows are dispatched in order)
     const uint32_t cy = kBlocks || !order ? blockIdx.y : order[blockIdx.y];
     __shared__ uint32_t stage[kEncStage * 64];
@@ -295,7 +337,8 @@ __global__ __launch_bounds__(64) void k_enc_entropy(const EncChunkDesc *__restri
         encode_fracmap_wg(einfo + (size_t)cy * kNVox, fmapbuf + (size_t)cy * (kFmapStride / 4), stage);
         return;
     }
-    const int bi = (kBlocks ? blockIdx.x : blockIdx.x - 1) * 64 + threadIdx.x;
+    int bi = (kBlocks ? blockIdx.x : blockIdx.x - 1) * 64 + threadIdx.x;
+    if (!kBlocks && lanes) bi = lanes[(size_t)cy * kNVox + bi];
     const QTab *qt;
     if (!kBlocks) {   // one chunk per workgroup: its priors and band starts into LDS
         const uint32_t *src = (const uint32_t *)(qtabs + cd[cy].qtab);
@@ -412,15 +455,16 @@ void launch_enc_transform(const EncodeArgs &a, hipStream_t s) {
                        a.einfo, a.levels, a.corr, a.n);
 }
 void launch_enc_entropy(const EncodeArgs &a, hipStream_t s) {
+    if (a.lanes) hipLaunchKernelGGL(k_enc_lane_order, dim3((unsigned)a.n), dim3(256), 0, s, a.einfo, a.lanes);
     hipLaunchKernelGGL(k_enc_entropy<false>, dim3(kNVox / 64 + 1, a.n), dim3(64), 0, s, a.cd, a.qtabs, a.einfo,
-                       a.levels, a.corr, a.slots, a.plen, a.fmap, 0, a.order);
+                       a.levels, a.corr, a.slots, a.plen, a.fmap, 0, a.order, a.lanes);
 }
 // independent blocks: a.n blocks, descriptor per block
 void launch_enc_blocks(const EncodeArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_enc_transform<true>, dim3((unsigned)a.n), dim3(256), 0, s, a.in, a.cd, a.qtabs, a.einfo,
                        a.levels, a.corr, a.n);
     hipLaunchKernelGGL(k_enc_entropy<true>, dim3(((unsigned)a.n + 63) / 64), dim3(64), 0, s, a.cd, a.qtabs,
-                       a.einfo, a.levels, a.corr, a.slots, a.plen, nullptr, a.n, nullptr);
+                       a.einfo, a.levels, a.corr, a.slots, a.plen, nullptr, a.n, nullptr, nullptr);
 }
 void launch_enc_assemble(const EncodeArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_enc_assemble, dim3(a.n), dim3(256), 0, s, a.cd, a.einfo, a.plen, a.fmap, a.dst, a.out,

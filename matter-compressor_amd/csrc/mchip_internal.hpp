@@ -108,6 +108,8 @@ struct EncodeArgs {
     EncChunkResult *res;    // [n]
     int n;
     const uint32_t *order;  // [n] chunk order of the entropy pass (null: 0..n-1)
+    uint16_t *lanes;        // [n*4096] block order of the entropy pass within each chunk, costliest
+                            // first (written by k_enc_lane_order; null: raster order)
     int slices;             // chunk slices of the sequence on two streams (1: none)
 };
 

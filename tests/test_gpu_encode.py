@@ -97,7 +97,8 @@ def test_sliced_encode_on_two_streams_equals_reference(gpu_ctx):
     """An encode sequence split into chunk slices (alternate slices on two
     streams, the entropy pass heaviest chunk first within each slice) writes
     the reference's blobs (manifest) for every slice count, eagerly and as HIP
-    graph replays with eager work and a scrambled output between them."""
+    graph replays (blocks costliest first) with eager work and a scrambled
+    output between them."""
     import mc_amd
     man = json.load(open(os.path.join(ROOT, "tests", "golden", "chunk_manifest.json")))
     specs = [man["plain64_q6"][3]] + [man["mixed256"][k] for k in (1, 4, 6, 11, 20)]
@@ -129,7 +130,8 @@ def test_sliced_encode_on_two_streams_equals_reference(gpu_ctx):
             assert (st == 0).all()
             check(f"enc_slices={ns}", lens)
         mc_amd.set_tuning("enc_slices", 3)
-        print("encode graph, 3 slices", flush=True)
+        mc_amd.set_tuning("enc_lanes", 1)
+        print("encode graph, 3 slices, blocks costliest first", flush=True)
         g, gn = gpu_ctx.encode_graph(d_in, qs, taus, d_out, offs, caps)
         try:
             for rep in range(3):
@@ -144,6 +146,7 @@ def test_sliced_encode_on_two_streams_equals_reference(gpu_ctx):
             gpu_ctx.graph_destroy(g)
     finally:
         mc_amd.set_tuning("enc_slices", 0)
+        mc_amd.set_tuning("enc_lanes", 0)
         d_in.free()
         d_out.free()
         d_tmp.free()
@@ -154,7 +157,8 @@ def test_encode_tiny_blocks_and_unaligned_outputs_match_oracle(gpu_ctx):
     copy), from chunks whose present blocks have payloads of a few bytes (a
     64-byte piece of the payload stream then spans many blocks), from a
     noise chunk (near-full slots) and from a mixed one, equal the oracle's
-    byte for byte."""
+    byte for byte, with the entropy pass's blocks in raster order and
+    costliest first (most blocks of the sparse chunks are empty)."""
     import mc_amd
     rng = np.random.default_rng(31)
     ch = np.zeros((5, 256, 256, 256), np.uint8)
@@ -172,23 +176,27 @@ def test_encode_tiny_blocks_and_unaligned_outputs_match_oracle(gpu_ctx):
     gpu_ctx.h2d(d_in, ch)
     offs = np.array([k * cap + (k % 4) for k in range(n)], np.uint64)
     caps = np.full(n, cap - 8, np.uint64)
+    wants = [ob.encode_chunk(ch[k], qs[k], taus[k]) for k in range(n)]
     try:
-        print("encoding 5 chunks at odd output offsets", flush=True)
-        lens, st = gpu_ctx.encode_chunks(d_in, qs, taus, d_out, offs, caps)
-        assert (st == 0).all(), st
-        for k in range(n):
-            want = ob.encode_chunk(ch[k], qs[k], taus[k])
-            b = np.empty(int(lens[k]), np.uint8)
-            gpu_ctx.d2h(b, d_out, offset=int(offs[k]))
-            assert b.tobytes() == want, f"chunk {k}: {len(b)} vs {len(want)} bytes"
+        for lanes in (0, 1):
+            mc_amd.set_tuning("enc_lanes", lanes)
+            print(f"encoding 5 chunks at odd output offsets, enc_lanes={lanes}", flush=True)
+            gpu_ctx.h2d(d_out, np.full(n * cap + 64, 0x5A, np.uint8))
+            lens, st = gpu_ctx.encode_chunks(d_in, qs, taus, d_out, offs, caps)
+            assert (st == 0).all(), st
+            for k in range(n):
+                b = np.empty(int(lens[k]), np.uint8)
+                gpu_ctx.d2h(b, d_out, offset=int(offs[k]))
+                assert b.tobytes() == wants[k], f"enc_lanes={lanes} chunk {k}: {len(b)} vs {len(wants[k])} bytes"
     finally:
+        mc_amd.set_tuning("enc_lanes", 0)
         d_in.free()
         d_out.free()
 
 
 def test_encode_autotuner_and_every_encode_shape_keep_blobs_identical(gpu_ctx):
-    """Every encode shape the autotuner searches (entropy-pass chunk order,
-    chunks per launch sequence, chunk slices) writes the reference's blobs
+    """Every encode shape the autotuner searches (entropy-pass chunk order and
+    block order, chunks per launch sequence, chunk slices) writes the reference's blobs
     (manifest); so do the autotuned shapes cached for the chunks' signature."""
     import mc_amd
     man = json.load(open(os.path.join(ROOT, "tests", "golden", "chunk_manifest.json")))
@@ -215,12 +223,19 @@ def test_encode_autotuner_and_every_encode_shape_keep_blobs_identical(gpu_ctx):
             assert f"{ob.xxh64(b):016x}" == s["blob_xxh64"], f"{what}: chunk {k}"
 
     try:
-        for key, vals, default in [("enc_order", [0, 1], 1), ("enc_batch", [2, 3, 0], 0), ("enc_slices", [2, 0], 0)]:
+        for key, vals, default in [("enc_order", [0, 1], 1), ("enc_lanes", [1, 0], 0), ("enc_batch", [2, 3, 0], 0),
+                                   ("enc_slices", [2, 0], 0)]:
             for v in vals:
                 mc_amd.set_tuning(key, v)
                 print(f"{key}={v}", flush=True)
                 check(f"{key}={v}")
             mc_amd.set_tuning(key, default)
+        mc_amd.set_tuning("enc_lanes", 1)
+        mc_amd.set_tuning("enc_slices", 2)
+        print("enc_lanes=1 enc_slices=2", flush=True)
+        check("enc_lanes=1 enc_slices=2")
+        mc_amd.set_tuning("enc_lanes", 0)
+        mc_amd.set_tuning("enc_slices", 0)
         print("autotune encode", flush=True)
         rep = gpu_ctx.autotune_encode(d_in, qs, taus, d_out, offs, caps)
         assert len(rep) >= 2 and all(r["ms"] > 0 for r in rep)
@@ -228,6 +243,7 @@ def test_encode_autotuner_and_every_encode_shape_keep_blobs_identical(gpu_ctx):
     finally:
         mc_amd.set_tuning("tuned_clear", 1)
         mc_amd.set_tuning("enc_order", 1)
+        mc_amd.set_tuning("enc_lanes", 0)
         mc_amd.set_tuning("enc_batch", 0)
         mc_amd.set_tuning("enc_slices", 0)
         d_in.free()
