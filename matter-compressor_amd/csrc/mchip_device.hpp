@@ -204,6 +204,51 @@ __device__ __forceinline__ void idct16_line(float v[16], const float *cm /* [k*1
     for (int n = 0; n < 8; ++n) { v[n] = e[n] + o[n]; v[15 - n] = e[n] - o[n]; }
 }
 
+// The same transform for a wave's lines at once, skipping input k wherever no
+// line of the wave has it nonzero (bit l of wk[k] is set when lane l's input
+// k is nonzero; a uniform branch per input).  The reference skips each zero
+// input of its line (src/mc_codec_float.h:109-126); a lane that runs an input
+// of value zero because another lane needs it adds a signed zero to a sum that
+// is never -0 (it starts at +0, its products are nonzero, and a sum that
+// cancels rounds to +0), which leaves the sum unchanged.  Bit-identical to
+// idct16_line.
+__device__ __forceinline__ void idct16_line_sparse(float v[16], const float *cm, const uint64_t (&wk)[16]) {
+    float e[8], o[8];
+    #pragma unroll
+    for (int n = 0; n < 8; ++n) { e[n] = 0.0f; o[n] = 0.0f; }
+    #pragma unroll
+    for (int k = 0; k < 16; k += 2) {
+        if (wk[k]) {
+            const float c = v[k];
+            #pragma unroll
+            for (int n = 0; n < 8; ++n) { float p = cm[k * 16 + n] * c; e[n] = e[n] + p; }
+        }
+    }
+    #pragma unroll
+    for (int k = 1; k < 16; k += 2) {
+        if (wk[k]) {
+            const float c = v[k];
+            #pragma unroll
+            for (int n = 0; n < 8; ++n) { float p = cm[k * 16 + n] * c; o[n] = o[n] + p; }
+        }
+    }
+    #pragma unroll
+    for (int n = 0; n < 8; ++n) { v[n] = e[n] + o[n]; v[15 - n] = e[n] - o[n]; }
+}
+
+// Per-input wave masks of a line (see idct16_line_sparse); returns whether
+// this lane's line has any nonzero input (either sign of zero counts as zero).
+__device__ __forceinline__ bool line_masks(const float (&v)[16], uint64_t (&wk)[16]) {
+    bool nz = false;
+    #pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const bool b = v[k] != 0.0f;
+        wk[k] = __builtin_amdgcn_ballot_w64(b);
+        nz |= b;
+    }
+    return nz;
+}
+
 // 1-D forward DCT-II of one line, in place (src/mc_codec_float.h:83-95).
 // An all-zero line maps to zeros either way.
 __device__ __forceinline__ void fdct16_line(float v[16], const float *cm) {

@@ -86,5 +86,14 @@ uint32_t dl_encode_corrections(const QTab *qt, uint32_t dc, uint32_t n, const ui
 
 // IDCT of one 16-point line with the device kernel's arithmetic.
 void dl_idct_line(float *v, const float *cmf) { idct16_line(v, cmf); }
+// The wave-sparse inverse with the caller's input masks (bit 0 of wk[k] set:
+// the wave runs input k); returns line_masks' verdict on the line.
+int dl_idct_line_sparse(float *v, const float *cmf, const uint64_t *wk) {
+    uint64_t own[16], m[16];
+    const bool nz = line_masks(*(const float(*)[16])v, own);
+    for (int k = 0; k < 16; ++k) m[k] = wk[k];
+    if (nz) idct16_line_sparse(v, cmf, m);
+    return nz;
+}
 void dl_fdct_line(float *v, const float *cmf) { fdct16_line(v, cmf); }
 }

@@ -21,4 +21,6 @@ static inline uint32_t __umul24(uint32_t a, uint32_t b) {
 }
 static inline int __clz(uint32_t x) { return x ? __builtin_clz(x) : 32; }
 static inline int __popc(uint32_t x) { return __builtin_popcount(x); }
+/* a wave of one lane: bit 0 is the lane's predicate */
+static inline uint64_t __builtin_amdgcn_ballot_w64(bool b) { return b ? 1u : 0u; }
 static inline uint4 make_uint4(uint32_t x, uint32_t y, uint32_t z, uint32_t w) { uint4 r = {x, y, z, w}; return r; }

@@ -39,6 +39,8 @@ def harness():
     H.dl_encode_corrections.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp, vp, vp, C.c_uint32]
     H.dl_encode_corrections.restype = C.c_uint32
     H.dl_idct_line.argtypes = [vp, vp]
+    H.dl_idct_line_sparse.argtypes = [vp, vp, vp]
+    H.dl_idct_line_sparse.restype = C.c_int
     H.dl_fdct_line.argtypes = [vp, vp]
     H.dl_set_variant.argtypes = [C.c_int]
     _h = H
@@ -230,3 +232,46 @@ def test_crafted_correction_payloads_decode_like_the_reference():
         assert np.array_equal(ob.dec_block(pay, q), want)
         if dc == 128:   # voxel 5: 128 +200 -> 255, +100 -> 255, -150 -> 105; voxel 8: 128 -90 +7 = 45
             assert want[5] == 105 and want[8] == 45 and want[0] == 128
+
+
+def test_wave_sparse_inverse_transform_matches_oracle_under_any_wave_mask():
+    """The reconstruction's inverse transform skips an input only when no line
+    of the wave needs it, so a line also runs inputs that are zero for it
+    (another lane's). With the masks of such a wave (every nonzero input of
+    the line, plus random zero inputs), and all-zero lines skipped, the three
+    passes equal the oracle's inverse DCT bit for bit, including sums that
+    cancel to exactly zero before a masked-in zero input is added."""
+    H = harness()
+    cm = np.zeros(256, np.float32)
+    ob.oracle().orc_dct_matrix(ob.p(cm))
+    cmm = cm.reshape(16, 16)
+    # pairs of same-parity inputs whose weights cancel for some output
+    cancel = [(k, j) for n in range(8) for k in range(16) for j in range(k + 1, 16)
+              if (k & 1) == (j & 1) and cmm[k, n] == -cmm[j, n]]
+    assert cancel, "the DCT matrix has no cancelling pair to exercise"
+    rng = np.random.default_rng(11)
+    step = np.float32(3.25)
+    for trial in range(24):
+        lv = rng.integers(-40, 41, 4096).astype(np.float32)
+        lv[rng.random(4096) < (0.97 if trial % 3 else 0.6)] = 0.0
+        blk = (lv * step).astype(np.float32)
+        if trial % 4 == 1:                       # exact cancellations in the x pass
+            c3 = blk.reshape(16, 16, 16)
+            for z, y in zip(rng.integers(0, 16, 40), rng.integers(0, 16, 40)):
+                k, j = cancel[int(rng.integers(len(cancel)))]
+                c3[z, y, :] = 0.0
+                c3[z, y, k] = c3[z, y, j] = step * 7
+        want = np.zeros_like(blk)
+        ob.oracle().orc_dct3_inv(ob.p(blk), ob.p(want))
+        got = blk.reshape(16, 16, 16).copy()
+        extra_p = (0.0, 0.3, 1.0)[trial % 3]
+        for axis in (2, 1, 0):
+            g = np.moveaxis(got, axis, -1).copy()
+            for idx in np.ndindex(16, 16):
+                line = np.ascontiguousarray(g[idx])
+                wk = ((line != 0) | (rng.random(16) < extra_p)).astype(np.uint64)
+                if not H.dl_idct_line_sparse(ob.p(line), ob.p(cm), ob.p(wk)):
+                    assert not line.any()
+                g[idx] = line
+            got = np.moveaxis(g, -1, axis)
+        assert np.array_equal(got.ravel().view(np.uint32), want.view(np.uint32)), f"trial {trial}"
