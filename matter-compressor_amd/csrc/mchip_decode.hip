@@ -356,12 +356,15 @@ __device__ void recon_single_lane(const uint8_t *payload, uint32_t len, const QT
     if (s_nsig == 0) {
         ((uint4 *)u8t)[t] = make_uint4(dc * 0x01010101u, dc * 0x01010101u, dc * 0x01010101u, dc * 0x01010101u);
     } else {
+        // as in recon_from_list: zero lines stay zero, and inputs that no
+        // line of the wave has nonzero are skipped
         float v[16];
+        uint64_t wk[16];
         {
             float *ln = tile + t * kPad;
             #pragma unroll
             for (int k = 0; k < 16; ++k) v[k] = ln[k];
-            idct16_line(v, cmf);
+            if (line_masks(v, wk)) idct16_line_sparse(v, cmf, wk);
             #pragma unroll
             for (int k = 0; k < 16; ++k) ln[k] = v[k];
         }
@@ -370,7 +373,7 @@ __device__ void recon_single_lane(const uint8_t *payload, uint32_t len, const QT
             float *ln = tile + (t >> 4) * 16 * kPad + (t & 15);
             #pragma unroll
             for (int k = 0; k < 16; ++k) v[k] = ln[k * kPad];
-            idct16_line(v, cmf);
+            if (line_masks(v, wk)) idct16_line_sparse(v, cmf, wk);
             #pragma unroll
             for (int k = 0; k < 16; ++k) ln[k * kPad] = v[k];
         }
@@ -379,7 +382,7 @@ __device__ void recon_single_lane(const uint8_t *payload, uint32_t len, const QT
             float *ln = tile + (t >> 4) * kPad + (t & 15);
             #pragma unroll
             for (int k = 0; k < 16; ++k) v[k] = ln[k * 16 * kPad];
-            idct16_line(v, cmf);
+            if (line_masks(v, wk)) idct16_line_sparse(v, cmf, wk);
             #pragma unroll
             for (int k = 0; k < 16; ++k) u8t[k * 256 + t] = recon_voxel(v[k], dc);
         }

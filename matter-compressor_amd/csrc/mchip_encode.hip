@@ -183,7 +183,9 @@ __global__ __launch_bounds__(256) void k_enc_transform(const uint8_t *__restrict
     }
     uint32_t ncorr = 0;
     if (tau > 0) {
-        // local reconstruction exactly as the decoder computes it
+        // local reconstruction exactly as the decoder computes it (an
+        // all-zero line transforms to zeros and is left as it is)
+        uint64_t wk[16];
         {
             const int y = t >> 4, x = t & 15;
             float *ln = tile + y * kPadE + x;
@@ -195,7 +197,7 @@ __global__ __launch_bounds__(256) void k_enc_transform(const uint8_t *__restrict
             float *ln = tile + t * kPadE;
             #pragma unroll
             for (int k = 0; k < 16; ++k) v[k] = ln[k];
-            idct16_line(v, cmf);
+            if (line_masks(v, wk)) idct16_line_sparse(v, cmf, wk);
             #pragma unroll
             for (int k = 0; k < 16; ++k) ln[k] = v[k];
         }
@@ -204,7 +206,7 @@ __global__ __launch_bounds__(256) void k_enc_transform(const uint8_t *__restrict
             float *ln = tile + (t >> 4) * 16 * kPadE + (t & 15);
             #pragma unroll
             for (int k = 0; k < 16; ++k) v[k] = ln[k * kPadE];
-            idct16_line(v, cmf);
+            if (line_masks(v, wk)) idct16_line_sparse(v, cmf, wk);
             #pragma unroll
             for (int k = 0; k < 16; ++k) ln[k * kPadE] = v[k];
         }
@@ -213,7 +215,7 @@ __global__ __launch_bounds__(256) void k_enc_transform(const uint8_t *__restrict
             float *ln = tile + (t >> 4) * kPadE + (t & 15);
             #pragma unroll
             for (int k = 0; k < 16; ++k) v[k] = ln[k * 16 * kPadE];
-            idct16_line(v, cmf);
+            if (line_masks(v, wk)) idct16_line_sparse(v, cmf, wk);
             #pragma unroll
             for (int k = 0; k < 16; ++k) {
                 int rv = (int)((uint32_t)lrint_low32(v[k]) + (uint32_t)dc);
