@@ -1014,8 +1014,9 @@ def main():
     ap.add_argument("--batched", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--streams", type=int, default=QUEUES - 1, help="HIP streams the per-chunk launches rotate over")
     ap.add_argument("--tune", action="append", default=[], help="key=value launch tuning (mchip_set_tuning)")
-    ap.add_argument("--timed-streams", type=int, default=4,
-                    help="streams whose kernels carry HIP timing events during the timed steps")
+    ap.add_argument("--timed-streams", type=int, default=1,
+                    help="streams whose kernels carry HIP timing events during the timed steps (each "
+                         "timed launch records two events; on 5 of 21 streams they cost about 7 %% of the rate)")
     ap.add_argument("--serve-batches", default="1000,1000,1000",
                     help="batches served at 20, 50 and 100 batches/s (serving leg); 0 skips the leg")
     ap.add_argument("--resident-batches", type=int, default=1000, help=argparse.SUPPRESS)
@@ -1234,8 +1235,10 @@ def main():
         step()
     for c_ in ctxs:
         c_.sync()
-    # kernel durations from HIP events on a spread subset of the streams (the
-    # events cost host time per launch; every stream runs the same kernels)
+    # kernel durations from HIP events on a spread subset of the streams (every
+    # stream runs the same kernels; the two event records around each timed
+    # launch slow the launching, so by default one stream carries them, about
+    # 30 launches of each kernel over 10 steps)
     every = max(1, len(ctxs) // max(1, args.timed_streams))
     timed_ctxs = ctxs[::every]
     for c_ in timed_ctxs:
