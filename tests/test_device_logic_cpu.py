@@ -173,6 +173,16 @@ def test_device_encoder_reproduces_reference_payloads(kat):
         t = qtab(q)
         n = H.dl_encode(ob.p(t), ob.p(bs), dc, eob, ob.p(L), len(cp), ob.p(Cw), ob.p(out), out.size)
         assert out[:n].tobytes() == pay, f"case {i} {kat['names'][i]}"
+        # a capacity below the payload: the first cap bytes are stored, nothing
+        # past them, and the full length is still returned
+        for cap in (4, 8, 12, 20, 36, 64, 68):
+            if cap >= len(pay):
+                continue
+            small = np.full(len(pay) + 64, 0xAB, np.uint8)
+            m = H.dl_encode(ob.p(t), ob.p(bs), dc, eob, ob.p(L), len(cp), ob.p(Cw), ob.p(small), cap)
+            assert m == len(pay), f"case {i} cap {cap}: length"
+            assert small[:cap].tobytes() == pay[:cap], f"case {i} cap {cap}: stored prefix"
+            assert (small[cap:] == 0xAB).all(), f"case {i} cap {cap}: bytes past the capacity"
 
 
 def test_device_line_transforms_match_oracle_dct():
