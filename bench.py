@@ -1175,6 +1175,15 @@ def main():
         dtb = (time.perf_counter() - t0) / max(2, args.steps // 2)
         bctx.close()
         extra["batched_gbs"] = round(n * mc_amd.CHUNK_BYTES / dtb / 1e9, 3)
+        # the batched decode's voxels against the reference's decoded hashes
+        # (the per-chunk leg rewrites the same output afterwards)
+        outb = np.empty((256, 256, 256), np.uint8)
+        nbad = 0
+        for i in range(n):
+            ctx.d2h(outb, d_out, offset=i * mc_amd.CHUNK_BYTES)
+            nbad += xxh64_hex(outb) != man[i]["decoded_xxh64"]
+        extra["batched_bit_exact"] = nbad == 0
+        log(f"[bench] batched decode {extra['batched_gbs']} GB/s, bit-exact={nbad == 0}")
     # ---- verify: blob hashes recomputed and checked against the stored ones,
     # for the resident blobs (one wave per blob) and for host blobs (gathered
     # into pinned slabs and uploaded), against the oracle's hash on one core;
