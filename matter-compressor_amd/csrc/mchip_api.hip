@@ -1331,42 +1331,49 @@ int mchip_decode_blocks_host(mchip_ctx *c, const uint8_t *const *payloads, const
                              const float *q, int n, uint8_t *out) {
     if (!c) return fail("null context");
     if (n <= 0) return 0;
+    // Present blocks go through the block-job path (the lane decoder, then
+    // the reconstruction; a payload it cannot list, an empty one included,
+    // falls back to the single-lane decode there); absent blocks are zeros.
+    // Job records and payloads are staged together in the pinned descriptor
+    // buffer: [BlockJob x n][payloads, 16-byte aligned].
     uint64_t tot = 0;
     for (int i = 0; i < n; ++i) if (payloads[i]) tot += ((uint64_t)lens[i] + 15) & ~15ull;
-    size_t meta = (size_t)n * (8 + 4 + 4);
-    const size_t pay_at = (meta + 255) & ~size_t(255);     // payloads gathered after the records
+    const size_t meta = sizeof(BlockJob) * (size_t)n;
+    const size_t pay_at = (meta + 255) & ~size_t(255);
     void *h = nullptr;
     if (desc_buffer(c, pay_at + tot, &h)) return -1;
-    uint64_t *hoff = (uint64_t *)h;
-    uint32_t *hlen = (uint32_t *)(hoff + n);
-    uint32_t *hq = hlen + n;
-    uint64_t at = 0;
+    BlockJob *jb = (BlockJob *)h;
+    uint8_t *hpay = (uint8_t *)h + pay_at;
+    int nj = 0;
+    uint64_t at = 0, sw = 0;
     for (int i = 0; i < n; ++i) {
-        if (!payloads[i]) { hoff[i] = ~0ull; hlen[i] = 0; hq[i] = 0; continue; }
-        if (qtab_slot(q[i], &hq[i])) return -1;
-        hoff[i] = at;
-        hlen[i] = lens[i];
+        if (!payloads[i]) continue;
+        uint32_t qs;
+        if (qtab_slot(q[i], &qs)) return -1;
+        jb[nj++] = BlockJob{at, sw, lens[i], qs, (uint32_t)i, 0};
+        if (lens[i]) memcpy(hpay + at, payloads[i], lens[i]);
         at += ((uint64_t)lens[i] + 15) & ~15ull;
+        sw += 8ull * lens[i] + 32ull;
     }
     if (ensure(c, &c->d_stage_in, &c->cap_stage_in, tot + 256) ||
-        ensure(c, &c->d_aux, &c->cap_aux, meta + 64) ||
-        ensure(c, &c->d_stage_out, &c->cap_stage_out, (size_t)n * kNVox))
+        ensure(c, &c->d_stage_out, &c->cap_stage_out, (size_t)n * kNVox) ||
+        (nj && (ensure(c, &c->d_jobs, &c->cap_jobs, sizeof(BlockJob) * nj) ||
+                ensure(c, &c->d_jscratch, &c->cap_jscratch, sizeof(uint32_t) * sw) ||
+                ensure(c, &c->d_jinfo, &c->cap_jinfo, sizeof(uint2) * nj))))
         return -1;
     uint8_t *din = (uint8_t *)c->d_stage_in;
-    uint8_t *hpay = (uint8_t *)h + pay_at;
-    for (int i = 0; i < n; ++i)
-        if (payloads[i] && lens[i]) memcpy(hpay + hoff[i], payloads[i], lens[i]);
     if (tot) HIPCHK(hipMemcpyAsync(din, hpay, tot, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->d_aux, h, meta, hipMemcpyHostToDevice, c->stream));
+    if (nj) HIPCHK(hipMemcpyAsync(c->d_jobs, jb, sizeof(BlockJob) * nj, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipEventRecord(c->ev_desc, c->stream));
     c->desc_pending = true;
-    const uint64_t *doff = (const uint64_t *)c->d_aux;
-    const uint32_t *dlen = (const uint32_t *)(doff + n);
-    const uint32_t *dq = dlen + n;
-    timed(c, "block_single", [&] {
-        launch_block_single(din, doff, dlen, dq, qtabs_now(), (uint8_t *)c->d_stage_out, n, c->stream);
-    });
-    HIPCHK(hipGetLastError());
+    if (nj < n) HIPCHK(hipMemsetAsync(c->d_stage_out, 0, (size_t)n * kNVox, c->stream));
+    if (nj) {
+        timed(c, "blocks_host", [&] {
+            launch_jobs(din, (const BlockJob *)c->d_jobs, nj, qtabs_now(), (uint32_t *)c->d_jscratch,
+                        (uint2 *)c->d_jinfo, (uint8_t *)c->d_stage_out, tuning().decoder, c->stream);
+        });
+        HIPCHK(hipGetLastError());
+    }
     HIPCHK(hipMemcpyAsync(out, c->d_stage_out, (size_t)n * kNVox, hipMemcpyDeviceToHost, c->stream));
     return mchip_ctx_sync(c);
 }

@@ -710,26 +710,6 @@ void launch_crop(const uint8_t *arena, const int32_t *bmap, const int64_t *origi
                        gx, dz, dy, dx, out, batch_stride);
 }
 
-// n independent payloads: block i at base + off[i] (len[i]; off[i] == ~0 ->
-// absent) with quality table qi[i], written to out + 4096*i.
-__global__ __launch_bounds__(256) void k_block_single(const uint8_t *__restrict__ base,
-                                                      const uint64_t *__restrict__ off,
-                                                      const uint32_t *__restrict__ len,
-                                                      const uint32_t *__restrict__ qi,
-                                                      const QTab *__restrict__ qtabs,
-                                                      uint8_t *__restrict__ out) {
-    __shared__ float tile[256 * kPad];
-    __shared__ __attribute__((aligned(16))) uint8_t u8t[kNVox];
-    const int i = blockIdx.x, t = threadIdx.x;
-    uint4 *dst = (uint4 *)(out + (size_t)i * kNVox) + t;
-    if (off[i] == ~0ull) { *dst = make_uint4(0, 0, 0, 0); return; }
-    recon_single_lane(base + off[i], len[i], qtabs + qi[i], tile, u8t);
-    *dst = ((const uint4 *)u8t)[t];
-}
-void launch_block_single(const uint8_t *base, const uint64_t *off, const uint32_t *len, const uint32_t *qi,
-                         const QTab *qtabs, uint8_t *out, int n, hipStream_t s) {
-    hipLaunchKernelGGL(k_block_single, dim3(n), dim3(256), 0, s, base, off, len, qi, qtabs, out);
-}
 
 // ---------------------------------------------------------------------------
 // Fixed device workspace of the decode entry point that allocates nothing

@@ -206,7 +206,5 @@ void launch_index(const DecodeArgs &a, hipStream_t s);
 hipError_t occupancy_persist(int decoder, int win, int *per_cu);   // resident persistent-entropy workgroups per CU
 void launch_entropy(const DecodeArgs &a, hipStream_t s);
 void launch_recon(const DecodeArgs &a, hipStream_t s);
-void launch_block_single(const uint8_t *base, const uint64_t *off, const uint32_t *len, const uint32_t *qi,
-                         const QTab *qtabs, uint8_t *out, int n, hipStream_t s);
 
 }  // namespace mchip
