@@ -42,6 +42,30 @@ def test_block_decode_corrupt_payloads_match_oracle(gpu_ctx, kat):
         assert np.array_equal(out[k], ob.dec_block(pays[k], qs[k])), f"trial {k}"
 
 
+def test_block_decode_absent_empty_and_present_in_one_call(gpu_ctx, kat):
+    """One call with absent blocks (no payload: zeros) between present ones,
+    an empty payload and a truncated one: every present block equals the
+    oracle's decode of its payload, every absent one is zeros."""
+    idx = [i for i in range(len(kat["q"])) if kat["offs"][i + 1] > kat["offs"][i]][:12]
+    pays, qs = [], []
+    for k, i in enumerate(idx):
+        if k % 3 == 0:
+            pays.append(None)
+            qs.append(6.0)
+        p = kat_payload(kat, i)
+        pays.append(p if k % 4 else p[: len(p) // 2])
+        qs.append(float(kat["q"][i]))
+    pays.insert(5, b"")
+    qs.insert(5, 6.0)
+    pays.append(None)
+    qs.append(1.0)
+    print(f"decoding {len(pays)} blocks, {sum(p is None for p in pays)} absent", flush=True)
+    out = gpu_ctx.decode_blocks_host(pays, np.array(qs, np.float32))
+    for k, p in enumerate(pays):
+        want = np.zeros(4096, np.uint8) if p is None else ob.dec_block(p, qs[k])
+        assert np.array_equal(out[k], want), f"block {k} ({'absent' if p is None else len(p)})"
+
+
 def _chunks(specs):
     import mc_amd
     return mc_amd.synth_chunks([s["seed"] for s in specs], [s["air"] for s in specs])
